@@ -1,0 +1,199 @@
+"""ctypes wrapper of the oracle (liboracle.so) -- TEST INFRASTRUCTURE ONLY.
+
+The oracle is the checker: only tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py use it.  Arrays are numpy, complex values
+complex128.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_libs = {}
+
+
+class C128(C.Structure):
+    _fields_ = [("re", C.c_double), ("im", C.c_double)]
+
+_P = C.c_void_p
+_I = C.c_int
+_D = C.c_double
+_L = C.c_int64
+
+
+def lib(fast: bool = False):
+    """liboracle.so (checker) or liboracle_fast.so (CPU-baseline timing build)."""
+    name = "liboracle_fast.so" if fast else "liboracle.so"
+    if name not in _libs:
+        path = os.path.join(HERE, name)
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} missing: run make -C oracle")
+        L = C.CDLL(path)
+        L.orc_frac_coord.argtypes = [_I, _I, _I, _D, _D, _D, _D, _D, C.POINTER(_I), C.POINTER(_I),
+                                     C.POINTER(_I), C.POINTER(_I)]
+        L.orc_frac_coord.restype = None
+        L.orc_frac_coord_packed.argtypes = [_I, _I, _I, _D, _D, _D, _D, _D, C.POINTER(_I), C.POINTER(_I)]
+        L.orc_frac_coord_packed.restype = None
+        L.orc_w_plane.argtypes = [_D, _D, _D]
+        L.orc_w_plane.restype = _I
+        L.orc_tower_index.argtypes = [_D, _D, _I, _I]
+        L.orc_tower_index.restype = _I
+        L.orc_floor_index.argtypes = [_D, _D]
+        L.orc_floor_index.restype = _I
+        L.orc_grid_wprojection_flat.argtypes = [_P, _I, _D, _P, _P, _P, _P, _L, _P, _I, _I, _I, _D, _D,
+                                                C.POINTER(_L)]
+        L.orc_grid_wprojection_flat.restype = C.c_uint64
+        L.orc_grid_wprojection_flat_omp.argtypes = [_P, _I, _D, _P, _P, _P, _P, _L, _P, _I, _I, _I, _D,
+                                                    _D, _I, C.POINTER(_L)]
+        L.orc_grid_wprojection_flat_omp.restype = C.c_uint64
+        L.orc_grid_wtowers_flat.argtypes = [_P, _I, _D, _P, _P, _P, _P, _L, _P, _I, _I, _I, _D, _D,
+                                            _I, _I, _D, _I, C.POINTER(_L)]
+        L.orc_grid_wtowers_flat.restype = C.c_uint64
+        L.orc_wtransfer.argtypes = [_P, _I, _D, _D]
+        L.orc_wtransfer.restype = None
+        L.orc_cipow.argtypes = [C128, _I]
+        L.orc_cipow.restype = C128
+        L.orc_fft2d.argtypes = [_P, _I, _I]
+        L.orc_fft2d.restype = None
+        L.orc_fft1d.argtypes = [_P, _I, _I]
+        L.orc_fft1d.restype = None
+        L.orc_fft_shift.argtypes = [_P, _I]
+        L.orc_fft_shift.restype = None
+        L.orc_make_hermitian.argtypes = [_P, _I]
+        L.orc_make_hermitian.restype = None
+        L.orc_weight_flat.argtypes = [_P, _I, _D, _P, _P, _P, _L]
+        L.orc_weight_flat.restype = None
+        L.orc_grid_to_image.argtypes = [_P, _I]
+        L.orc_grid_to_image.restype = None
+        L.orc_dft_flat.argtypes = [_P, _P, _P, _L, _I, _D, _P, _P, _P, _P, _L, _I]
+        L.orc_dft_flat.restype = None
+        _libs[name] = L
+    return _libs[name]
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def frac_coord(N, S, R, theta, u, v, du=0.0, dv=0.0):
+    gx, gy, xf, yf = _I(), _I(), _I(), _I()
+    lib().orc_frac_coord(N, S, R, theta, u, v, du, dv, C.byref(gx), C.byref(gy), C.byref(xf),
+                         C.byref(yf))
+    return gx.value, gy.value, xf.value, yf.value
+
+
+def frac_coord_packed(N, S, R, theta, u, v, du=0.0, dv=0.0):
+    go, so = _I(), _I()
+    lib().orc_frac_coord_packed(N, S, R, theta, u, v, du, dv, C.byref(go), C.byref(so))
+    return go.value, so.value
+
+
+def w_plane(w, w_min, w_step):
+    return lib().orc_w_plane(w, w_min, w_step)
+
+
+def tower_index(u, theta, N, chunk):
+    return lib().orc_tower_index(u, theta, N, chunk)
+
+
+def floor_index(w, winc):
+    return lib().orc_floor_index(w, winc)
+
+
+def grid_wprojection(N, theta, u, v, w, vis, kern, w_min, w_step, nthreads=0, fast=False):
+    """Returns (grid [N][N] complex128, flops, n_bad)."""
+    u, v, w = _c(u, np.float64), _c(v, np.float64), _c(w, np.float64)
+    vis = _c(vis, np.complex128)
+    kern = _c(kern, np.complex128)
+    P, R, _, S, _ = kern.shape
+    g = np.zeros((N, N), np.complex128)
+    bad = _L()
+    L = lib(fast)
+    if nthreads and nthreads > 1:
+        fl = L.orc_grid_wprojection_flat_omp(g.ctypes.data, N, theta, u.ctypes.data, v.ctypes.data,
+                                             w.ctypes.data, vis.ctypes.data, len(u), kern.ctypes.data,
+                                             P, R, S, w_min, w_step, nthreads, C.byref(bad))
+    else:
+        fl = L.orc_grid_wprojection_flat(g.ctypes.data, N, theta, u.ctypes.data, v.ctypes.data,
+                                         w.ctypes.data, vis.ctypes.data, len(u), kern.ctypes.data,
+                                         P, R, S, w_min, w_step, C.byref(bad))
+    return g, int(fl), bad.value
+
+
+def grid_wtowers(N, theta, u, v, w, vis, kern, w_min, w_step, subgrid, margin, winc, nthreads=1,
+                 fast=False):
+    u, v, w = _c(u, np.float64), _c(v, np.float64), _c(w, np.float64)
+    vis = _c(vis, np.complex128)
+    kern = _c(kern, np.complex128)
+    P, R, _, S, _ = kern.shape
+    g = np.zeros((N, N), np.complex128)
+    bad = _L()
+    fl = lib(fast).orc_grid_wtowers_flat(g.ctypes.data, N, theta, u.ctypes.data, v.ctypes.data,
+                                         w.ctypes.data, vis.ctypes.data, len(u), kern.ctypes.data,
+                                         P, R, S, w_min, w_step, subgrid, margin, winc, nthreads,
+                                         C.byref(bad))
+    return g, int(fl), bad.value
+
+
+def cipow(z: complex, e: int) -> complex:
+    r = lib().orc_cipow(C128(z.real, z.imag), int(e))
+    return complex(r.re, r.im)
+
+
+def wtransfer(n, theta, winc):
+    out = np.zeros((n, n), np.complex128)
+    lib().orc_wtransfer(out.ctypes.data, n, theta, winc)
+    return out
+
+
+def fft2d(a, sign):
+    a = np.array(a, dtype=np.complex128, order="C", copy=True)
+    lib().orc_fft2d(a.ctypes.data, a.shape[0], sign)
+    return a
+
+
+def fft1d(a, sign):
+    a = np.array(a, dtype=np.complex128, order="C", copy=True)
+    lib().orc_fft1d(a.ctypes.data, a.shape[0], sign)
+    return a
+
+
+def fft_shift(a):
+    a = np.array(a, dtype=np.complex128, order="C", copy=True)
+    lib().orc_fft_shift(a.ctypes.data, a.shape[0])
+    return a
+
+
+def make_hermitian(a):
+    a = np.array(a, dtype=np.complex128, order="C", copy=True)
+    lib().orc_make_hermitian(a.ctypes.data, a.shape[0])
+    return a
+
+
+def weight(N, theta, u, v, vis):
+    u, v = _c(u, np.float64), _c(v, np.float64)
+    vis = np.array(vis, dtype=np.complex128, copy=True)
+    wg = np.zeros((N, N), np.uint32)
+    lib().orc_weight_flat(wg.ctypes.data, N, theta, u.ctypes.data, v.ctypes.data, vis.ctypes.data,
+                          len(u))
+    return vis, wg
+
+
+def grid_to_image(g):
+    g = np.array(g, dtype=np.complex128, order="C", copy=True)
+    lib().orc_grid_to_image(g.ctypes.data, g.shape[0])
+    return g
+
+
+def dft(px, py, N, lam, u, v, w, vis, nthreads=0):
+    px, py = _c(px, np.int32), _c(py, np.int32)
+    u, v, w = _c(u, np.float64), _c(v, np.float64), _c(w, np.float64)
+    vis = _c(vis, np.complex128)
+    out = np.zeros(len(px), np.complex128)
+    lib().orc_dft_flat(out.ctypes.data, px.ctypes.data, py.ctypes.data, len(px), N, lam,
+                       u.ctypes.data, v.ctypes.data, w.ctypes.data, vis.ctypes.data, len(u),
+                       nthreads or (os.cpu_count() or 1))
+    return out

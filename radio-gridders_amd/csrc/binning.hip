@@ -1,0 +1,141 @@
+// binning.hip -- scan kernels of the device-side counting sort.
+#include "binning.hpp"
+
+namespace rg {
+
+// Exclusive block scan of uint2 values (x and y scanned independently).
+// Returns the exclusive prefix of the calling thread and the block total.
+template <int NT>
+__device__ inline uint2 block_exclusive_scan(uint2 v, uint2 &total, uint2 *lds /* NT */)
+{
+    lds[threadIdx.x] = v;
+    __syncthreads();
+    // Hillis-Steele inclusive scan in LDS (NT <= 1024)
+    for (int off = 1; off < NT; off <<= 1) {
+        uint2 add = make_uint2(0, 0);
+        if ((int)threadIdx.x >= off)
+            add = lds[threadIdx.x - off];
+        __syncthreads();
+        uint2 cur = lds[threadIdx.x];
+        cur.x += add.x;
+        cur.y += add.y;
+        lds[threadIdx.x] = cur;
+        __syncthreads();
+    }
+    uint2 inc = lds[threadIdx.x];
+    total = lds[NT - 1];
+    __syncthreads();
+    return make_uint2(inc.x - v.x, inc.y - v.y);
+}
+
+__device__ inline uint32_t items_of(uint32_t c, uint32_t per_item)
+{
+    return (c + per_item - 1) / per_item;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const uint32_t *count, int64_t nkeys,
+                                                            uint32_t per_item, uint2 *block_sums)
+{
+    __shared__ uint2 lds[SCAN_BLOCK];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    uint2 acc = make_uint2(0, 0);
+    for (int j = 0; j < SCAN_PER_THREAD; ++j) {
+        const int64_t k = base + (int64_t)j * SCAN_BLOCK + threadIdx.x;
+        if (k < nkeys) {
+            const uint32_t c = count[k];
+            acc.x += c;
+            acc.y += items_of(c, per_item);
+        }
+    }
+    uint2 total;
+    block_exclusive_scan<SCAN_BLOCK>(acc, total, lds);
+    if (threadIdx.x == 0)
+        block_sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_blocks(uint2 *block_sums, int nblocks,
+                                                      unsigned long long *stats)
+{
+    __shared__ uint2 lds[1024];
+    const int per = (nblocks + 1023) / 1024;
+    const int b0 = threadIdx.x * per;
+    uint2 acc = make_uint2(0, 0);
+    for (int j = 0; j < per; ++j) {
+        if (b0 + j < nblocks) {
+            const uint2 v = block_sums[b0 + j];
+            acc.x += v.x;
+            acc.y += v.y;
+        }
+    }
+    uint2 total;
+    uint2 run = block_exclusive_scan<1024>(acc, total, lds);
+    for (int j = 0; j < per; ++j) {
+        if (b0 + j < nblocks) {
+            const uint2 v = block_sums[b0 + j];
+            block_sums[b0 + j] = run;
+            run.x += v.x;
+            run.y += v.y;
+        }
+    }
+    if (threadIdx.x == 0) {
+        block_sums[nblocks] = total;
+        stats[CNT_GRIDDED] = total.x;
+        stats[CNT_ITEMS] = total.y;
+    }
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t *count, int64_t nkeys,
+                                                          uint32_t per_item,
+                                                          const uint2 *block_sums,
+                                                          uint32_t *vis_start,
+                                                          uint32_t *item_start, uint32_t *cursor,
+                                                          Item *items)
+{
+    __shared__ uint32_t cnt[SCAN_TILE];
+    __shared__ uint2 lds[SCAN_BLOCK];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    for (int j = 0; j < SCAN_PER_THREAD; ++j) {
+        const int e = j * SCAN_BLOCK + threadIdx.x;
+        cnt[e] = (base + e < nkeys) ? count[base + e] : 0u;
+    }
+    __syncthreads();
+    const int e0 = threadIdx.x * SCAN_PER_THREAD;
+    uint2 acc = make_uint2(0, 0);
+    for (int j = 0; j < SCAN_PER_THREAD; ++j) {
+        const uint32_t c = cnt[e0 + j];
+        acc.x += c;
+        acc.y += items_of(c, per_item);
+    }
+    uint2 total;
+    uint2 run = block_exclusive_scan<SCAN_BLOCK>(acc, total, lds);
+    const uint2 boff = block_sums[blockIdx.x];
+    run.x += boff.x;
+    run.y += boff.y;
+    for (int j = 0; j < SCAN_PER_THREAD; ++j) {
+        const int64_t k = base + e0 + j;
+        if (k >= nkeys)
+            break;
+        const uint32_t c = cnt[e0 + j];
+        vis_start[k] = run.x;
+        item_start[k] = run.y;
+        cursor[k] = run.x;
+        const uint32_t ni = items_of(c, per_item);
+        for (uint32_t q = 0; q < ni; ++q) {
+            Item it;
+            it.key = (uint32_t)k;
+            it.start = run.x + q * per_item;
+            it.count = min(per_item, c - q * per_item);
+            items[run.y + q] = it;
+        }
+        run.x += c;
+        run.y += ni;
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+        // totals live after the last block's entry of block_sums
+        const uint2 tot = block_sums[gridDim.x];
+        vis_start[nkeys] = tot.x;
+        item_start[nkeys] = tot.y;
+    }
+}
+
+} // namespace rg

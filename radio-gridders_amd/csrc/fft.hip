@@ -1,0 +1,294 @@
+// fft.hip -- batched 2-D FFTs, fft-shift and transposes.
+//
+// n <= 512: row pass (B rows per block, padded LDS) then column pass
+// (CB columns per block, each row segment of the block is one 128-byte line).
+// n > 512: row pass, tiled transpose, row pass, transpose back.
+#include "fft_core.hpp"
+
+namespace rg {
+
+// ---- row pass ------------------------------------------------------------------
+// Transforms rows [blockIdx.x*B, +B) of nrows rows of length n stored
+// contiguously.  row_flags (optional): bit r set = row r non-zero; rows with
+// the bit clear are left untouched (they are zero and stay zero).
+template <typename T2, int LOGN, int B, bool INV>
+__global__ __launch_bounds__(B *((1 << LOGN) / 8)) void k_fft_rows(T2 *data, int64_t nrows,
+                                                                   const T2 *__restrict__ twg,
+                                                                   const uint32_t *row_flags)
+{
+    constexpr int n = 1 << LOGN;
+    constexpr int NT = B * (n / 8);
+    constexpr int PAD = B > 1 ? 1 : 0;
+    constexpr int LS = B + PAD;
+    constexpr bool TW_LDS = n * sizeof(T2) <= 8192;
+    __shared__ T2 lds[n * LS];
+    __shared__ T2 tws[TW_LDS ? n : 1];
+    const T2 *tw = twg;
+    if (TW_LDS) {
+        for (int k = threadIdx.x; k < n; k += NT)
+            tws[k] = twg[k];
+        tw = tws;
+    }
+    const int64_t row0 = (int64_t)blockIdx.x * B;
+    // block-uniform skip when every row of the block is empty
+    bool any = true;
+    if (row_flags) {
+        any = false;
+        for (int l = 0; l < B; ++l) {
+            const int64_t r = row0 + l;
+            if (r < nrows && ((row_flags[r >> 5] >> (r & 31)) & 1u))
+                any = true;
+        }
+    }
+    if (!any)
+        return;
+    // load: global order (line-major), element e -> (line, idx)
+    for (int e = threadIdx.x; e < B * n; e += NT) {
+        const int line = e / n, idx = e - line * n;
+        const int64_t r = row0 + line;
+        T2 val;
+        val.x = 0;
+        val.y = 0;
+        if (r < nrows)
+            val = data[r * n + idx];
+        lds[idx * LS + line] = val;
+    }
+    __syncthreads();
+    fft_block<T2, LOGN, B, LS, INV>(lds, tw);
+    for (int e = threadIdx.x; e < B * n; e += NT) {
+        const int line = e / n, idx = e - line * n;
+        const int64_t r = row0 + line;
+        if (r < nrows) {
+            if (!row_flags || ((row_flags[r >> 5] >> (r & 31)) & 1u))
+                data[r * n + idx] = lds[idx * LS + line];
+        }
+    }
+}
+
+// ---- column pass (n <= 512) ----------------------------------------------------
+// Block (cb, a): columns [cb*CB, +CB) of array a (n x n, contiguous).
+template <typename T2, int LOGN, int CB, bool INV>
+__global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_fft_cols(T2 *data,
+                                                                    const T2 *__restrict__ twg)
+{
+    constexpr int n = 1 << LOGN;
+    constexpr int NT = CB * (n / 8);
+    __shared__ T2 lds[n * CB];
+    __shared__ T2 tw[n];
+    for (int k = threadIdx.x; k < n; k += NT)
+        tw[k] = twg[k];
+    T2 *a = data + (size_t)blockIdx.y * n * n + (size_t)blockIdx.x * CB;
+    for (int e = threadIdx.x; e < n * CB; e += NT) {
+        const int row = e / CB, col = e - row * CB;
+        lds[e] = a[(size_t)row * n + col];
+    }
+    __syncthreads();
+    fft_block<T2, LOGN, CB, CB, INV>(lds, tw);
+    for (int e = threadIdx.x; e < n * CB; e += NT) {
+        const int row = e / CB, col = e - row * CB;
+        a[(size_t)row * n + col] = lds[e];
+    }
+}
+
+// ---- transpose -------------------------------------------------------------------
+template <typename T2>
+__global__ __launch_bounds__(256) void k_transpose(const T2 *__restrict__ in, T2 *__restrict__ out,
+                                                   int n)
+{
+    __shared__ T2 tile[32][33];
+    const size_t off = (size_t)blockIdx.z * n * n;
+    const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
+    const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;
+    for (int j = ty; j < 32; j += 8)
+        tile[j][tx] = in[off + (size_t)(y0 + j) * n + x0 + tx];
+    __syncthreads();
+    for (int j = ty; j < 32; j += 8)
+        out[off + (size_t)(x0 + j) * n + y0 + tx] = tile[tx][j];
+}
+
+// ---- fft shift (fft_shift_kernel, common_kernels.cuh:17-32) ------------------------
+template <typename T2>
+__global__ __launch_bounds__(256) void k_fft_shift(T2 *data, int n, int batch)
+{
+    const int64_t half = (int64_t)n * (n / 2);
+    const int64_t total = half * batch;
+    const int64_t nn = (int64_t)n * n;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t b = t / half, e = t - b * half;
+        const int64_t y = e / (n / 2), x = e - y * (n / 2);
+        const int64_t ix0 = y * n + x;
+        const int64_t ix1 = (ix0 + (int64_t)(n + 1) * (n / 2)) % nn;
+        T2 *a = data + b * nn;
+        const T2 tmp = a[ix0];
+        a[ix0] = a[ix1];
+        a[ix1] = tmp;
+    }
+}
+
+int fft_shift(void *data, int type, int n, int batch, hipStream_t s)
+{
+    if (n <= 0 || (n & 1))
+        return fail(RG_ERR_PARAM, "fft_shift needs an even size (got %d)", n);
+    const int64_t total = (int64_t)n * (n / 2) * batch;
+    int64_t blocks = (total + 255) / 256;
+    if (blocks > 65536)
+        blocks = 65536;
+    if (type == RG_C128)
+        hipLaunchKernelGGL(k_fft_shift<double2>, dim3((unsigned)blocks), dim3(256), 0, s,
+                           (double2 *)data, n, batch);
+    else
+        hipLaunchKernelGGL(k_fft_shift<float2>, dim3((unsigned)blocks), dim3(256), 0, s,
+                           (float2 *)data, n, batch);
+    RG_HIP(hipGetLastError());
+    return RG_OK;
+}
+
+// ---- twiddles ------------------------------------------------------------------------
+
+int get_twiddles(rg_context *ctx, int n, int type, int sign, const void **out)
+{
+    TwKey key(n, type, sign);
+    auto it = ctx->twiddles.find(key);
+    if (it != ctx->twiddles.end()) {
+        *out = it->second.p;
+        return RG_OK;
+    }
+    DevBuf buf;
+    const size_t esz = type == RG_C128 ? 16 : 8;
+    RG_TRY(buf.ensure(esz * n));
+    std::vector<double> hd(2 * (size_t)n);
+    for (int k = 0; k < n; ++k) {
+        // exact quarter-wave symmetric values from one accurate evaluation
+        const double a = 2.0 * M_PI * (double)k / (double)n;
+        hd[2 * k] = cos(a);
+        hd[2 * k + 1] = sign * sin(a);
+    }
+    if (type == RG_C128) {
+        RG_HIP(hipMemcpy(buf.p, hd.data(), esz * n, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> hf(2 * (size_t)n);
+        for (size_t i = 0; i < hf.size(); ++i)
+            hf[i] = (float)hd[i];
+        RG_HIP(hipMemcpy(buf.p, hf.data(), esz * n, hipMemcpyHostToDevice));
+    }
+    *out = buf.p;
+    ctx->twiddles.emplace(key, buf);
+    return RG_OK;
+}
+
+// ---- dispatch ----------------------------------------------------------------------
+
+template <typename T2, int LOGN, bool INV>
+static int rows_t(T2 *data, int64_t nrows, const T2 *tw, const uint32_t *flags, hipStream_t s)
+{
+    constexpr int n = 1 << LOGN;
+    constexpr int TPL = n / 8;
+    constexpr int B = TPL >= 256 ? 1 : 256 / TPL;
+    const int64_t blocks = (nrows + B - 1) / B;
+    hipLaunchKernelGGL((k_fft_rows<T2, LOGN, B, INV>), dim3((unsigned)blocks), dim3(B * TPL), 0, s,
+                       data, nrows, tw, flags);
+    RG_HIP(hipGetLastError());
+    return RG_OK;
+}
+
+template <typename T2, int LOGN, bool INV>
+static int cols_t(T2 *data, int batch, const T2 *tw, hipStream_t s)
+{
+    constexpr int n = 1 << LOGN;
+    constexpr int CB = sizeof(T2) == 8 ? 16 : 8;
+    constexpr int cb = CB > n ? n : CB;
+    hipLaunchKernelGGL((k_fft_cols<T2, LOGN, cb, INV>), dim3(n / cb, batch), dim3(cb * (n / 8)), 0,
+                       s, data, tw);
+    RG_HIP(hipGetLastError());
+    return RG_OK;
+}
+
+template <typename T2, bool INV>
+int fft_rows_dispatch(T2 *data, int n, int64_t nrows, const T2 *tw, const uint32_t *flags,
+                      hipStream_t s)
+{
+    switch (n) {
+    case 8: return rows_t<T2, 3, INV>(data, nrows, tw, flags, s);
+    case 16: return rows_t<T2, 4, INV>(data, nrows, tw, flags, s);
+    case 32: return rows_t<T2, 5, INV>(data, nrows, tw, flags, s);
+    case 64: return rows_t<T2, 6, INV>(data, nrows, tw, flags, s);
+    case 128: return rows_t<T2, 7, INV>(data, nrows, tw, flags, s);
+    case 256: return rows_t<T2, 8, INV>(data, nrows, tw, flags, s);
+    case 512: return rows_t<T2, 9, INV>(data, nrows, tw, flags, s);
+    case 1024: return rows_t<T2, 10, INV>(data, nrows, tw, flags, s);
+    case 2048: return rows_t<T2, 11, INV>(data, nrows, tw, flags, s);
+    case 4096: return rows_t<T2, 12, INV>(data, nrows, tw, flags, s);
+    default: return fail(RG_ERR_PARAM, "FFT size %d unsupported (power of two 8..4096)", n);
+    }
+}
+
+template <typename T2, bool INV>
+static int fft_cols_dispatch(T2 *data, int n, int batch, const T2 *tw, hipStream_t s)
+{
+    switch (n) {
+    case 8: return cols_t<T2, 3, INV>(data, batch, tw, s);
+    case 16: return cols_t<T2, 4, INV>(data, batch, tw, s);
+    case 32: return cols_t<T2, 5, INV>(data, batch, tw, s);
+    case 64: return cols_t<T2, 6, INV>(data, batch, tw, s);
+    case 128: return cols_t<T2, 7, INV>(data, batch, tw, s);
+    case 256: return cols_t<T2, 8, INV>(data, batch, tw, s);
+    case 512: return cols_t<T2, 9, INV>(data, batch, tw, s);
+    default: return fail(RG_ERR_PARAM, "column pass size %d unsupported", n);
+    }
+}
+
+template int fft_rows_dispatch<float2, true>(float2 *, int, int64_t, const float2 *,
+                                             const uint32_t *, hipStream_t);
+template int fft_rows_dispatch<double2, true>(double2 *, int, int64_t, const double2 *,
+                                              const uint32_t *, hipStream_t);
+template int fft_rows_dispatch<float2, false>(float2 *, int, int64_t, const float2 *,
+                                              const uint32_t *, hipStream_t);
+template int fft_rows_dispatch<double2, false>(double2 *, int, int64_t, const double2 *,
+                                               const uint32_t *, hipStream_t);
+
+template <typename T2>
+static int fft2d_t(rg_context *ctx, T2 *data, int type, int n, int batch, int sign, hipStream_t s)
+{
+    const void *twv = nullptr;
+    RG_TRY(get_twiddles(ctx, n, type, sign, &twv));
+    const T2 *tw = (const T2 *)twv;
+    const bool inv = sign > 0;
+    const int64_t nrows = (int64_t)n * batch;
+    if (n <= 512) {
+        if (inv) {
+            RG_TRY((fft_rows_dispatch<T2, true>(data, n, nrows, tw, nullptr, s)));
+            return fft_cols_dispatch<T2, true>(data, n, batch, tw, s);
+        }
+        RG_TRY((fft_rows_dispatch<T2, false>(data, n, nrows, tw, nullptr, s)));
+        return fft_cols_dispatch<T2, false>(data, n, batch, tw, s);
+    }
+    // rows, transpose, rows, transpose back
+    RG_TRY(ctx->scratch[11].ensure(sizeof(T2) * (size_t)n * n * batch));
+    T2 *tmp = ctx->scratch[11].as<T2>();
+    dim3 tg(n / 32, n / 32, batch);
+    for (int pass = 0; pass < 2; ++pass) {
+        if (inv)
+            RG_TRY((fft_rows_dispatch<T2, true>(data, n, nrows, tw, nullptr, s)));
+        else
+            RG_TRY((fft_rows_dispatch<T2, false>(data, n, nrows, tw, nullptr, s)));
+        hipLaunchKernelGGL(k_transpose<T2>, tg, dim3(256), 0, s, (const T2 *)data, tmp, n);
+        RG_HIP(hipGetLastError());
+        std::swap(data, tmp);
+    }
+    // two transposes leave the result in the original array
+    return RG_OK;
+}
+
+int fft2d(rg_context *ctx, void *data, int type, int n, int batch, int sign, hipStream_t s)
+{
+    if (n < 8 || n > 4096 || (n & (n - 1)))
+        return fail(RG_ERR_PARAM, "FFT size %d unsupported (power of two 8..4096)", n);
+    if (batch < 1 || (sign != 1 && sign != -1))
+        return fail(RG_ERR_PARAM, "invalid FFT batch/sign");
+    if (type == RG_C128)
+        return fft2d_t<double2>(ctx, (double2 *)data, type, n, batch, sign, s);
+    return fft2d_t<float2>(ctx, (float2 *)data, type, n, batch, sign, s);
+}
+
+} // namespace rg

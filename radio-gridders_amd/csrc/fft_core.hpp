@@ -1,0 +1,174 @@
+// fft_core.hpp -- LDS Stockham FFT of a block of lines (device code).
+//
+// A block holds B lines of length n = 2^LOGN in LDS, element (line, idx) at
+// lds[idx * LS + line] with LS = B + PAD.  The block's threads are mapped
+// line = tid % B, lt = tid / B (TPL = n / 8 threads per line), so
+// neighbouring lanes touch neighbouring LDS words.  Every stage gives each
+// thread 8 values: one radix-8, two radix-4 or four radix-2 butterflies
+// (Stockham autosort, decimation in time:
+//   X[(j/Ns)*Ns*R + j%Ns + r*Ns] = DFT_R over r' of x[j + r'*n/R] w^(r'*(j%Ns)),
+//   w = exp(sign 2 pi i / (Ns R)) ).
+// Twiddles come from an LDS table tw[k] = exp(sign 2 pi i k / n), k < n.
+#pragma once
+
+#include "binning.hpp"
+
+namespace rg {
+
+template <typename T2> __device__ inline T2 cmul(T2 a, T2 b)
+{
+    T2 r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.x * b.y + a.y * b.x;
+    return r;
+}
+template <typename T2> __device__ inline T2 cadd(T2 a, T2 b)
+{
+    T2 r;
+    r.x = a.x + b.x;
+    r.y = a.y + b.y;
+    return r;
+}
+template <typename T2> __device__ inline T2 csub(T2 a, T2 b)
+{
+    T2 r;
+    r.x = a.x - b.x;
+    r.y = a.y - b.y;
+    return r;
+}
+// multiply by sign*i
+template <bool INV, typename T2> __device__ inline T2 mul_si(T2 a)
+{
+    T2 r;
+    if (INV) { // +i
+        r.x = -a.y;
+        r.y = a.x;
+    } else { // -i
+        r.x = a.y;
+        r.y = -a.x;
+    }
+    return r;
+}
+
+template <bool INV, typename T2> __device__ inline void dft2(T2 &a, T2 &b)
+{
+    const T2 t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+
+template <bool INV, typename T2> __device__ inline void dft4(T2 &x0, T2 &x1, T2 &x2, T2 &x3)
+{
+    const T2 a0 = cadd(x0, x2), a1 = csub(x0, x2);
+    const T2 a2 = cadd(x1, x3), a3 = mul_si<INV>(csub(x1, x3));
+    x0 = cadd(a0, a2);
+    x2 = csub(a0, a2);
+    x1 = cadd(a1, a3);
+    x3 = csub(a1, a3);
+}
+
+template <bool INV, typename T2> __device__ inline void dft8(T2 *x)
+{
+    using T = decltype(x[0].x);
+    const T h = (T)0.70710678118654752440084436210484903928483593768847;
+    // even / odd radix-4
+    T2 e0 = x[0], e1 = x[2], e2 = x[4], e3 = x[6];
+    T2 o0 = x[1], o1 = x[3], o2 = x[5], o3 = x[7];
+    dft4<INV>(e0, e1, e2, e3);
+    dft4<INV>(o0, o1, o2, o3);
+    // o_k *= w8^k, w8 = (1 + s i)/sqrt2
+    T2 t1, t3;
+    if (INV) {
+        t1.x = (o1.x - o1.y) * h;
+        t1.y = (o1.y + o1.x) * h;
+        t3.x = (-o3.x - o3.y) * h;
+        t3.y = (-o3.y + o3.x) * h;
+    } else {
+        t1.x = (o1.x + o1.y) * h;
+        t1.y = (o1.y - o1.x) * h;
+        t3.x = (-o3.x + o3.y) * h;
+        t3.y = (-o3.y - o3.x) * h;
+    }
+    const T2 t2 = mul_si<INV>(o2);
+    x[0] = cadd(e0, o0);
+    x[4] = csub(e0, o0);
+    x[1] = cadd(e1, t1);
+    x[5] = csub(e1, t1);
+    x[2] = cadd(e2, t2);
+    x[6] = csub(e2, t2);
+    x[3] = cadd(e3, t3);
+    x[7] = csub(e3, t3);
+}
+
+// radix of the stage with current sub-transform size Ns
+template <int LOGN> __host__ __device__ constexpr int stage_radix(int logns)
+{
+    return (LOGN - logns) >= 3 ? 8 : ((LOGN - logns) == 2 ? 4 : 2);
+}
+
+// One Stockham stage with radix R on all lines of the block.
+template <typename T2, int LOGN, int B, int LS, int R, bool INV>
+__device__ inline void fft_stage(T2 *lds, const T2 *tw, int logns)
+{
+    constexpr int n = 1 << LOGN;
+    constexpr int TPL = n / 8;
+    constexpr int NBUT = n / R;
+    constexpr int PER = NBUT / TPL; // butterflies per thread: 1, 2 or 4
+    const int line = threadIdx.x % B;
+    const int lt = threadIdx.x / B;
+    const int Ns = 1 << logns;
+    T2 v[8];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int j = lt + q * TPL;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            v[q * R + r] = lds[(j + r * NBUT) * LS + line];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int j = lt + q * TPL;
+        const int k = j & (Ns - 1);
+        const int tstep = k * (n >> (logns + (R == 8 ? 3 : (R == 4 ? 2 : 1))));
+#pragma unroll
+        for (int r = 1; r < R; ++r)
+            v[q * R + r] = cmul(v[q * R + r], tw[r * tstep]);
+        if (R == 8)
+            dft8<INV>(&v[q * R]);
+        else if (R == 4)
+            dft4<INV>(v[q * R], v[q * R + 1], v[q * R + 2], v[q * R + 3]);
+        else
+            dft2<INV>(v[q * R], v[q * R + 1]);
+        const int d = (j >> logns) * (Ns * R) + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            lds[(d + r * Ns) * LS + line] = v[q * R + r];
+    }
+    __syncthreads();
+}
+
+// Full transform of the block (data already in lds, synchronised).
+template <typename T2, int LOGN, int B, int LS, bool INV>
+__device__ inline void fft_block(T2 *lds, const T2 *tw)
+{
+    int logns = 0;
+#pragma unroll
+    for (int st = 0; st < 8; ++st) {
+        if (logns >= LOGN)
+            break;
+        const int rem = LOGN - logns;
+        if (rem >= 3) {
+            fft_stage<T2, LOGN, B, LS, 8, INV>(lds, tw, logns);
+            logns += 3;
+        } else if (rem == 2) {
+            fft_stage<T2, LOGN, B, LS, 4, INV>(lds, tw, logns);
+            logns += 2;
+        } else {
+            fft_stage<T2, LOGN, B, LS, 2, INV>(lds, tw, logns);
+            logns += 1;
+        }
+    }
+}
+
+} // namespace rg

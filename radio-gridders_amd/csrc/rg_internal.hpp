@@ -1,0 +1,163 @@
+// rg_internal.hpp -- shared definitions of librg (host + device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "rg.h"
+
+namespace rg {
+
+// ---- errors --------------------------------------------------------------
+
+void set_error(const char *fmt, ...);
+int fail(int code, const char *fmt, ...);
+
+#define RG_HIP(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t rg_e_ = (expr);                                                         \
+        if (rg_e_ != hipSuccess)                                                           \
+            return ::rg::fail(RG_ERR_HIP, "%s:%d: %s: %s", __FILE__, __LINE__, #expr,      \
+                              hipGetErrorString(rg_e_));                                   \
+    } while (0)
+
+#define RG_TRY(expr)                                                                       \
+    do {                                                                                   \
+        int rg_s_ = (expr);                                                                \
+        if (rg_s_ != RG_OK)                                                                \
+            return rg_s_;                                                                  \
+    } while (0)
+
+// ---- coordinates -----------------------------------------------------------
+// FP64 index math, written exactly as the reference so that host and device
+// round identically: contraction is off inside these functions.
+
+// frac_coord (src/reference/grid.c:37-57, src/common/radio.cuh:156-176):
+// footprint top-left (gx, gy) and oversample indices (xf, yf).
+__host__ __device__ inline void frac_coord(int N, int S, int R, double theta, double u,
+                                           double v, double du, double dv, int &gx, int &gy,
+                                           int &xf, int &yf)
+{
+#pragma clang fp contract(off)
+    double x = theta * (u - du);
+    double y = theta * (v - dv);
+    int flx = (int)floor(x + .5 / R);
+    int fly = (int)floor(y + .5 / R);
+    xf = (int)floor((x - (double)flx) * R + .5);
+    yf = (int)floor((y - (double)fly) * R + .5);
+    // x + .5/R can round to the next integer, leaving (x - flx) R + .5 just
+    // below 0 (or reaching R): keep the oversample index inside the table.
+    xf = xf < 0 ? 0 : (xf >= R ? R - 1 : xf);
+    yf = yf < 0 ? 0 : (yf >= R ? R - 1 : yf);
+    gx = flx + N / 2 - S / 2;
+    gy = fly + N / 2 - S / 2;
+}
+
+// W-plane index (grid.c:168-169, grid_cuda.cu:84).
+__host__ __device__ inline int w_plane(double w, double w_min, double w_step)
+{
+#pragma clang fp contract(off)
+    return (int)floor((w - w_min) / w_step + .5);
+}
+
+// W-towers chunk index (radio.cuh:361-362): double division, truncated.
+__host__ __device__ inline int tower_index(double u, double theta, int N, int chunk)
+{
+#pragma clang fp contract(off)
+    double q = (floor(u * theta + 0.5) + N / 2) / chunk;
+    return (int)q;
+}
+
+// W-towers floor index (grid.c:453-455).
+__host__ __device__ inline int floor_index(double w, double winc)
+{
+#pragma clang fp contract(off)
+    return (int)floor(w / winc + 0.5);
+}
+
+// ---- device buffers ----------------------------------------------------------
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t n);
+    void release();
+    template <typename T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+struct PinnedBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t n);
+    void release();
+};
+
+// Twiddle table key: (n, type, sign)
+using TwKey = std::tuple<int, int, int>;
+
+} // namespace rg
+
+// Context: scratch memory, twiddle tables and the last call's counters.
+struct rg_context {
+    int device = 0;
+    // counters[0..7] written by kernels (int64): see Counter enum in gridding code
+    rg::DevBuf counters;
+    rg::PinnedBuf counters_host;
+    hipStream_t last_stream = nullptr;
+    int last_kind = 0; // 0 none, 1 wproj, 2 wtowers
+    int64_t last_nvis = 0;
+    // generic scratch arrays
+    rg::DevBuf scratch[12];
+    std::map<rg::TwKey, rg::DevBuf> twiddles;
+    // kernel timers (RG_TIMER_*): recorded (start, stop) event pairs
+    bool timing = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> timer_events[RG_TIMER__COUNT];
+    std::vector<hipEvent_t> event_pool;
+};
+
+namespace rg {
+
+enum Counter {
+    CNT_DROPPED = 0,
+    CNT_BAD_PLANE = 1,
+    CNT_ITEMS = 2,
+    CNT_TASKS = 3,
+    CNT_TOWERS = 4,
+    CNT_GRIDDED = 5,
+    CNT_MAX_ITEMS = 6,
+    CNT__N = 8
+};
+
+int ensure_counters(rg_context *ctx);
+
+// Records a (start, stop) event pair around the launches issued while it is
+// alive, when the context's timing is enabled.
+struct KernelTimer {
+    rg_context *ctx;
+    int which;
+    hipStream_t s;
+    hipEvent_t start = nullptr;
+    KernelTimer(rg_context *c, int w, hipStream_t st);
+    ~KernelTimer();
+};
+// twiddle table exp(sign * 2 pi i k / n), k < n, of `type`
+int get_twiddles(rg_context *ctx, int n, int type, int sign, const void **out);
+
+// internal entry points shared between translation units
+int wproj_plain(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis,
+                const rg_wkern *wk, void *grid, hipStream_t s);
+int wproj_batched(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis,
+                  const int64_t *offsets, int n_chan, const rg_wkern *wk, void *grid,
+                  int grid_type, hipStream_t s);
+int wtowers_run(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis,
+                const rg_wkern *wk, void *grid, int grid_type, hipStream_t s);
+int fft2d(rg_context *ctx, void *data, int type, int n, int batch, int sign, hipStream_t s);
+int fft_shift(void *data, int type, int n, int batch, hipStream_t s);
+int zero_fill(void *p, size_t bytes, hipStream_t s);
+
+} // namespace rg

@@ -1,0 +1,163 @@
+// tiles.hpp -- tiled Romein accumulation kernel shared by W-projection and
+// W-towers (device code).
+//
+// One workgroup of S*S threads takes one work item: up to K sorted records of
+// one key.  A key is (group, tile): the group selects the target array (a
+// channel grid for W-projection, the subgrid of a (tower, floor) task for
+// W-towers) and the tile a TILE x TILE block of footprint top-left positions,
+// whose footprints cover (TILE + S - 1)^2 cells held in LDS.  Thread
+// (myU, myV) handles, for every record, the footprint cell congruent to
+// (myU, myV) modulo S (Romein); it keeps a register sum while that cell stays
+// the same and adds it into LDS when the cell changes.  Two threads never
+// share a cell, so the LDS updates need no atomics.  At the end the tile is
+// added to the target with global atomics (neighbouring tiles and items of
+// the same tile overlap).
+#pragma once
+
+#include "binning.hpp"
+
+namespace rg {
+
+template <typename T> __device__ inline void two_sum_into(T &hi, T &lo, T b)
+{
+#pragma clang fp contract(off)
+    const T s = hi + b;
+    const T bp = s - hi;
+    const T e = (hi - (s - bp)) + (b - bp);
+    hi = s;
+    lo += e;
+}
+
+struct TileSink {
+    void *base;              // target arrays, group g at base + g * gsize^2 complex elements
+    int gsize;               // edge of a target array (N or n)
+    int ntx;                 // tiles per row
+    int ntiles;              // tiles per group
+    const int32_t *group_map; // optional: group -> rank (slot of the target array)
+    int32_t group_offset;     // rank - offset = target slot
+    uint32_t *row_flags;      // optional: bit per target row, gsize bits per slot
+};
+
+template <typename T, int S, int TILE, bool COMP, typename GT>
+__global__ __launch_bounds__(S *S) void k_tiles(const Rec<T> *__restrict__ recs,
+                                                const Item *__restrict__ items, uint32_t item0,
+                                                uint32_t item_end,
+                                                const unsigned long long *__restrict__ stats,
+                                                const typename Vec2<T>::type *__restrict__ kern,
+                                                TileSink sink)
+{
+    constexpr int NT = S * S;
+    constexpr int RT = TILE + S - 1;
+    constexpr int CELLS = RT * RT;
+    constexpr int LC = COMP ? CELLS : 1;
+    __shared__ T s_re[CELLS];
+    __shared__ T s_im[CELLS];
+    __shared__ T s_lre[LC];
+    __shared__ T s_lim[LC];
+    __shared__ Rec<T> s_rec[NT];
+    __shared__ uint32_t s_rowmask[(RT + 31) / 32];
+
+    const uint32_t end = stats ? (uint32_t)stats[CNT_ITEMS] : item_end;
+    const uint32_t it = item0 + blockIdx.x;
+    if (it >= end)
+        return;
+    const Item item = items[it];
+
+    for (int c = threadIdx.x; c < CELLS; c += NT) {
+        s_re[c] = T(0);
+        s_im[c] = T(0);
+        if (COMP) {
+            s_lre[c] = T(0);
+            s_lim[c] = T(0);
+        }
+    }
+    if (threadIdx.x < (RT + 31) / 32)
+        s_rowmask[threadIdx.x] = 0;
+
+    const int myU = threadIdx.x % S, myV = threadIdx.x / S;
+    int cur = -1;
+    T sr = T(0), si = T(0);
+
+    for (uint32_t b = 0; b < item.count; b += NT) {
+        const uint32_t nloc = min((uint32_t)NT, item.count - b);
+        __syncthreads();
+        if (threadIdx.x < nloc)
+            s_rec[threadIdx.x] = recs[item.start + b + threadIdx.x];
+        __syncthreads();
+        for (uint32_t j = 0; j < nloc; ++j) {
+            const Rec<T> r = s_rec[j];
+            const int lx = (int)(r.pos & 0xffffu), ly = (int)(r.pos >> 16);
+            const int cx = (myU - lx) & (S - 1);
+            const int cy = (myV - ly) & (S - 1);
+            const int cell = (ly + cy) * RT + (lx + cx);
+            if (cell != cur) {
+                if (cur >= 0) {
+                    if (COMP) {
+                        two_sum_into(s_re[cur], s_lre[cur], sr);
+                        two_sum_into(s_im[cur], s_lim[cur], si);
+                    } else {
+                        s_re[cur] += sr;
+                        s_im[cur] += si;
+                    }
+                }
+                cur = cell;
+                sr = T(0);
+                si = T(0);
+            }
+            const typename Vec2<T>::type k = kern[(size_t)r.kidx * NT + cy * S + cx];
+            sr += k.x * r.vis.x + k.y * r.vis.y;
+            si += k.x * r.vis.y - k.y * r.vis.x;
+        }
+    }
+    if (cur >= 0) {
+        if (COMP) {
+            two_sum_into(s_re[cur], s_lre[cur], sr);
+            two_sum_into(s_im[cur], s_lim[cur], si);
+        } else {
+            s_re[cur] += sr;
+            s_im[cur] += si;
+        }
+    }
+    __syncthreads();
+
+    const uint32_t group = item.key / (uint32_t)sink.ntiles;
+    const int tile = (int)(item.key - group * (uint32_t)sink.ntiles);
+    const int tx = tile % sink.ntx, ty = tile / sink.ntx;
+    const int ox = tx * TILE - S, oy = ty * TILE - S;
+    int64_t slot = (int64_t)group;
+    if (sink.group_map)
+        slot = (int64_t)sink.group_map[group] - sink.group_offset;
+    const int G = sink.gsize;
+    GT *g = reinterpret_cast<GT *>(sink.base) + (size_t)slot * G * G * 2;
+    for (int c = threadIdx.x; c < CELLS; c += NT) {
+        const int ry = c / RT, rx = c - ry * RT;
+        const int gx = ox + rx, gy = oy + ry;
+        if ((unsigned)gx < (unsigned)G && (unsigned)gy < (unsigned)G) {
+            GT re = (GT)s_re[c], im = (GT)s_im[c];
+            if (COMP) {
+                re += (GT)s_lre[c];
+                im += (GT)s_lim[c];
+            }
+            if (re != GT(0) || im != GT(0)) {
+                GT *dst = g + 2 * ((size_t)gy * G + gx);
+                atomicAdd(dst, re);
+                atomicAdd(dst + 1, im);
+                if (sink.row_flags)
+                    atomicOr(&s_rowmask[ry >> 5], 1u << (ry & 31));
+            }
+        }
+    }
+    if (sink.row_flags) {
+        __syncthreads();
+        if (threadIdx.x < RT) {
+            const int ry = threadIdx.x;
+            if ((s_rowmask[ry >> 5] >> (ry & 31)) & 1u) {
+                const int gy = oy + ry;
+                uint32_t *f = sink.row_flags + (size_t)slot * ((G + 31) / 32);
+                atomicOr(&f[gy >> 5], 1u << (gy & 31));
+            }
+        }
+    }
+}
+
+} // namespace rg

@@ -1,0 +1,724 @@
+// wtowers.hip -- W-towers gridder (wtowers_CUDA_flat, grid_cuda.cu:522-746;
+// CPU semantics grid_wtowers, src/reference/grid.c:295-555).
+//
+// A visibility belongs to tower (cx, cy) = flat chunk rule (radio.cuh:361-362)
+// and floor wp = floor(w / winc + .5) (grid.c:453-455).  Each non-empty
+// (tower, floor) task grids its visibilities into an n x n subgrid with
+// offsets (u_mid, v_mid, w_mid = wp * winc), inverse-FFTs it and adds it to
+// the tower's subimage times F^-wp, F = exp(2 pi i winc (1 - sqrt(1-l^2-m^2)))
+// (the reference's fresnel_subimg_mul chain followed by the w = 0 transfer,
+// common_kernels.cu:13-40, which multiply to exactly this factor).  Each
+// tower's subimage is forward-FFT'd and folded / n^2 into the FP64 master grid
+// (add_subs2main_kernel, common_kernels.cu:43-79).
+//
+// PLAIN mode keeps the reference's launch structure in FP64: per task a
+// scatter_grid_kernel_flat-shaped Romein scatter <<<128, 128>>> with FP64
+// atomics, a 2-D inverse FFT and a Fresnel multiply-add, one task at a time.
+// BATCHED mode processes `floor_batch` tasks per round: tiled LDS Romein into
+// their subgrids, one row-IFFT pass over all non-empty rows, and a column pass
+// that inverse-transforms each column block, applies the w-screen and sums all
+// floors of a tower in registers before one read-modify-write of the
+// subimage.
+#include <algorithm>
+
+#include "binning_impl.hpp"
+#include "fft_core.hpp"
+#include "tiles.hpp"
+
+namespace rg {
+
+template <typename T2, bool INV>
+int fft_rows_dispatch(T2 *data, int n, int64_t nrows, const T2 *tw, const uint32_t *flags,
+                      hipStream_t s);
+
+// ---- binning -------------------------------------------------------------------------
+
+struct TowerKeyer {
+    const double *u, *v, *w;
+    const void *vis;
+    int vis_type;
+    int N, S, R, P;
+    double theta, w_min, w_step;
+    int n, chunk, cc;
+    double winc;
+    int wp_min, nwp;
+    int tile, ntx, ntiles; // tile == 0: key = task, pos = raw subgrid corner + S
+
+    __device__ int classify(int64_t i, uint32_t &key, uint32_t &pos, uint32_t &kidx) const
+    {
+        const double ui = u[i], vi = v[i], wi = w[i];
+        const int cx = tower_index(ui, theta, N, chunk);
+        const int cy = tower_index(vi, theta, N, chunk);
+        if (cx < 0 || cx >= cc || cy < 0 || cy >= cc)
+            return VIS_DROPPED;
+        const int wp = floor_index(wi, winc);
+        if (wp < wp_min || wp >= wp_min + nwp)
+            return VIS_DROPPED;
+        const int x_min = chunk * cx - N / 2;
+        const int y_min = chunk * cy - N / 2;
+        const double u_mid = (double)(x_min + chunk / 2) / theta;
+        const double v_mid = (double)(y_min + chunk / 2) / theta;
+        const double w_mid = (double)wp * winc;
+        int gx, gy, xf, yf;
+        frac_coord(n, S, R, theta, ui, vi, u_mid, v_mid, gx, gy, xf, yf);
+        if (gx <= -S || gx >= n || gy <= -S || gy >= n)
+            return VIS_DROPPED;
+        const int p = w_plane(wi - w_mid, w_min, w_step);
+        if (p < 0 || p >= P)
+            return VIS_BAD_PLANE;
+        const uint32_t task = (uint32_t)((cy * cc + cx) * nwp + (wp - wp_min));
+        if (tile > 0) {
+            const int tx = (gx + S) / tile, ty = (gy + S) / tile;
+            const int lx = gx + S - tx * tile, ly = gy + S - ty * tile;
+            key = task * (uint32_t)ntiles + (uint32_t)(ty * ntx + tx);
+            pos = (uint32_t)lx | ((uint32_t)ly << 16);
+        } else {
+            key = task;
+            pos = (uint32_t)(gx + S) | ((uint32_t)(gy + S) << 16);
+        }
+        kidx = (uint32_t)((p * R + yf) * R + xf);
+        return VIS_OK;
+    }
+
+    template <typename T> __device__ typename Vec2<T>::type vis_value(int64_t i) const
+    {
+        typename Vec2<T>::type r;
+        if (vis_type == RG_C128) {
+            const double2 d = reinterpret_cast<const double2 *>(vis)[i];
+            r.x = (T)d.x;
+            r.y = (T)d.y;
+        } else {
+            const float2 f = reinterpret_cast<const float2 *>(vis)[i];
+            r.x = (T)f.x;
+            r.y = (T)f.y;
+        }
+        return r;
+    }
+};
+
+// min / max floor index over visibilities that fall in the tower set
+__global__ __launch_bounds__(256) void k_floor_range(const double *u, const double *v,
+                                                     const double *w, int64_t n, double theta,
+                                                     int N, int chunk, int cc, double winc,
+                                                     int *range)
+{
+    __shared__ int smin[256], smax[256];
+    int lo = INT_MAX, hi = INT_MIN;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int cx = tower_index(u[i], theta, N, chunk);
+        const int cy = tower_index(v[i], theta, N, chunk);
+        if (cx < 0 || cx >= cc || cy < 0 || cy >= cc)
+            continue;
+        const int wp = floor_index(w[i], winc);
+        lo = min(lo, wp);
+        hi = max(hi, wp);
+    }
+    smin[threadIdx.x] = lo;
+    smax[threadIdx.x] = hi;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            smin[threadIdx.x] = min(smin[threadIdx.x], smin[threadIdx.x + s]);
+            smax[threadIdx.x] = max(smax[threadIdx.x], smax[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (smin[0] != INT_MAX)
+            atomicMin(&range[0], smin[0]);
+        if (smax[0] != INT_MIN)
+            atomicMax(&range[1], smax[0]);
+    }
+}
+
+__global__ void k_range_init(int *range)
+{
+    range[0] = INT_MAX;
+    range[1] = INT_MIN;
+}
+
+// per task slot: visibility count and first work item
+__global__ __launch_bounds__(256) void k_task_info(const uint32_t *vis_start,
+                                                   const uint32_t *item_start, int64_t nslots,
+                                                   int ntiles, uint32_t *nvis, uint32_t *item0)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nslots)
+        return;
+    nvis[t] = vis_start[(t + 1) * ntiles] - vis_start[t * ntiles];
+    item0[t] = item_start[t * ntiles];
+}
+
+// ---- screen ------------------------------------------------------------------------------
+
+// 1 - sqrt(1 - l^2 - m^2) at subimage pixel (x, y) in the fft-shifted layout
+// (zero image position at (0, 0), grid.c:317-319).
+__device__ inline double screen_base(int x, int y, int n, double theta)
+{
+    const int xs = x < n / 2 ? x : x - n;
+    const int ys = y < n / 2 ? y : y - n;
+    const double l = theta * (double)xs / n;
+    const double m = theta * (double)ys / n;
+    return 1 - sqrt(1 - l * l - m * m);
+}
+
+template <typename T> struct Trig;
+template <> struct Trig<float> {
+    __device__ static void rot(double rev, float &c, float &s)
+    {
+        const float f = (float)(rev - rint(rev));
+        sincospif(2.0f * f, &s, &c);
+    }
+};
+template <> struct Trig<double> {
+    __device__ static void rot(double rev, double &c, double &s)
+    {
+        const double f = rev - rint(rev);
+        sincospi(2.0 * f, &s, &c);
+    }
+};
+
+// ---- batched column pass -------------------------------------------------------------------
+// Block (cb, r): column block cb of tower run r (consecutive tasks of one tower
+// inside the batch).  For every task: load the column block of its subgrid
+// (rows not flagged are zero and are not read), inverse FFT, multiply by
+// exp(-2 pi i wp winc base), accumulate; then add the sum into the tower's
+// subimage and leave the subgrid rows it read zeroed.
+template <typename T, int LOGN, int CB>
+__global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
+    typename Vec2<T>::type *G, const uint32_t *flags, typename Vec2<T>::type *subimg,
+    const int32_t *run_first, const int32_t *run_count, int run_base, int32_t t0,
+    const int32_t *task_wp, const int32_t *task_tower, const typename Vec2<T>::type *__restrict__ twg,
+    double theta, double winc)
+{
+    using T2 = typename Vec2<T>::type;
+    constexpr int n = 1 << LOGN;
+    constexpr int NT = CB * (n / 8);
+    constexpr int PER = n * CB / NT; // 8
+    constexpr int FW = (n + 31) / 32;
+    __shared__ T2 lds[n * CB];
+    __shared__ T2 tw[n];
+    for (int k = threadIdx.x; k < n; k += NT)
+        tw[k] = twg[k];
+    const int r = run_base + blockIdx.y;
+    const int first = run_first[r], count = run_count[r];
+    const int tower = task_tower[first];
+    const int c0 = blockIdx.x * CB;
+
+    double base[PER];
+    T2 acc[PER];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = threadIdx.x + q * NT;
+        const int row = e / CB, col = c0 + (e - (e / CB) * CB);
+        base[q] = screen_base(col, row, n, theta);
+        acc[q].x = T(0);
+        acc[q].y = T(0);
+    }
+    for (int k = first; k < first + count; ++k) {
+        const int slot = k - t0;
+        T2 *g = G + (size_t)slot * n * n + c0;
+        const uint32_t *f = flags + (size_t)slot * FW;
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = threadIdx.x + q * NT;
+            const int row = e / CB, col = e - row * CB;
+            T2 val;
+            val.x = T(0);
+            val.y = T(0);
+            if ((f[row >> 5] >> (row & 31)) & 1u) {
+                val = g[(size_t)row * n + col];
+                T2 z;
+                z.x = T(0);
+                z.y = T(0);
+                g[(size_t)row * n + col] = z;
+            }
+            lds[e] = val;
+        }
+        __syncthreads();
+        fft_block<T2, LOGN, CB, CB, true>(lds, tw);
+        const double ph0 = -(double)task_wp[k] * winc;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int e = threadIdx.x + q * NT;
+            T c, s;
+            Trig<T>::rot(ph0 * base[q], c, s);
+            const T2 val = lds[e];
+            acc[q].x += val.x * c - val.y * s;
+            acc[q].y += val.x * s + val.y * c;
+        }
+    }
+    T2 *si = subimg + (size_t)tower * n * n + c0;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int e = threadIdx.x + q * NT;
+        const int row = e / CB, col = e - row * CB;
+        T2 cur = si[(size_t)row * n + col];
+        cur.x += acc[q].x;
+        cur.y += acc[q].y;
+        si[(size_t)row * n + col] = cur;
+    }
+}
+
+// ---- fold ------------------------------------------------------------------------------------
+// Owner computes: each master pixel sums the towers whose window covers it.
+template <typename T2>
+__global__ __launch_bounds__(256) void k_fold(double2 *grid, int N, const T2 *subimg,
+                                              const int32_t *tower_rank, int n, int margin,
+                                              int chunk, int cc)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)N * N)
+        return;
+    const int py = (int)(idx / N), px = (int)(idx - (int64_t)py * N);
+    const int x = px - N / 2, y = py - N / 2;
+    const double norm = (double)n * (double)n;
+    double re = 0.0, im = 0.0;
+    bool any = false;
+    const int ax = x + N / 2 + margin / 2, ay = y + N / 2 + margin / 2;
+    const int cxh = min(cc - 1, ax / chunk), cyh = min(cc - 1, ay / chunk);
+    for (int cy = max(0, cyh - 2); cy <= cyh; ++cy) {
+        const int y0 = chunk * cy - N / 2 - margin / 2;
+        if (y < y0 || y >= y0 + n)
+            continue;
+        for (int cx = max(0, cxh - 2); cx <= cxh; ++cx) {
+            const int x0 = chunk * cx - N / 2 - margin / 2;
+            if (x < x0 || x >= x0 + n)
+                continue;
+            const int rk = tower_rank[cy * cc + cx];
+            if (rk < 0)
+                continue;
+            const T2 s = subimg[(size_t)rk * n * n + (size_t)(y - y0) * n + (x - x0)];
+            re += (double)s.x / norm;
+            im += (double)s.y / norm;
+            any = true;
+        }
+    }
+    if (any) {
+        double2 g = grid[idx];
+        g.x += re;
+        g.y += im;
+        grid[idx] = g;
+    }
+}
+
+// ---- plain-mode kernels ---------------------------------------------------------------------
+
+// Romein scatter of one task's records, launched <<<128, 128>>> like
+// scatter_grid_kernel_flat in wtowers_CUDA_flat (grid_cuda.cu:708).
+__global__ __launch_bounds__(128) void k_tower_plain_scatter(const Rec<double> *__restrict__ recs,
+                                                             uint32_t start, uint32_t count,
+                                                             int nbins,
+                                                             const double2 *__restrict__ kern,
+                                                             int S, double2 *sub, int n)
+{
+    const uint32_t per = count / nbins;
+    const uint32_t i0 = start + per * blockIdx.x;
+    const uint32_t i1 = (blockIdx.x == (unsigned)nbins - 1) ? start + count : i0 + per;
+    const int SS = S * S;
+    for (int pos = threadIdx.x; pos < SS; pos += blockDim.x) {
+        const int myU = pos % S, myV = pos / S;
+        int cu = INT_MIN, cv = INT_MIN;
+        double sr = 0.0, si = 0.0;
+        for (uint32_t i = i0; i < i1; ++i) {
+            const Rec<double> r = recs[i];
+            const int gx = (int)(r.pos & 0xffffu) - S, gy = (int)(r.pos >> 16) - S;
+            int conv_u = (myU - gx) % S, conv_v = (myV - gy) % S;
+            if (conv_u < 0)
+                conv_u += S;
+            if (conv_v < 0)
+                conv_v += S;
+            const int gu = gx + conv_u, gv = gy + conv_v;
+            if (gu != cu || gv != cv) {
+                if (cu != INT_MIN && (unsigned)cu < (unsigned)n && (unsigned)cv < (unsigned)n) {
+                    double *g = reinterpret_cast<double *>(sub + (size_t)cv * n + cu);
+                    atomicAdd(g, sr);
+                    atomicAdd(g + 1, si);
+                }
+                sr = si = 0.0;
+                cu = gu;
+                cv = gv;
+            }
+            const double2 k = kern[(size_t)r.kidx * SS + (size_t)conv_v * S + conv_u];
+            sr += k.x * r.vis.x + k.y * r.vis.y;
+            si += k.x * r.vis.y - k.y * r.vis.x;
+        }
+        if (cu != INT_MIN && (unsigned)cu < (unsigned)n && (unsigned)cv < (unsigned)n) {
+            double *g = reinterpret_cast<double *>(sub + (size_t)cv * n + cu);
+            atomicAdd(g, sr);
+            atomicAdd(g + 1, si);
+        }
+    }
+}
+
+// subimg = F^k * subimg + subgrid (fresnel_subimg_mul, common_kernels.cu:13-28,
+// with the power taken as grid.c:475 does for skipped floors).
+__global__ __launch_bounds__(256) void k_fresnel_mac(const double2 *sub, double2 *img, int n,
+                                                     int k, double theta, double winc)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)n * n)
+        return;
+    const int y = (int)(idx / n), x = (int)(idx - (int64_t)y * n);
+    double c, s;
+    Trig<double>::rot((double)k * winc * screen_base(x, y, n, theta), c, s);
+    const double2 a = img[idx], b = sub[idx];
+    img[idx] = make_double2(a.x * c - a.y * s + b.x, a.x * s + a.y * c + b.y);
+}
+
+// subimg /= F^last_wp (w0_transfer_kernel, common_kernels.cu:31-40)
+__global__ __launch_bounds__(256) void k_w0_transfer(double2 *img, int n, int last_wp, double theta,
+                                                     double winc)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)n * n)
+        return;
+    const int y = (int)(idx / n), x = (int)(idx - (int64_t)y * n);
+    double c, s;
+    Trig<double>::rot(-(double)last_wp * winc * screen_base(x, y, n, theta), c, s);
+    const double2 a = img[idx];
+    img[idx] = make_double2(a.x * c - a.y * s, a.x * s + a.y * c);
+}
+
+// ---- host -------------------------------------------------------------------------------------
+
+struct TaskList {
+    std::vector<int32_t> slot, tower_rank, wp;
+    std::vector<uint32_t> item0, item1, nvis;
+    std::vector<int32_t> tower_of_rank; // tower index of each active tower
+    std::vector<int32_t> rank_of_tower; // -1 when inactive
+};
+
+template <typename T, int LOGN>
+static int launch_cols_t(typename Vec2<T>::type *G, const uint32_t *flags,
+                         typename Vec2<T>::type *subimg, const int32_t *rf, const int32_t *rc,
+                         int run_base, int nruns, int32_t t0, const int32_t *twp,
+                         const int32_t *ttw, const void *tw, double theta, double winc,
+                         hipStream_t s)
+{
+    constexpr int n = 1 << LOGN;
+    constexpr int CB0 = sizeof(T) == 4 ? 16 : 8;
+    constexpr int CB = CB0 > n ? n : CB0;
+    hipLaunchKernelGGL((k_tower_cols<T, LOGN, CB>), dim3(n / CB, nruns), dim3(CB * (n / 8)), 0, s,
+                       G, flags, subimg, rf, rc, run_base, t0, twp, ttw,
+                       (const typename Vec2<T>::type *)tw, theta, winc);
+    RG_HIP(hipGetLastError());
+    return RG_OK;
+}
+
+template <typename T>
+static int launch_cols(int n, typename Vec2<T>::type *G, const uint32_t *flags,
+                       typename Vec2<T>::type *subimg, const int32_t *rf, const int32_t *rc,
+                       int run_base, int nruns, int32_t t0, const int32_t *twp, const int32_t *ttw,
+                       const void *tw, double theta, double winc, hipStream_t s)
+{
+    switch (n) {
+    case 32: return launch_cols_t<T, 5>(G, flags, subimg, rf, rc, run_base, nruns, t0, twp, ttw, tw, theta, winc, s);
+    case 64: return launch_cols_t<T, 6>(G, flags, subimg, rf, rc, run_base, nruns, t0, twp, ttw, tw, theta, winc, s);
+    case 128: return launch_cols_t<T, 7>(G, flags, subimg, rf, rc, run_base, nruns, t0, twp, ttw, tw, theta, winc, s);
+    case 256: return launch_cols_t<T, 8>(G, flags, subimg, rf, rc, run_base, nruns, t0, twp, ttw, tw, theta, winc, s);
+    case 512: return launch_cols_t<T, 9>(G, flags, subimg, rf, rc, run_base, nruns, t0, twp, ttw, tw, theta, winc, s);
+    default: return fail(RG_ERR_UNSUPPORTED, "batched W-towers supports subgrid sizes 32..512 (got %d)", n);
+    }
+}
+
+template <typename T, int S, int TILE>
+static int launch_tower_tiles(const BinResult &br, uint32_t item0, uint32_t item1,
+                              const rg_wkern *wk, const TileSink &sink, hipStream_t s)
+{
+    if (item1 > item0)
+        hipLaunchKernelGGL((k_tiles<T, S, TILE, false, T>), dim3(item1 - item0), dim3(S * S), 0, s,
+                           (const Rec<T> *)br.recs, (const Item *)br.items, item0, item1,
+                           (const unsigned long long *)nullptr,
+                           (const typename Vec2<T>::type *)wk->data, sink);
+    RG_HIP(hipGetLastError());
+    return RG_OK;
+}
+
+template <typename T>
+static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis,
+                     const rg_wkern *wk, void *grid, hipStream_t s)
+{
+    using T2 = typename Vec2<T>::type;
+    const int N = p->grid_size, n = p->subgrid_size, M = p->subgrid_margin, S = wk->size;
+    const int chunk = n - M;
+    const int cc = N / chunk + 1; // grid_cuda.cu:537-539
+    const bool plain = p->mode == RG_MODE_PLAIN;
+    const int TILE = plain ? 0 : (p->tile > 0 ? p->tile : 32);
+    if (!plain && TILE != 32)
+        return fail(RG_ERR_UNSUPPORTED, "batched W-towers tile must be 32 (got %d)", TILE);
+    if (!plain && S != 8 && S != 16)
+        return fail(RG_ERR_UNSUPPORTED, "batched W-towers supports kernel sizes 8 and 16");
+    if (!plain && n < 32)
+        return fail(RG_ERR_UNSUPPORTED, "batched W-towers needs subgrid size >= 32 (got %d)", n);
+    if (n + 2 * S >= 65536)
+        return fail(RG_ERR_UNSUPPORTED, "subgrid too large");
+
+    RG_TRY(zero_fill(grid, sizeof(double2) * (size_t)N * N, s));
+    if (vis->n == 0)
+        return RG_OK;
+
+    // floor range (one small read-back)
+    RG_TRY(ctx->scratch[8].ensure(64));
+    int *d_range = ctx->scratch[8].as<int>();
+    hipLaunchKernelGGL(k_range_init, dim3(1), dim3(1), 0, s, d_range);
+    {
+        int64_t blocks = (vis->n + 255) / 256;
+        if (blocks > 4096)
+            blocks = 4096;
+        hipLaunchKernelGGL(k_floor_range, dim3((unsigned)blocks), dim3(256), 0, s, vis->u, vis->v,
+                           vis->w, vis->n, p->theta, N, chunk, cc, p->winc, d_range);
+        RG_HIP(hipGetLastError());
+    }
+    int h_range[2];
+    RG_HIP(hipMemcpyAsync(h_range, d_range, sizeof(h_range), hipMemcpyDeviceToHost, s));
+    RG_HIP(hipStreamSynchronize(s));
+    if (h_range[0] > h_range[1]) {
+        // every visibility outside the tower set
+        unsigned long long cnt[CNT__N] = {0};
+        cnt[CNT_DROPPED] = (unsigned long long)vis->n;
+        RG_HIP(hipMemcpyAsync(ctx->counters.p, cnt, sizeof(cnt), hipMemcpyHostToDevice, s));
+        RG_HIP(hipStreamSynchronize(s));
+        return RG_OK;
+    }
+    const int wp_min = h_range[0], nwp = h_range[1] - h_range[0] + 1;
+
+    TowerKeyer kf;
+    kf.u = vis->u;
+    kf.v = vis->v;
+    kf.w = vis->w;
+    kf.vis = vis->vis;
+    kf.vis_type = vis->vis_type;
+    kf.N = N;
+    kf.S = S;
+    kf.R = wk->oversampling;
+    kf.P = wk->plane_count;
+    kf.theta = p->theta;
+    kf.w_min = wk->w_min;
+    kf.w_step = wk->w_step;
+    kf.n = n;
+    kf.chunk = chunk;
+    kf.cc = cc;
+    kf.winc = p->winc;
+    kf.wp_min = wp_min;
+    kf.nwp = nwp;
+    kf.tile = TILE;
+    kf.ntx = TILE > 0 ? (n + S + TILE - 1) / TILE : 1;
+    kf.ntiles = kf.ntx * kf.ntx;
+    const int64_t nslots = (int64_t)cc * cc * nwp;
+    const int64_t nkeys = nslots * kf.ntiles;
+    const uint32_t per_item = p->vis_per_item > 0 ? (uint32_t)p->vis_per_item : 2048u;
+
+    BinResult br;
+    RG_TRY((bin_visibilities<TowerKeyer, T>(ctx, kf, vis->n, nkeys, per_item, s, br)));
+
+    // task table (read back)
+    RG_TRY(ctx->scratch[9].ensure(sizeof(uint32_t) * 2 * nslots));
+    uint32_t *d_nvis = ctx->scratch[9].as<uint32_t>();
+    uint32_t *d_item0 = d_nvis + nslots;
+    hipLaunchKernelGGL(k_task_info, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s,
+                       (const uint32_t *)br.vis_start, (const uint32_t *)br.item_start, nslots,
+                       kf.ntiles, d_nvis, d_item0);
+    RG_HIP(hipGetLastError());
+    std::vector<uint32_t> h_info(2 * nslots);
+    RG_HIP(hipMemcpyAsync(h_info.data(), d_nvis, sizeof(uint32_t) * 2 * nslots,
+                          hipMemcpyDeviceToHost, s));
+    RG_HIP(hipStreamSynchronize(s));
+
+    TaskList tl;
+    tl.rank_of_tower.assign((size_t)cc * cc, -1);
+    std::vector<int32_t> slot_rank(nslots, -1);
+    for (int64_t t = 0; t < nslots; ++t) {
+        const uint32_t nv = h_info[t];
+        if (!nv)
+            continue;
+        const int tower = (int)(t / nwp);
+        if (tl.rank_of_tower[tower] < 0) {
+            tl.rank_of_tower[tower] = (int32_t)tl.tower_of_rank.size();
+            tl.tower_of_rank.push_back(tower);
+        }
+        slot_rank[t] = (int32_t)tl.slot.size();
+        tl.slot.push_back((int32_t)t);
+        tl.tower_rank.push_back(tl.rank_of_tower[tower]);
+        tl.wp.push_back(wp_min + (int)(t % nwp));
+        tl.item0.push_back(h_info[nslots + t]);
+        tl.nvis.push_back(nv);
+    }
+    const int ntasks = (int)tl.slot.size();
+    const int ntowers = (int)tl.tower_of_rank.size();
+    // end of each task's item range: next slot's first item
+    tl.item1.resize(ntasks);
+    for (int k = 0; k < ntasks; ++k)
+        tl.item1[k] = (tl.slot[k] + 1 < nslots) ? h_info[nslots + tl.slot[k] + 1] : 0u;
+    if (ntasks > 0 && tl.slot[ntasks - 1] + 1 >= nslots) {
+        uint32_t last = 0;
+        RG_HIP(hipMemcpyAsync(&last, br.item_start + nkeys, sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, s));
+        RG_HIP(hipStreamSynchronize(s));
+        tl.item1[ntasks - 1] = last;
+    }
+    {
+        unsigned long long tc[2] = {(unsigned long long)ntasks, (unsigned long long)ntowers};
+        RG_HIP(hipMemcpyAsync(ctx->counters.as<unsigned long long>() + CNT_TASKS, tc, sizeof(tc),
+                              hipMemcpyHostToDevice, s));
+    }
+    if (ntasks == 0)
+        return RG_OK;
+
+    // subimages (one per active tower)
+    const size_t nn = (size_t)n * n;
+    RG_TRY(ctx->scratch[10].ensure(sizeof(T2) * nn * ntowers));
+    T2 *subimg = ctx->scratch[10].as<T2>();
+    RG_TRY(zero_fill(subimg, sizeof(T2) * nn * ntowers, s));
+
+    const void *tw_inv = nullptr;
+    RG_TRY(get_twiddles(ctx, n, sizeof(T) == 4 ? RG_C64 : RG_C128, +1, &tw_inv));
+
+    // device tables: rank_of_tower, slot_rank, task_wp, task_tower, run tables
+    const int Q = plain ? 1 : (p->floor_batch > 0 ? p->floor_batch : 128);
+    std::vector<int32_t> run_first, run_count, batch_run0;
+    for (int t0 = 0; t0 < ntasks; t0 += Q) {
+        const int t1 = std::min(ntasks, t0 + Q);
+        batch_run0.push_back((int32_t)run_first.size());
+        for (int k = t0; k < t1;) {
+            int k1 = k;
+            while (k1 < t1 && tl.tower_rank[k1] == tl.tower_rank[k])
+                ++k1;
+            run_first.push_back(k);
+            run_count.push_back(k1 - k);
+            k = k1;
+        }
+    }
+    batch_run0.push_back((int32_t)run_first.size());
+    const size_t nruns = run_first.size();
+    std::vector<int32_t> tables;
+    const size_t o_rot = 0, o_sr = o_rot + (size_t)cc * cc, o_wp = o_sr + nslots,
+                 o_tt = o_wp + ntasks, o_rf = o_tt + ntasks, o_rc = o_rf + nruns,
+                 o_end = o_rc + nruns;
+    tables.resize(o_end);
+    std::copy(tl.rank_of_tower.begin(), tl.rank_of_tower.end(), tables.begin() + o_rot);
+    std::copy(slot_rank.begin(), slot_rank.end(), tables.begin() + o_sr);
+    std::copy(tl.wp.begin(), tl.wp.end(), tables.begin() + o_wp);
+    std::copy(tl.tower_rank.begin(), tl.tower_rank.end(), tables.begin() + o_tt);
+    std::copy(run_first.begin(), run_first.end(), tables.begin() + o_rf);
+    std::copy(run_count.begin(), run_count.end(), tables.begin() + o_rc);
+    RG_TRY(ctx->scratch[8].ensure(64 + sizeof(int32_t) * o_end));
+    int32_t *d_tab = reinterpret_cast<int32_t *>(ctx->scratch[8].as<char>() + 64);
+    RG_HIP(hipMemcpyAsync(d_tab, tables.data(), sizeof(int32_t) * o_end, hipMemcpyHostToDevice, s));
+
+    const int FW = (n + 31) / 32;
+    if (plain) {
+        // reference structure: one task at a time
+        RG_TRY(ctx->scratch[7].ensure(sizeof(double2) * nn));
+        double2 *sub = ctx->scratch[7].as<double2>();
+        const Rec<double> *recs = (const Rec<double> *)br.recs;
+        // keys are task slots here, so a task's records start at the running
+        // sum of the counts of the slots before it
+        std::vector<uint32_t> start_of(ntasks);
+        {
+            uint32_t run = 0;
+            int k2 = 0;
+            for (int64_t t = 0; t < nslots; ++t) {
+                if (k2 < ntasks && tl.slot[k2] == t)
+                    start_of[k2++] = run;
+                run += h_info[t];
+            }
+        }
+        const int blocks_px = (int)((nn + 255) / 256);
+        int k = 0;
+        for (int r = 0; r < ntowers; ++r) {
+            double2 *img = (double2 *)subimg + (size_t)r * nn;
+            int last_wp = 0;
+            bool first = true;
+            for (; k < ntasks && tl.tower_rank[k] == r; ++k) {
+                RG_TRY(zero_fill(sub, sizeof(double2) * nn, s));
+                hipLaunchKernelGGL(k_tower_plain_scatter, dim3(128), dim3(128), 0, s, recs,
+                                   start_of[k],
+                                   tl.nvis[k], 128, (const double2 *)wk->data, S, sub, n);
+                RG_HIP(hipGetLastError());
+                RG_TRY(fft2d(ctx, sub, RG_C128, n, 1, +1, s));
+                if (first) {
+                    last_wp = tl.wp[k];
+                    first = false;
+                }
+                hipLaunchKernelGGL(k_fresnel_mac, dim3(blocks_px), dim3(256), 0, s,
+                                   (const double2 *)sub, img, n, tl.wp[k] - last_wp, p->theta,
+                                   p->winc);
+                RG_HIP(hipGetLastError());
+                last_wp = tl.wp[k];
+            }
+            if (last_wp != 0) {
+                hipLaunchKernelGGL(k_w0_transfer, dim3(blocks_px), dim3(256), 0, s, img, n, last_wp,
+                                   p->theta, p->winc);
+                RG_HIP(hipGetLastError());
+            }
+            RG_TRY(fft2d(ctx, img, RG_C128, n, 1, -1, s));
+        }
+    } else {
+        RG_TRY(ctx->scratch[7].ensure(sizeof(T2) * nn * Q + sizeof(uint32_t) * FW * Q + 256));
+        T2 *G = ctx->scratch[7].as<T2>();
+        uint32_t *flags = reinterpret_cast<uint32_t *>(G + nn * Q);
+        RG_TRY(zero_fill(G, sizeof(T2) * nn * Q, s));
+        int b = 0;
+        for (int t0 = 0; t0 < ntasks; t0 += Q, ++b) {
+            const int t1 = std::min(ntasks, t0 + Q);
+            RG_TRY(zero_fill(flags, sizeof(uint32_t) * FW * Q, s));
+            TileSink sink;
+            sink.base = G;
+            sink.gsize = n;
+            sink.ntx = kf.ntx;
+            sink.ntiles = kf.ntiles;
+            sink.group_map = d_tab + o_sr;
+            sink.group_offset = t0;
+            sink.row_flags = flags;
+            const uint32_t i0 = tl.item0[t0], i1 = tl.item1[t1 - 1];
+            {
+                KernelTimer kt(ctx, RG_TIMER_TOWER_TILES, s);
+                if (S == 16)
+                    RG_TRY((launch_tower_tiles<T, 16, 32>(br, i0, i1, wk, sink, s)));
+                else
+                    RG_TRY((launch_tower_tiles<T, 8, 32>(br, i0, i1, wk, sink, s)));
+            }
+            {
+                KernelTimer kt(ctx, RG_TIMER_TOWER_ROWS, s);
+                RG_TRY((fft_rows_dispatch<T2, true>(G, n, (int64_t)n * (t1 - t0),
+                                                    (const T2 *)tw_inv, flags, s)));
+            }
+            const int rb = batch_run0[b], nr = batch_run0[b + 1] - batch_run0[b];
+            {
+                KernelTimer kt(ctx, RG_TIMER_TOWER_COLS, s);
+                RG_TRY(launch_cols<T>(n, G, flags, subimg, d_tab + o_rf, d_tab + o_rc, rb, nr, t0,
+                                      d_tab + o_wp, d_tab + o_tt, tw_inv, p->theta, p->winc, s));
+            }
+        }
+    }
+
+    KernelTimer kt_finish(ctx, RG_TIMER_TOWER_FINISH, s);
+    if (!plain)
+        RG_TRY(fft2d(ctx, subimg, sizeof(T) == 4 ? RG_C64 : RG_C128, n, ntowers, -1, s));
+    const int64_t npx = (int64_t)N * N;
+    hipLaunchKernelGGL((k_fold<T2>), dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, s,
+                       (double2 *)grid, N, (const T2 *)subimg, (const int32_t *)(d_tab + o_rot), n,
+                       M, chunk, cc);
+    RG_HIP(hipGetLastError());
+    return RG_OK;
+}
+
+int wtowers_run(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis,
+                const rg_wkern *wk, void *grid, int grid_type, hipStream_t s)
+{
+    if (grid_type != RG_C128)
+        return fail(RG_ERR_PARAM, "W-towers folds into an FP64 (RG_C128) master grid");
+    if (p->mode == RG_MODE_PLAIN) {
+        if (vis->vis_type != RG_C128 || wk->type != RG_C128)
+            return fail(RG_ERR_PARAM, "plain mode is FP64: visibilities and kernels must be RG_C128");
+        return wtowers_t<double>(ctx, p, vis, wk, grid, s);
+    }
+    if (wk->type == RG_C64)
+        return wtowers_t<float>(ctx, p, vis, wk, grid, s);
+    return wtowers_t<double>(ctx, p, vis, wk, grid, s);
+}
+
+} // namespace rg

@@ -1,0 +1,77 @@
+"""W-towers on the GPU (plain and batched) against the oracle."""
+import numpy as np
+import pytest
+import torch
+
+import pyoracle as orc
+import rgamd
+import synth
+from gpu_util import kern, rel_max, rel_rms, small_obs, stage
+
+pytestmark = pytest.mark.gpu
+
+SUB, MARGIN, WINC = 64, 16, 10.0
+
+
+@pytest.fixture(scope="module")
+def case():
+    obs, vis = small_obs(n_st=14, n_dumps=32, arm_m=700.0, lam=2560.0)  # N = 256
+    w_step = 1.0
+    table = rgamd.make_wkernels(11, 8, 16, obs.theta, -5.0, w_step)
+    ref, fl, bad = orc.grid_wtowers(obs.grid_size, obs.theta, obs.u, obs.v, obs.w, vis, table,
+                                    -5.0, w_step, SUB, MARGIN, WINC, nthreads=8)
+    assert bad == 0 and fl == 8 * 256 * obs.n_vis
+    return obs, vis, table, ref
+
+
+def run(ctx, obs, vis, table, mode, dt=torch.complex128, **kw):
+    _, fv = stage(obs.u, obs.v, obs.w, vis, dt)
+    kt, wk = kern(table, -5.0, 1.0, dt)
+    N = obs.grid_size
+    g = torch.empty((N, N), dtype=torch.complex128, device="cuda")
+    ctx.grid(rgamd.params(N, obs.theta, mode=mode, subgrid=SUB, margin=MARGIN, winc=WINC, **kw),
+             fv, wk, g)
+    st = ctx.stats()
+    torch.cuda.synchronize()
+    return g.cpu().numpy(), st
+
+
+def test_plain_fp64_matches_oracle(ctx, case):
+    obs, vis, table, ref = case
+    g, st = run(ctx, obs, vis, table, rgamd.RG_MODE_PLAIN)
+    assert rel_max(g, ref) < 1e-11
+    assert st.n_tasks > 1 and st.n_towers >= 1
+
+
+@pytest.mark.parametrize("floor_batch", [1, 5, 128])
+def test_batched_fp64_matches_oracle(ctx, case, floor_batch):
+    obs, vis, table, ref = case
+    g, st = run(ctx, obs, vis, table, rgamd.RG_MODE_BATCHED, floor_batch=floor_batch,
+                vis_per_item=50)
+    assert rel_max(g, ref) < 1e-11
+    assert st.n_gridded == obs.n_vis and st.n_dropped == 0
+
+
+def test_batched_fp32_within_tolerance(ctx, case):
+    obs, vis, table, ref = case
+    g, _ = run(ctx, obs, vis, table, rgamd.RG_MODE_BATCHED, dt=torch.complex64)
+    img, img_ref = orc.grid_to_image(g), orc.grid_to_image(ref)
+    assert rel_max(img, img_ref) < 1e-5 and rel_rms(img, img_ref) < 1e-5
+
+
+def test_task_count_matches_binning(ctx, case):
+    obs, vis, table, ref = case
+    _, st = run(ctx, obs, vis, table, rgamd.RG_MODE_BATCHED)
+    N, chunk = obs.grid_size, SUB - MARGIN
+    tasks = {(orc.tower_index(obs.u[i], obs.theta, N, chunk),
+              orc.tower_index(obs.v[i], obs.theta, N, chunk),
+              orc.floor_index(obs.w[i], WINC)) for i in range(obs.n_vis)}
+    assert st.n_tasks == len(tasks)
+    assert st.n_towers == len({(a, b) for a, b, _ in tasks})
+
+
+def test_fp64_required_for_plain(ctx, case):
+    obs, vis, table, ref = case
+    with pytest.raises(rgamd.RGError) as e:
+        run(ctx, obs, vis, table, rgamd.RG_MODE_PLAIN, dt=torch.complex64)
+    assert e.value.code == rgamd.RG_ERR_PARAM
