@@ -10,7 +10,8 @@ the timed region; a step is one full gridding pass (sort, scatter, subgrid
 FFTs and w-screens, fold) over all visibilities.  The final N^2 image FFT is
 not part of a step.
 
-Prints ONE JSON line.  value = FP32 batched W-towers Mvis/s.
+Prints ONE JSON line.  value = FP32 batched W-projection Mvis/s; the W-towers
+numbers of the same stream are reported alongside under results.
 """
 import argparse
 import json
@@ -43,6 +44,11 @@ def main():
     ap.add_argument("--skip-plain", action="store_true", help="skip the FP64 plain runs")
     ap.add_argument("--only", choices=["all", "wproj", "wtowers"], default="all")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--calibrate", action="store_true", help="run a 1 GiB copy for counter scaling")
+    ap.add_argument("--dft-pixels", type=int, default=65536,
+                    help="seeded pixels of the FP64 DFT oracle comparison (0 = skip)")
+    ap.add_argument("--tower-band", type=int, default=0, help="W-towers band height (8 or 16)")
+    ap.add_argument("--floor-batch", type=int, default=0, help="W-towers tasks per batch")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the gridding path is "
@@ -86,6 +92,14 @@ def main():
     ctx = rgamd.Context(0)
     ctx.set_timing(True)
     stream = torch.cuda.current_stream()
+    if args.calibrate:
+        # a plain 1 GiB device copy: the reference dispatch for scaling the
+        # FETCH_SIZE / WRITE_SIZE counters of the gridding kernels
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     def timed(label, p, fv, wk, grid, steps, warmup):
         for _ in range(warmup):
@@ -125,6 +139,7 @@ def main():
                 "rms_rel": float(torch.sqrt((d ** 2).mean()) / torch.sqrt((b.abs() ** 2).mean()))}
 
     res = {}
+    dft_images = {}
     K, W = args.steps, args.warmup
     if args.only in ("all", "wproj"):
         g64 = torch.empty((N, N), dtype=torch.complex128, device="cuda")
@@ -141,7 +156,10 @@ def main():
                                   K, W)
         if not args.skip_plain:
             ctx.grid(rgamd.params(N, theta, rgamd.RG_MODE_PLAIN), fv128, k128_p, g64)
-        r["error_fp32_batched_vs_fp64"] = errs(image(g32), image(g64))
+        i32, i64 = image(g32), image(g64)
+        r["error_fp32_batched_vs_fp64"] = errs(i32, i64)
+        dft_images["wproj_fp64_plain" if not args.skip_plain else "wproj_fp64_batched"] = i64
+        dft_images["wproj_fp32_batched"] = i32
         torch.cuda.synchronize()
         res["wproj"] = r
         del g32
@@ -150,17 +168,25 @@ def main():
         g32 = torch.empty((N, N), dtype=torch.complex128, device="cuda")
         r = {}
         pp = rgamd.params(N, theta, rgamd.RG_MODE_PLAIN, subgrid=SUB, margin=MARGIN, winc=WINC)
-        pb = rgamd.params(N, theta, rgamd.RG_MODE_BATCHED, subgrid=SUB, margin=MARGIN, winc=WINC)
+        pb = rgamd.params(N, theta, rgamd.RG_MODE_BATCHED, subgrid=SUB, margin=MARGIN, winc=WINC,
+                          tile=args.tower_band, floor_batch=args.floor_batch)
         if not args.skip_plain:
             r["fp64_plain"] = timed("W-towers FP64 plain", pp, fv128, k128_t, g64, 1, 0)
         else:
             ctx.grid(pb, fv128, k128_t, g64)
         r["fp32_batched"] = timed("W-towers FP32 batched", pb, fv64, k64_t, g32, K, W)
-        r["error_fp32_batched_vs_fp64"] = errs(image(g32), image(g64))
+        i32, i64 = image(g32), image(g64)
+        r["error_fp32_batched_vs_fp64"] = errs(i32, i64)
+        dft_images["wtowers_fp64" + ("_plain" if not args.skip_plain else "_batched")] = i64
+        dft_images["wtowers_fp32_batched"] = i32
         res["wtowers"] = r
 
-    # headline: FP32 batched W-towers (W-projection when only that ran)
-    head_kind = "wtowers" if "wtowers" in res else "wproj"
+    dft = None
+    if args.dft_pixels > 0 and dft_images:
+        dft = dft_errors(ctx, fv128, N, obs.lam, theta, S, dft_images, args.dft_pixels)
+
+    # headline: FP32 batched W-projection (W-towers when only that ran)
+    head_kind = "wproj" if "wproj" in res else "wtowers"
     head = res[head_kind]["fp32_batched"]
     plain = res[head_kind].get("fp64_plain")
     # dominant kernel roofline (algorithmic bytes per launch / event-timed duration)
@@ -205,17 +231,56 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32 (FP64 index math, FP64 fold)",
         "data": "synthetic (seeded layout, Earth-rotation tracks, 50 point sources)",
-        "config": {"workload": "C2: 512 stations Exp(3 km), 64 dumps/4 h, dec -30, 150 MHz, bl<10 km, "
-                               f"{nv} vis, theta 0.1, lambda 20480 (2048^2), S16 R8, "
-                               "W-towers 512/16/10" if head_kind == "wtowers" else "W-projection",
+        "config": {"workload": ("512 stations Exp(3 km), 64 dumps over 4 h, dec -30, 150 MHz, "
+                                f"baselines < 10 km: {nv} visibilities gridded on 2048^2 "
+                                "(theta 0.1, lambda 20480), W-kernels S16 x R8 "
+                                + ("W-projection (w-step 10)" if head_kind == "wproj"
+                                   else "W-towers subgrid 512, margin 16, winc 10")),
                    "n_vis": nv, "grid": N, "path": head_kind},
         "speedup_vs_fp64_plain": (head["mvis_s"] / plain["mvis_s"]) if plain else None,
         "error_vs_fp64": res[head_kind]["error_fp32_batched_vs_fp64"],
+        "error_vs_dft": dft,
         "roofline": roof,
         "cpu_baseline": cpu,
         "results": res,
     }
     print(json.dumps(out), flush=True)
+
+
+def dft_errors(ctx, fv, N, lam, theta, S, images, npix, seed=4):
+    """Images against the FP64 direct DFT on `npix` seeded pixels, after the
+    layout map (DFT pixel (x, y) is gridder pixel ((N - x) % N, (N - y) % N))
+    and division by the taper A(l) A(m).  Reported for the whole image and the
+    inner field |l|, |m| < theta / 4."""
+    import torch
+
+    import rgamd
+
+    rng = np.random.default_rng(seed)
+    idx = rng.choice(N * N, size=min(npix, N * N), replace=False)
+    ys, xs = np.divmod(idx, N)
+    px = torch.from_numpy(xs.astype(np.int32)).cuda()
+    py = torch.from_numpy(ys.astype(np.int32)).cuda()
+    out = torch.empty(len(idx), dtype=torch.complex128, device="cuda")
+    t = time.perf_counter()
+    ctx.dft(fv, px, py, N, lam, out, rgamd.RG_DFT_FP64)
+    torch.cuda.synchronize()
+    t_dft = time.perf_counter() - t
+    d = out.cpu().numpy()
+    taper = rgamd.pswf_eval(math.pi * S / 4, 2.0 * (np.arange(N) - N // 2) / N)
+    r, c = (N - xs) % N, (N - ys) % N
+    inner = (np.abs(r - N // 2) < N // 8) & (np.abs(c - N // 2) < N // 8)
+    res = {"pixels": int(len(idx)), "dft_seconds": t_dft}
+    for name, img in images.items():
+        g = img.cpu().numpy()[r, c] / (taper[c] * taper[r])
+        e = np.abs(g - d)
+        res[name] = {"max_rel": float(e.max() / np.abs(d).max()),
+                     "rms_rel": float(np.sqrt(np.mean(e ** 2)) / np.sqrt(np.mean(np.abs(d) ** 2))),
+                     "inner_max_rel": float(e[inner].max() / np.abs(d[inner]).max()),
+                     "inner_rms_rel": float(np.sqrt(np.mean(e[inner] ** 2)) /
+                                            np.sqrt(np.mean(np.abs(d[inner]) ** 2)))}
+    log(f"[bench] DFT oracle ({len(idx)} pixels, {t_dft:.1f}s): {res}")
+    return res
 
 
 def cpu_baseline(obs, vis128, tab_p, w_min, w_step, seconds):

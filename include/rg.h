@@ -88,7 +88,8 @@ typedef struct rg_grid_params {
     int32_t subgrid_size;   /* W-towers n (0 = W-projection) */
     int32_t subgrid_margin; /* W-towers margin M */
     double winc;            /* W-towers floor spacing (wavelengths) */
-    int32_t tile;           /* batched tile edge in cells (0 = tuned default) */
+    int32_t tile;           /* batched W-projection: tile edge in cells (32 or 48, 0 = 32);
+                               batched W-towers: band height in rows (8 or 16, 0 = 8) */
     int32_t vis_per_item;   /* batched visibilities per work item (0 = default) */
     int32_t compensated;    /* batched FP32: 0 plain, 1 two-sum in the tile (default) */
     int32_t floor_batch;    /* batched W-towers: (tower, floor) tasks per batch (0 = default) */
@@ -187,6 +188,8 @@ int rg_predict_points(const double *u, const double *v, const double *w, int64_t
 /* Normalised zeroth prolate spheroidal wave function psi_0(c, x) / psi_0(c, 0),
  * |x| <= 1, from its Legendre expansion (host). */
 double rg_pswf(double c, double x);
+/* rg_pswf at n points (the expansion is computed once). */
+int rg_pswf_eval(double c, const double *x, int64_t n, double *out);
 
 /* W-kernel table generator (host, FP64): plane p holds the oversampled
  * kernel of the image-plane function A(l) A(m) exp(-2 pi i w (n - 1)),

@@ -148,6 +148,87 @@ __device__ inline void fft_stage(T2 *lds, const T2 *tw, int logns)
     __syncthreads();
 }
 
+// ---- line-major variant ---------------------------------------------------------
+// Lines stored contiguously (line l at base + l * LSTR); thread t of a round
+// works on line t / TPL, position t % TPL.  `active` threads do the work, all
+// threads reach the barriers.
+template <typename T2, int LOGN, int R, bool INV>
+__device__ inline void fft_stage_lm(T2 *line, const T2 *tw, int logns, int lt, bool active)
+{
+    constexpr int n = 1 << LOGN;
+    constexpr int TPL = n / 8;
+    constexpr int NBUT = n / R;
+    constexpr int PER = NBUT / TPL;
+    const int Ns = 1 << logns;
+    T2 v[8];
+    if (active) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int j = lt + q * TPL;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                v[q * R + r] = line[j + r * NBUT];
+        }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int j = lt + q * TPL;
+            const int k = j & (Ns - 1);
+            const int tstep = k * (n >> (logns + (R == 8 ? 3 : (R == 4 ? 2 : 1))));
+#pragma unroll
+            for (int r = 1; r < R; ++r)
+                v[q * R + r] = cmul(v[q * R + r], tw[r * tstep]);
+            if (R == 8)
+                dft8<INV>(&v[q * R]);
+            else if (R == 4)
+                dft4<INV>(v[q * R], v[q * R + 1], v[q * R + 2], v[q * R + 3]);
+            else
+                dft2<INV>(v[q * R], v[q * R + 1]);
+            const int d = (j >> logns) * (Ns * R) + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                line[d + r * Ns] = v[q * R + r];
+        }
+    }
+    __syncthreads();
+}
+
+// Transforms nl contiguous lines of length n = 2^LOGN (line stride LSTR) with
+// NT threads, B = NT / (n / 8) lines per round.
+template <typename T2, int LOGN, int NT, int LSTR, bool INV>
+__device__ inline void fft_lines_lm(T2 *lds, const T2 *tw, int nl)
+{
+    constexpr int n = 1 << LOGN;
+    constexpr int TPL = n / 8;
+    static_assert(TPL <= NT, "line longer than 8 x threads");
+    constexpr int B = NT / TPL;
+    const int lt = threadIdx.x % TPL;
+    for (int l0 = 0; l0 < nl; l0 += B) {
+        const int l = l0 + (int)threadIdx.x / TPL;
+        const bool active = (int)threadIdx.x < B * TPL && l < nl;
+        T2 *line = lds + (size_t)(active ? l : 0) * LSTR;
+        int logns = 0;
+#pragma unroll
+        for (int st = 0; st < 8; ++st) {
+            if (logns >= LOGN)
+                break;
+            const int rem = LOGN - logns;
+            if (rem >= 3) {
+                fft_stage_lm<T2, LOGN, 8, INV>(line, tw, logns, lt, active);
+                logns += 3;
+            } else if (rem == 2) {
+                fft_stage_lm<T2, LOGN, 4, INV>(line, tw, logns, lt, active);
+                logns += 2;
+            } else {
+                fft_stage_lm<T2, LOGN, 2, INV>(line, tw, logns, lt, active);
+                logns += 1;
+            }
+        }
+    }
+}
+
 // Full transform of the block (data already in lds, synchronised).
 template <typename T2, int LOGN, int B, int LS, bool INV>
 __device__ inline void fft_block(T2 *lds, const T2 *tw)

@@ -530,6 +530,18 @@ double rg_pswf(double c, double x)
     return pswf_eval(d, x) / pswf_eval(d, 0.0);
 }
 
+int rg_pswf_eval(double c, const double *x, int64_t n, double *out)
+{
+    if (n < 0 || (n > 0 && (!x || !out)))
+        return fail(RG_ERR_PARAM, "invalid arguments");
+    std::vector<double> d;
+    pswf_coeffs(c, d);
+    const double a0 = pswf_eval(d, 0.0);
+    for (int64_t i = 0; i < n; ++i)
+        out[i] = pswf_eval(d, x[i]) / a0;
+    return RG_OK;
+}
+
 int rg_wkern_generate(void *out_host, int32_t type, int32_t planes, int32_t R, int32_t S,
                       double theta, double w_min, double w_step, int32_t M, double c,
                       int32_t nthreads)

@@ -15,6 +15,7 @@
 #pragma once
 
 #include "binning.hpp"
+#include "romein.hpp"
 
 namespace rg {
 
@@ -78,36 +79,61 @@ __global__ __launch_bounds__(S *S) void k_tiles(const Rec<T> *__restrict__ recs,
     int cur = -1;
     T sr = T(0), si = T(0);
 
+    // Records are staged NT at a time and consumed U at a time: the U tap
+    // loads are issued together, then the U updates run in stream order.  A
+    // partial batch is padded with zero-weight records (tile corner 0).
+    constexpr int U = 8;
+    static_assert(NT % U == 0, "record batch must be a multiple of the unroll");
     for (uint32_t b = 0; b < item.count; b += NT) {
         const uint32_t nloc = min((uint32_t)NT, item.count - b);
+        const uint32_t nlocU = (nloc + U - 1) / U * U;
         __syncthreads();
-        if (threadIdx.x < nloc)
+        if (threadIdx.x < nloc) {
             s_rec[threadIdx.x] = recs[item.start + b + threadIdx.x];
-        __syncthreads();
-        for (uint32_t j = 0; j < nloc; ++j) {
-            const Rec<T> r = s_rec[j];
-            const int lx = (int)(r.pos & 0xffffu), ly = (int)(r.pos >> 16);
-            const int cx = (myU - lx) & (S - 1);
-            const int cy = (myV - ly) & (S - 1);
-            const int cell = (ly + cy) * RT + (lx + cx);
-            if (cell != cur) {
-                if (cur >= 0) {
-                    if (COMP) {
-                        two_sum_into(s_re[cur], s_lre[cur], sr);
-                        two_sum_into(s_im[cur], s_lim[cur], si);
-                    } else {
-                        s_re[cur] += sr;
-                        s_im[cur] += si;
-                    }
-                }
-                cur = cell;
-                sr = T(0);
-                si = T(0);
-            }
-            const typename Vec2<T>::type k = kern[(size_t)r.kidx * NT + cy * S + cx];
-            sr += k.x * r.vis.x + k.y * r.vis.y;
-            si += k.x * r.vis.y - k.y * r.vis.x;
+        } else if (threadIdx.x < nlocU) {
+            Rec<T> z;
+            z.pos = 0;
+            z.kidx = 0;
+            z.vis.x = T(0);
+            z.vis.y = T(0);
+            s_rec[threadIdx.x] = z;
         }
+        __syncthreads();
+        using B = RecBatch<T, U>;
+        auto fetch = [&](uint32_t j, B &bt) {
+#pragma unroll
+            for (int q = 0; q < U; ++q) {
+                const Rec<T> r = s_rec[j + q];
+                const int lx = (int)(r.pos & 0xffffu), ly = (int)(r.pos >> 16);
+                const int cx = (myU - lx) & (S - 1);
+                const int cy = (myV - ly) & (S - 1);
+                bt.cell[q] = (ly + cy) * RT + (lx + cx);
+                bt.v[q] = r.vis;
+                bt.k[q] = kern[r.kidx * (uint32_t)NT + (uint32_t)(cy * S + cx)];
+            }
+        };
+        auto update = [&](const B &bt) {
+#pragma unroll
+            for (int q = 0; q < U; ++q) {
+                if (bt.cell[q] != cur) {
+                    if (cur >= 0) {
+                        if (COMP) {
+                            two_sum_into(s_re[cur], s_lre[cur], sr);
+                            two_sum_into(s_im[cur], s_lim[cur], si);
+                        } else {
+                            s_re[cur] += sr;
+                            s_im[cur] += si;
+                        }
+                    }
+                    cur = bt.cell[q];
+                    sr = T(0);
+                    si = T(0);
+                }
+                sr += bt.k[q].x * bt.v[q].x + bt.k[q].y * bt.v[q].y;
+                si += bt.k[q].x * bt.v[q].y - bt.k[q].y * bt.v[q].x;
+            }
+        };
+        pipelined_records<U, B>(nlocU, fetch, update);
     }
     if (cur >= 0) {
         if (COMP) {

@@ -23,6 +23,7 @@
 
 #include "binning_impl.hpp"
 #include "fft_core.hpp"
+#include "romein.hpp"
 #include "tiles.hpp"
 
 namespace rg {
@@ -42,7 +43,9 @@ struct TowerKeyer {
     int n, chunk, cc;
     double winc;
     int wp_min, nwp;
-    int tile, ntx, ntiles; // tile == 0: key = task, pos = raw subgrid corner + S
+    // band_h == 0: key = task; else key = task * nk + (gy + S) / band_h.
+    // pos is always the raw subgrid corner + S: (gx + S) | (gy + S) << 16.
+    int band_h, nk;
 
     __device__ int classify(int64_t i, uint32_t &key, uint32_t &pos, uint32_t &kidx) const
     {
@@ -67,15 +70,8 @@ struct TowerKeyer {
         if (p < 0 || p >= P)
             return VIS_BAD_PLANE;
         const uint32_t task = (uint32_t)((cy * cc + cx) * nwp + (wp - wp_min));
-        if (tile > 0) {
-            const int tx = (gx + S) / tile, ty = (gy + S) / tile;
-            const int lx = gx + S - tx * tile, ly = gy + S - ty * tile;
-            key = task * (uint32_t)ntiles + (uint32_t)(ty * ntx + tx);
-            pos = (uint32_t)lx | ((uint32_t)ly << 16);
-        } else {
-            key = task;
-            pos = (uint32_t)(gx + S) | ((uint32_t)(gy + S) << 16);
-        }
+        key = band_h > 0 ? task * (uint32_t)nk + (uint32_t)((gy + S) / band_h) : task;
+        pos = (uint32_t)(gx + S) | ((uint32_t)(gy + S) << 16);
         kidx = (uint32_t)((p * R + yf) * R + xf);
         return VIS_OK;
     }
@@ -179,31 +175,140 @@ template <> struct Trig<double> {
     }
 };
 
+// ---- batched band pass -------------------------------------------------------------------
+// Block (b, j): band b (subgrid rows [H b, H b + H)) of task t0 + j of the batch.
+// The records of keys [b, b + S/H] of the task -- every footprint that reaches
+// into the band -- are accumulated Romein-style into the band in LDS: thread
+// (myU, myV) keeps the register sum of its current footprint cell and adds it
+// into the band when the cell changes; it only ever owns cells congruent to
+// (myU, myV) modulo S, so the LDS updates need no atomics.  The band's rows are
+// then inverse-FFT'd in LDS and the band is written once to the task's subgrid
+// with plain stores; band_flag says whether the band holds data.
+template <typename T, int S, int H, int LOGN>
+__global__ __launch_bounds__(S *S) void k_tower_bands(const Rec<T> *__restrict__ recs,
+                                                      const uint32_t *__restrict__ vis_start, int nk,
+                                                      const int32_t *__restrict__ task_slot, int t0,
+                                                      const typename Vec2<T>::type *__restrict__ kern,
+                                                      const typename Vec2<T>::type *__restrict__ twg,
+                                                      typename Vec2<T>::type *G, uint8_t *band_flag)
+{
+    using T2 = typename Vec2<T>::type;
+    constexpr int n = 1 << LOGN;
+    constexpr int NT = S * S;
+    constexpr int NB = n / H;
+    __shared__ T2 band[H * n];
+    __shared__ T2 tw[n];
+    __shared__ Rec<T> s_rec[NT];
+    const int b = blockIdx.x, j = blockIdx.y;
+    const int slot = task_slot[t0 + j];
+    const uint32_t r0 = vis_start[(int64_t)slot * nk + b];
+    const uint32_t r1 = vis_start[(int64_t)slot * nk + min(b + S / H + 1, nk)];
+    if (r0 == r1) {
+        if (threadIdx.x == 0)
+            band_flag[j * NB + b] = 0;
+        return;
+    }
+    for (int e = threadIdx.x; e < H * n; e += NT) {
+        band[e].x = T(0);
+        band[e].y = T(0);
+    }
+    for (int e = threadIdx.x; e < n; e += NT)
+        tw[e] = twg[e];
+    const int myU = threadIdx.x % S, myV = threadIdx.x / S;
+    int cur = -1;
+    T sr = T(0), si = T(0);
+    // U records at a time: U tap loads in flight, then the U updates in order.
+    // Padding records lie outside every band.
+    constexpr int U = 8;
+    static_assert(NT % U == 0, "record batch must be a multiple of the unroll");
+    for (uint32_t base = r0; base < r1; base += NT) {
+        const uint32_t nloc = min((uint32_t)NT, r1 - base);
+        const uint32_t nlocU = (nloc + U - 1) / U * U;
+        __syncthreads();
+        if (threadIdx.x < nloc) {
+            s_rec[threadIdx.x] = recs[base + threadIdx.x];
+        } else if (threadIdx.x < nlocU) {
+            Rec<T> z;
+            z.pos = 0xffffffffu;
+            z.kidx = 0;
+            z.vis.x = T(0);
+            z.vis.y = T(0);
+            s_rec[threadIdx.x] = z;
+        }
+        __syncthreads();
+        using B = RecBatch<T, U>;
+        auto fetch = [&](uint32_t j, B &bt) {
+#pragma unroll
+            for (int q = 0; q < U; ++q) {
+                const Rec<T> r = s_rec[j + q];
+                const int gx = (int)(r.pos & 0xffffu) - S, gy = (int)(r.pos >> 16) - S;
+                const int cx = (myU - gx) & (S - 1), cy = (myV - gy) & (S - 1);
+                const int col = gx + cx, row = gy + cy - H * b;
+                const bool in = (unsigned)row < (unsigned)H && (unsigned)col < (unsigned)n;
+                bt.cell[q] = in ? row * n + col : -1;
+                bt.v[q] = r.vis;
+                bt.k[q] = kern[in ? r.kidx * (uint32_t)NT + (uint32_t)(cy * S + cx) : 0u];
+            }
+        };
+        // lanes whose cell lies outside the band skip the record (no flush)
+        auto update = [&](const B &bt) {
+#pragma unroll
+            for (int q = 0; q < U; ++q) {
+                if (bt.cell[q] >= 0) {
+                    if (bt.cell[q] != cur) {
+                        if (cur >= 0) {
+                            band[cur].x += sr;
+                            band[cur].y += si;
+                        }
+                        cur = bt.cell[q];
+                        sr = T(0);
+                        si = T(0);
+                    }
+                    sr += bt.k[q].x * bt.v[q].x + bt.k[q].y * bt.v[q].y;
+                    si += bt.k[q].x * bt.v[q].y - bt.k[q].y * bt.v[q].x;
+                }
+            }
+        };
+        pipelined_records<U, B>(nlocU, fetch, update);
+    }
+    if (cur >= 0) {
+        band[cur].x += sr;
+        band[cur].y += si;
+    }
+    __syncthreads();
+    fft_lines_lm<T2, LOGN, NT, n, true>(band, tw, H);
+    T2 *g = G + (size_t)j * n * n + (size_t)b * H * n;
+    for (int e = threadIdx.x; e < H * n; e += NT)
+        g[e] = band[e];
+    if (threadIdx.x == 0)
+        band_flag[j * NB + b] = 1;
+}
+
 // ---- batched column pass -------------------------------------------------------------------
-// Block (cb, r): column block cb of tower run r (consecutive tasks of one tower
-// inside the batch).  For every task: load the column block of its subgrid
-// (rows not flagged are zero and are not read), inverse FFT, multiply by
-// exp(-2 pi i wp winc base), accumulate; then add the sum into the tower's
-// subimage and leave the subgrid rows it read zeroed.
-template <typename T, int LOGN, int CB>
+// Block (cb, c): column block cb of chunk c (up to a few consecutive tasks of
+// one tower).  For every task: load the column block of its subgrid (rows of
+// empty bands are zero and are not read), inverse FFT in LDS, multiply by
+// exp(-2 pi i wp winc base), accumulate in registers; write the chunk's sum
+// to its partial image.
+template <typename T, int LOGN, int CB, int H>
 __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
-    typename Vec2<T>::type *G, const uint32_t *flags, typename Vec2<T>::type *subimg,
-    const int32_t *run_first, const int32_t *run_count, int run_base, int32_t t0,
-    const int32_t *task_wp, const int32_t *task_tower, const typename Vec2<T>::type *__restrict__ twg,
+    const typename Vec2<T>::type *__restrict__ G, const uint8_t *__restrict__ band_flag,
+    typename Vec2<T>::type *P, const int32_t *__restrict__ chunk_first,
+    const int32_t *__restrict__ chunk_count, int chunk_base, int32_t t0,
+    const int32_t *__restrict__ task_wp, const typename Vec2<T>::type *__restrict__ twg,
     double theta, double winc)
 {
     using T2 = typename Vec2<T>::type;
     constexpr int n = 1 << LOGN;
     constexpr int NT = CB * (n / 8);
     constexpr int PER = n * CB / NT; // 8
-    constexpr int FW = (n + 31) / 32;
+    constexpr int NB = n / H;
     __shared__ T2 lds[n * CB];
     __shared__ T2 tw[n];
     for (int k = threadIdx.x; k < n; k += NT)
         tw[k] = twg[k];
-    const int r = run_base + blockIdx.y;
-    const int first = run_first[r], count = run_count[r];
-    const int tower = task_tower[first];
+    const int c = chunk_base + blockIdx.y;
+    const int first = chunk_first[c], count = chunk_count[c];
     const int c0 = blockIdx.x * CB;
 
     double base[PER];
@@ -218,8 +323,8 @@ __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
     }
     for (int k = first; k < first + count; ++k) {
         const int slot = k - t0;
-        T2 *g = G + (size_t)slot * n * n + c0;
-        const uint32_t *f = flags + (size_t)slot * FW;
+        const T2 *g = G + (size_t)slot * n * n + c0;
+        const uint8_t *f = band_flag + (size_t)slot * NB;
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
@@ -228,13 +333,8 @@ __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
             T2 val;
             val.x = T(0);
             val.y = T(0);
-            if ((f[row >> 5] >> (row & 31)) & 1u) {
+            if (f[row / H])
                 val = g[(size_t)row * n + col];
-                T2 z;
-                z.x = T(0);
-                z.y = T(0);
-                g[(size_t)row * n + col] = z;
-            }
             lds[e] = val;
         }
         __syncthreads();
@@ -243,22 +343,40 @@ __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const int e = threadIdx.x + q * NT;
-            T c, s;
-            Trig<T>::rot(ph0 * base[q], c, s);
+            T cs, sn;
+            Trig<T>::rot(ph0 * base[q], cs, sn);
             const T2 val = lds[e];
-            acc[q].x += val.x * c - val.y * s;
-            acc[q].y += val.x * s + val.y * c;
+            acc[q].x += val.x * cs - val.y * sn;
+            acc[q].y += val.x * sn + val.y * cs;
         }
     }
-    T2 *si = subimg + (size_t)tower * n * n + c0;
+    T2 *pp = P + (size_t)blockIdx.y * n * n + c0;
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         const int e = threadIdx.x + q * NT;
         const int row = e / CB, col = e - row * CB;
-        T2 cur = si[(size_t)row * n + col];
-        cur.x += acc[q].x;
-        cur.y += acc[q].y;
-        si[(size_t)row * n + col] = cur;
+        pp[(size_t)row * n + col] = acc[q];
+    }
+}
+
+// subimage[tower of run r] += sum of the partial images of the run's chunks
+template <typename T2>
+__global__ __launch_bounds__(256) void k_tower_reduce(const T2 *__restrict__ P, T2 *subimg,
+                                                      int64_t nn, const int32_t *run_tower,
+                                                      const int32_t *run_chunk0,
+                                                      const int32_t *run_nchunk, int run_base)
+{
+    const int r = run_base + blockIdx.y;
+    const int tower = run_tower[r], c0 = run_chunk0[r], nc = run_nchunk[r];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        T2 a = subimg[(size_t)tower * nn + i];
+        for (int c = c0; c < c0 + nc; ++c) {
+            const T2 p = P[(size_t)c * nn + i];
+            a.x += p.x;
+            a.y += p.y;
+        }
+        subimg[(size_t)tower * nn + i] = a;
     }
 }
 
@@ -386,55 +504,106 @@ __global__ __launch_bounds__(256) void k_w0_transfer(double2 *img, int n, int la
 
 struct TaskList {
     std::vector<int32_t> slot, tower_rank, wp;
-    std::vector<uint32_t> item0, item1, nvis;
+    std::vector<uint32_t> nvis;
     std::vector<int32_t> tower_of_rank; // tower index of each active tower
     std::vector<int32_t> rank_of_tower; // -1 when inactive
 };
 
-template <typename T, int LOGN>
-static int launch_cols_t(typename Vec2<T>::type *G, const uint32_t *flags,
-                         typename Vec2<T>::type *subimg, const int32_t *rf, const int32_t *rc,
-                         int run_base, int nruns, int32_t t0, const int32_t *twp,
-                         const int32_t *ttw, const void *tw, double theta, double winc,
+// Band height: rows of the subgrid one workgroup owns.  H rows of n complex
+// values take H n sizeof(T2) bytes of LDS; 8 rows of FP32 leave room for four
+// workgroups per CU.  FP64 supports 8 only.
+static int band_height_of(const rg_grid_params *p, bool fp32)
+{
+    if (p->tile == 16 && fp32)
+        return 16;
+    return 8;
+}
+
+template <typename T, int S, int H, int LOGN>
+static int launch_bands_h(const Rec<T> *recs, const uint32_t *vis_start, int nk,
+                          const int32_t *task_slot, int t0, int ntask, const void *kern,
+                          const void *tw, void *G, uint8_t *flags, hipStream_t s)
+{
+    using T2 = typename Vec2<T>::type;
+    constexpr int n = 1 << LOGN;
+    hipLaunchKernelGGL((k_tower_bands<T, S, H, LOGN>), dim3(n / H, ntask), dim3(S * S), 0, s, recs,
+                       vis_start, nk, task_slot, t0, (const T2 *)kern, (const T2 *)tw, (T2 *)G,
+                       flags);
+    RG_HIP(hipGetLastError());
+    return RG_OK;
+}
+
+template <typename T, int S, int LOGN>
+static int launch_bands_t(int H, const Rec<T> *recs, const uint32_t *vis_start, int nk,
+                          const int32_t *task_slot, int t0, int ntask, const void *kern,
+                          const void *tw, void *G, uint8_t *flags, hipStream_t s)
+{
+    if constexpr (sizeof(T) == 4) {
+        if (H == 16)
+            return launch_bands_h<T, S, 16, LOGN>(recs, vis_start, nk, task_slot, t0, ntask, kern,
+                                                  tw, G, flags, s);
+    }
+    return launch_bands_h<T, S, 8, LOGN>(recs, vis_start, nk, task_slot, t0, ntask, kern, tw, G,
+                                         flags, s);
+}
+
+template <typename T, int LOGN, int H>
+static int launch_cols_h(const void *G, const uint8_t *flags, void *P, const int32_t *cf,
+                         const int32_t *cc, int chunk_base, int nchunks, int32_t t0,
+                         const int32_t *twp, const void *tw, double theta, double winc,
                          hipStream_t s)
 {
+    using T2 = typename Vec2<T>::type;
     constexpr int n = 1 << LOGN;
     constexpr int CB0 = sizeof(T) == 4 ? 16 : 8;
     constexpr int CB = CB0 > n ? n : CB0;
-    hipLaunchKernelGGL((k_tower_cols<T, LOGN, CB>), dim3(n / CB, nruns), dim3(CB * (n / 8)), 0, s,
-                       G, flags, subimg, rf, rc, run_base, t0, twp, ttw,
-                       (const typename Vec2<T>::type *)tw, theta, winc);
+    hipLaunchKernelGGL((k_tower_cols<T, LOGN, CB, H>), dim3(n / CB, nchunks), dim3(CB * (n / 8)),
+                       0, s, (const T2 *)G, flags, (T2 *)P, cf, cc, chunk_base, t0, twp,
+                       (const T2 *)tw, theta, winc);
     RG_HIP(hipGetLastError());
     return RG_OK;
 }
 
-template <typename T>
-static int launch_cols(int n, typename Vec2<T>::type *G, const uint32_t *flags,
-                       typename Vec2<T>::type *subimg, const int32_t *rf, const int32_t *rc,
-                       int run_base, int nruns, int32_t t0, const int32_t *twp, const int32_t *ttw,
-                       const void *tw, double theta, double winc, hipStream_t s)
+template <typename T, int LOGN>
+static int launch_cols_t(int H, const void *G, const uint8_t *flags, void *P, const int32_t *cf,
+                         const int32_t *cc, int chunk_base, int nchunks, int32_t t0,
+                         const int32_t *twp, const void *tw, double theta, double winc,
+                         hipStream_t s)
+{
+    if (H == 16)
+        return launch_cols_h<T, LOGN, 16>(G, flags, P, cf, cc, chunk_base, nchunks, t0, twp, tw,
+                                          theta, winc, s);
+    return launch_cols_h<T, LOGN, 8>(G, flags, P, cf, cc, chunk_base, nchunks, t0, twp, tw, theta,
+                                     winc, s);
+}
+
+template <typename T, int S>
+static int launch_bands(int n, int H, const Rec<T> *recs, const uint32_t *vis_start, int nk,
+                        const int32_t *task_slot, int t0, int ntask, const void *kern,
+                        const void *tw, void *G, uint8_t *flags, hipStream_t s)
 {
     switch (n) {
-    case 32: return launch_cols_t<T, 5>(G, flags, subimg, rf, rc, run_base, nruns, t0, twp, ttw, tw, theta, winc, s);
-    case 64: return launch_cols_t<T, 6>(G, flags, subimg, rf, rc, run_base, nruns, t0, twp, ttw, tw, theta, winc, s);
-    case 128: return launch_cols_t<T, 7>(G, flags, subimg, rf, rc, run_base, nruns, t0, twp, ttw, tw, theta, winc, s);
-    case 256: return launch_cols_t<T, 8>(G, flags, subimg, rf, rc, run_base, nruns, t0, twp, ttw, tw, theta, winc, s);
-    case 512: return launch_cols_t<T, 9>(G, flags, subimg, rf, rc, run_base, nruns, t0, twp, ttw, tw, theta, winc, s);
-    default: return fail(RG_ERR_UNSUPPORTED, "batched W-towers supports subgrid sizes 32..512 (got %d)", n);
+    case 64: return launch_bands_t<T, S, 6>(H, recs, vis_start, nk, task_slot, t0, ntask, kern, tw, G, flags, s);
+    case 128: return launch_bands_t<T, S, 7>(H, recs, vis_start, nk, task_slot, t0, ntask, kern, tw, G, flags, s);
+    case 256: return launch_bands_t<T, S, 8>(H, recs, vis_start, nk, task_slot, t0, ntask, kern, tw, G, flags, s);
+    case 512: return launch_bands_t<T, S, 9>(H, recs, vis_start, nk, task_slot, t0, ntask, kern, tw, G, flags, s);
+    default: return fail(RG_ERR_UNSUPPORTED, "batched W-towers supports subgrid sizes 64..512 (got %d)", n);
     }
 }
 
-template <typename T, int S, int TILE>
-static int launch_tower_tiles(const BinResult &br, uint32_t item0, uint32_t item1,
-                              const rg_wkern *wk, const TileSink &sink, hipStream_t s)
+template <typename T>
+static int launch_cols(int n, int H, const void *G, const uint8_t *flags, void *P, const int32_t *cf,
+                       const int32_t *cc, int chunk_base, int nchunks, int32_t t0,
+                       const int32_t *twp, const void *tw, double theta, double winc,
+                       hipStream_t s)
 {
-    if (item1 > item0)
-        hipLaunchKernelGGL((k_tiles<T, S, TILE, false, T>), dim3(item1 - item0), dim3(S * S), 0, s,
-                           (const Rec<T> *)br.recs, (const Item *)br.items, item0, item1,
-                           (const unsigned long long *)nullptr,
-                           (const typename Vec2<T>::type *)wk->data, sink);
-    RG_HIP(hipGetLastError());
-    return RG_OK;
+    switch (n) {
+    case 64: return launch_cols_t<T, 6>(H, G, flags, P, cf, cc, chunk_base, nchunks, t0, twp, tw, theta, winc, s);
+    case 128: return launch_cols_t<T, 7>(H, G, flags, P, cf, cc, chunk_base, nchunks, t0, twp, tw, theta, winc, s);
+    case 256: return launch_cols_t<T, 8>(H, G, flags, P, cf, cc, chunk_base, nchunks, t0, twp, tw, theta, winc, s);
+    case 512: return launch_cols_t<T, 9>(H, G, flags, P, cf, cc, chunk_base, nchunks, t0, twp, tw, theta, winc, s);
+    default: return fail(RG_ERR_UNSUPPORTED, "batched W-towers supports subgrid sizes 64..512 (got %d)", n);
+    }
 }
 
 template <typename T>
@@ -446,13 +615,11 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
     const int chunk = n - M;
     const int cc = N / chunk + 1; // grid_cuda.cu:537-539
     const bool plain = p->mode == RG_MODE_PLAIN;
-    const int TILE = plain ? 0 : (p->tile > 0 ? p->tile : 32);
-    if (!plain && TILE != 32)
-        return fail(RG_ERR_UNSUPPORTED, "batched W-towers tile must be 32 (got %d)", TILE);
+    const int H = plain ? 0 : band_height_of(p, sizeof(T) == 4);
     if (!plain && S != 8 && S != 16)
         return fail(RG_ERR_UNSUPPORTED, "batched W-towers supports kernel sizes 8 and 16");
-    if (!plain && n < 32)
-        return fail(RG_ERR_UNSUPPORTED, "batched W-towers needs subgrid size >= 32 (got %d)", n);
+    if (!plain && (n < 64 || n > 512))
+        return fail(RG_ERR_UNSUPPORTED, "batched W-towers supports subgrid sizes 64..512 (got %d)", n);
     if (n + 2 * S >= 65536)
         return fail(RG_ERR_UNSUPPORTED, "subgrid too large");
 
@@ -504,34 +671,33 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
     kf.winc = p->winc;
     kf.wp_min = wp_min;
     kf.nwp = nwp;
-    kf.tile = TILE;
-    kf.ntx = TILE > 0 ? (n + S + TILE - 1) / TILE : 1;
-    kf.ntiles = kf.ntx * kf.ntx;
+    kf.band_h = H;
+    kf.nk = H > 0 ? (n + S - 1) / H + 1 : 1;
     const int64_t nslots = (int64_t)cc * cc * nwp;
-    const int64_t nkeys = nslots * kf.ntiles;
-    const uint32_t per_item = p->vis_per_item > 0 ? (uint32_t)p->vis_per_item : 2048u;
+    const int64_t nkeys = nslots * kf.nk;
+    // work items are not used by this path: one item per key keeps the list small
+    const uint32_t per_item = 0x40000000u;
 
     BinResult br;
     RG_TRY((bin_visibilities<TowerKeyer, T>(ctx, kf, vis->n, nkeys, per_item, s, br)));
 
-    // task table (read back)
+    // visibilities per task slot (read back)
     RG_TRY(ctx->scratch[9].ensure(sizeof(uint32_t) * 2 * nslots));
     uint32_t *d_nvis = ctx->scratch[9].as<uint32_t>();
     uint32_t *d_item0 = d_nvis + nslots;
     hipLaunchKernelGGL(k_task_info, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s,
                        (const uint32_t *)br.vis_start, (const uint32_t *)br.item_start, nslots,
-                       kf.ntiles, d_nvis, d_item0);
+                       kf.nk, d_nvis, d_item0);
     RG_HIP(hipGetLastError());
-    std::vector<uint32_t> h_info(2 * nslots);
-    RG_HIP(hipMemcpyAsync(h_info.data(), d_nvis, sizeof(uint32_t) * 2 * nslots,
-                          hipMemcpyDeviceToHost, s));
+    std::vector<uint32_t> h_nvis(nslots);
+    RG_HIP(hipMemcpyAsync(h_nvis.data(), d_nvis, sizeof(uint32_t) * nslots, hipMemcpyDeviceToHost,
+                          s));
     RG_HIP(hipStreamSynchronize(s));
 
     TaskList tl;
     tl.rank_of_tower.assign((size_t)cc * cc, -1);
-    std::vector<int32_t> slot_rank(nslots, -1);
     for (int64_t t = 0; t < nslots; ++t) {
-        const uint32_t nv = h_info[t];
+        const uint32_t nv = h_nvis[t];
         if (!nv)
             continue;
         const int tower = (int)(t / nwp);
@@ -539,26 +705,13 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
             tl.rank_of_tower[tower] = (int32_t)tl.tower_of_rank.size();
             tl.tower_of_rank.push_back(tower);
         }
-        slot_rank[t] = (int32_t)tl.slot.size();
         tl.slot.push_back((int32_t)t);
         tl.tower_rank.push_back(tl.rank_of_tower[tower]);
         tl.wp.push_back(wp_min + (int)(t % nwp));
-        tl.item0.push_back(h_info[nslots + t]);
         tl.nvis.push_back(nv);
     }
     const int ntasks = (int)tl.slot.size();
     const int ntowers = (int)tl.tower_of_rank.size();
-    // end of each task's item range: next slot's first item
-    tl.item1.resize(ntasks);
-    for (int k = 0; k < ntasks; ++k)
-        tl.item1[k] = (tl.slot[k] + 1 < nslots) ? h_info[nslots + tl.slot[k] + 1] : 0u;
-    if (ntasks > 0 && tl.slot[ntasks - 1] + 1 >= nslots) {
-        uint32_t last = 0;
-        RG_HIP(hipMemcpyAsync(&last, br.item_start + nkeys, sizeof(uint32_t),
-                              hipMemcpyDeviceToHost, s));
-        RG_HIP(hipStreamSynchronize(s));
-        tl.item1[ntasks - 1] = last;
-    }
     {
         unsigned long long tc[2] = {(unsigned long long)ntasks, (unsigned long long)ntowers};
         RG_HIP(hipMemcpyAsync(ctx->counters.as<unsigned long long>() + CNT_TASKS, tc, sizeof(tc),
@@ -576,46 +729,61 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
     const void *tw_inv = nullptr;
     RG_TRY(get_twiddles(ctx, n, sizeof(T) == 4 ? RG_C64 : RG_C128, +1, &tw_inv));
 
-    // device tables: rank_of_tower, slot_rank, task_wp, task_tower, run tables
-    const int Q = plain ? 1 : (p->floor_batch > 0 ? p->floor_batch : 128);
-    std::vector<int32_t> run_first, run_count, batch_run0;
+    // Batches of Q tasks; inside a batch each tower's run of tasks is cut into
+    // chunks of at most TPB tasks (one column-pass workgroup each).
+    const int Q = plain ? 1 : (p->floor_batch > 0 ? p->floor_batch : 256);
+    const int TPB = 8;
+    std::vector<int32_t> chunk_first, chunk_count, batch_chunk0, run_tower, run_chunk0, run_nchunk,
+        batch_run0;
     for (int t0 = 0; t0 < ntasks; t0 += Q) {
         const int t1 = std::min(ntasks, t0 + Q);
-        batch_run0.push_back((int32_t)run_first.size());
+        batch_chunk0.push_back((int32_t)chunk_first.size());
+        batch_run0.push_back((int32_t)run_tower.size());
         for (int k = t0; k < t1;) {
             int k1 = k;
             while (k1 < t1 && tl.tower_rank[k1] == tl.tower_rank[k])
                 ++k1;
-            run_first.push_back(k);
-            run_count.push_back(k1 - k);
+            run_tower.push_back(tl.tower_rank[k]);
+            run_chunk0.push_back((int32_t)chunk_first.size() - batch_chunk0.back());
+            int nc = 0;
+            for (int c = k; c < k1; c += TPB, ++nc) {
+                chunk_first.push_back(c);
+                chunk_count.push_back(std::min(TPB, k1 - c));
+            }
+            run_nchunk.push_back(nc);
             k = k1;
         }
     }
-    batch_run0.push_back((int32_t)run_first.size());
-    const size_t nruns = run_first.size();
+    batch_chunk0.push_back((int32_t)chunk_first.size());
+    batch_run0.push_back((int32_t)run_tower.size());
+    int max_chunks = 0;
+    for (size_t b = 0; b + 1 < batch_chunk0.size(); ++b)
+        max_chunks = std::max(max_chunks, batch_chunk0[b + 1] - batch_chunk0[b]);
+
     std::vector<int32_t> tables;
-    const size_t o_rot = 0, o_sr = o_rot + (size_t)cc * cc, o_wp = o_sr + nslots,
-                 o_tt = o_wp + ntasks, o_rf = o_tt + ntasks, o_rc = o_rf + nruns,
-                 o_end = o_rc + nruns;
+    const size_t o_rot = 0, o_slot = o_rot + (size_t)cc * cc, o_wp = o_slot + ntasks,
+                 o_cf = o_wp + ntasks, o_cc = o_cf + chunk_first.size(),
+                 o_rt = o_cc + chunk_count.size(), o_rc0 = o_rt + run_tower.size(),
+                 o_rnc = o_rc0 + run_chunk0.size(), o_end = o_rnc + run_nchunk.size();
     tables.resize(o_end);
     std::copy(tl.rank_of_tower.begin(), tl.rank_of_tower.end(), tables.begin() + o_rot);
-    std::copy(slot_rank.begin(), slot_rank.end(), tables.begin() + o_sr);
+    std::copy(tl.slot.begin(), tl.slot.end(), tables.begin() + o_slot);
     std::copy(tl.wp.begin(), tl.wp.end(), tables.begin() + o_wp);
-    std::copy(tl.tower_rank.begin(), tl.tower_rank.end(), tables.begin() + o_tt);
-    std::copy(run_first.begin(), run_first.end(), tables.begin() + o_rf);
-    std::copy(run_count.begin(), run_count.end(), tables.begin() + o_rc);
+    std::copy(chunk_first.begin(), chunk_first.end(), tables.begin() + o_cf);
+    std::copy(chunk_count.begin(), chunk_count.end(), tables.begin() + o_cc);
+    std::copy(run_tower.begin(), run_tower.end(), tables.begin() + o_rt);
+    std::copy(run_chunk0.begin(), run_chunk0.end(), tables.begin() + o_rc0);
+    std::copy(run_nchunk.begin(), run_nchunk.end(), tables.begin() + o_rnc);
     RG_TRY(ctx->scratch[8].ensure(64 + sizeof(int32_t) * o_end));
     int32_t *d_tab = reinterpret_cast<int32_t *>(ctx->scratch[8].as<char>() + 64);
     RG_HIP(hipMemcpyAsync(d_tab, tables.data(), sizeof(int32_t) * o_end, hipMemcpyHostToDevice, s));
 
-    const int FW = (n + 31) / 32;
     if (plain) {
-        // reference structure: one task at a time
+        // reference structure: one task at a time.  Keys are task slots here,
+        // so a task's records start at the running sum of the slot counts.
         RG_TRY(ctx->scratch[7].ensure(sizeof(double2) * nn));
         double2 *sub = ctx->scratch[7].as<double2>();
         const Rec<double> *recs = (const Rec<double> *)br.recs;
-        // keys are task slots here, so a task's records start at the running
-        // sum of the counts of the slots before it
         std::vector<uint32_t> start_of(ntasks);
         {
             uint32_t run = 0;
@@ -623,7 +791,7 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
             for (int64_t t = 0; t < nslots; ++t) {
                 if (k2 < ntasks && tl.slot[k2] == t)
                     start_of[k2++] = run;
-                run += h_info[t];
+                run += h_nvis[t];
             }
         }
         const int blocks_px = (int)((nn + 255) / 256);
@@ -635,8 +803,8 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
             for (; k < ntasks && tl.tower_rank[k] == r; ++k) {
                 RG_TRY(zero_fill(sub, sizeof(double2) * nn, s));
                 hipLaunchKernelGGL(k_tower_plain_scatter, dim3(128), dim3(128), 0, s, recs,
-                                   start_of[k],
-                                   tl.nvis[k], 128, (const double2 *)wk->data, S, sub, n);
+                                   start_of[k], tl.nvis[k], 128, (const double2 *)wk->data, S, sub,
+                                   n);
                 RG_HIP(hipGetLastError());
                 RG_TRY(fft2d(ctx, sub, RG_C128, n, 1, +1, s));
                 if (first) {
@@ -657,40 +825,43 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
             RG_TRY(fft2d(ctx, img, RG_C128, n, 1, -1, s));
         }
     } else {
-        RG_TRY(ctx->scratch[7].ensure(sizeof(T2) * nn * Q + sizeof(uint32_t) * FW * Q + 256));
+        const int NB = n / H;
+        const size_t g_bytes = sizeof(T2) * nn * Q;
+        const size_t p_bytes = sizeof(T2) * nn * (size_t)max_chunks;
+        RG_TRY(ctx->scratch[7].ensure(g_bytes + p_bytes + (size_t)Q * NB + 256));
         T2 *G = ctx->scratch[7].as<T2>();
-        uint32_t *flags = reinterpret_cast<uint32_t *>(G + nn * Q);
-        RG_TRY(zero_fill(G, sizeof(T2) * nn * Q, s));
+        T2 *Pp = reinterpret_cast<T2 *>(ctx->scratch[7].as<char>() + g_bytes);
+        uint8_t *flags = reinterpret_cast<uint8_t *>(ctx->scratch[7].as<char>() + g_bytes + p_bytes);
+        const Rec<T> *recs = (const Rec<T> *)br.recs;
         int b = 0;
         for (int t0 = 0; t0 < ntasks; t0 += Q, ++b) {
             const int t1 = std::min(ntasks, t0 + Q);
-            RG_TRY(zero_fill(flags, sizeof(uint32_t) * FW * Q, s));
-            TileSink sink;
-            sink.base = G;
-            sink.gsize = n;
-            sink.ntx = kf.ntx;
-            sink.ntiles = kf.ntiles;
-            sink.group_map = d_tab + o_sr;
-            sink.group_offset = t0;
-            sink.row_flags = flags;
-            const uint32_t i0 = tl.item0[t0], i1 = tl.item1[t1 - 1];
             {
                 KernelTimer kt(ctx, RG_TIMER_TOWER_TILES, s);
                 if (S == 16)
-                    RG_TRY((launch_tower_tiles<T, 16, 32>(br, i0, i1, wk, sink, s)));
+                    RG_TRY((launch_bands<T, 16>(n, H, recs, br.vis_start, kf.nk, d_tab + o_slot, t0,
+                                               t1 - t0, wk->data, tw_inv, G, flags, s)));
                 else
-                    RG_TRY((launch_tower_tiles<T, 8, 32>(br, i0, i1, wk, sink, s)));
+                    RG_TRY((launch_bands<T, 8>(n, H, recs, br.vis_start, kf.nk, d_tab + o_slot, t0,
+                                              t1 - t0, wk->data, tw_inv, G, flags, s)));
             }
+            const int cb0 = batch_chunk0[b], ncb = batch_chunk0[b + 1] - batch_chunk0[b];
             {
-                KernelTimer kt(ctx, RG_TIMER_TOWER_ROWS, s);
-                RG_TRY((fft_rows_dispatch<T2, true>(G, n, (int64_t)n * (t1 - t0),
-                                                    (const T2 *)tw_inv, flags, s)));
+                KernelTimer kt(ctx, RG_TIMER_TOWER_COLS, s);
+                RG_TRY(launch_cols<T>(n, H, G, flags, Pp, d_tab + o_cf, d_tab + o_cc, cb0, ncb, t0,
+                                      d_tab + o_wp, tw_inv, p->theta, p->winc, s));
             }
             const int rb = batch_run0[b], nr = batch_run0[b + 1] - batch_run0[b];
             {
-                KernelTimer kt(ctx, RG_TIMER_TOWER_COLS, s);
-                RG_TRY(launch_cols<T>(n, G, flags, subimg, d_tab + o_rf, d_tab + o_rc, rb, nr, t0,
-                                      d_tab + o_wp, d_tab + o_tt, tw_inv, p->theta, p->winc, s));
+                KernelTimer kt(ctx, RG_TIMER_TOWER_ROWS, s);
+                int64_t gx = (int64_t)((nn + 255) / 256);
+                if (gx > 1024)
+                    gx = 1024;
+                hipLaunchKernelGGL((k_tower_reduce<T2>), dim3((unsigned)gx, nr), dim3(256), 0, s,
+                                   (const T2 *)Pp, subimg, (int64_t)nn,
+                                   (const int32_t *)(d_tab + o_rt), (const int32_t *)(d_tab + o_rc0),
+                                   (const int32_t *)(d_tab + o_rnc), rb);
+                RG_HIP(hipGetLastError());
             }
         }
     }

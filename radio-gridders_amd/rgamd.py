@@ -88,6 +88,7 @@ SIGNATURES = [
     ("rg_dft_image", C.c_int, [_P, C.POINTER(FlatVis), _I, _D, _P, _I, _I, _I, _P]),
     ("rg_predict_points", C.c_int, [_P, _P, _P, _L, _P, _P, _P, _I, _P, _I, _P]),
     ("rg_pswf", C.c_double, [_D, _D]),
+    ("rg_pswf_eval", C.c_int, [_D, _P, _L, _P]),
     ("rg_wkern_generate", C.c_int, [_P, _I, _I, _I, _I, _D, _D, _D, _I, _D, _I]),
     ("rg_wtowers_flat", C.c_int, [C.c_char_p, C.c_char_p, _P, _P, C.c_int, _D, _D, _D, _D, C.c_int, C.c_int, _D]),
     ("rg_wtowers", C.c_int, [C.c_char_p, C.c_char_p, _P, _P, C.c_int, _D, _D, _D, _D, C.c_int, C.c_int, _D]),
@@ -255,6 +256,13 @@ def predict_points(u, v, w, l, m, flux, out, stream=None):
 
 def pswf(c: float, x: float) -> float:
     return lib().rg_pswf(float(c), float(x))
+
+
+def pswf_eval(c: float, x) -> np.ndarray:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.empty_like(x)
+    check(lib().rg_pswf_eval(float(c), x.ctypes.data, x.size, out.ctypes.data))
+    return out
 
 
 def make_wkernels(planes: int, oversampling: int, size: int, theta: float, w_min: float,
