@@ -27,6 +27,21 @@ import numpy as np  # noqa: E402
 
 METRIC = "Mvis/s gridded (2048² grid): FP32 batched vs FP64 plain at rel err ≤1e-5"
 HBM_PEAK_GBS = 8000.0
+PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
+
+
+def profiled_traffic(name):
+    """HBM bytes per launch of a kernel, from the FETCH_SIZE / WRITE_SIZE
+    summary under profiles/ (counters scaled by a 1 GiB copy), or None."""
+    path = os.path.join(PROFILES, name)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_dispatch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def log(*a):
@@ -192,15 +207,22 @@ def main():
     # dominant kernel roofline (algorithmic bytes per launch / event-timed duration)
     nv = obs.n_vis
     roof = None
+    # the counter summaries were taken on the default 512-station, 64-dump workload
+    default_workload = (args.stations, args.dumps) == (512, 64) and not args.tower_band \
+        and not args.floor_batch
     if head_kind == "wproj":
         t = head["timers"].get("wproj_tiles")
         per_vis = 24 + 8 + S * S * 8
         byt = nv * per_vis
         if t:
             ach = byt / (t["ms_per_step"] / t["launches_per_step"] * 1e-3) / 1e9 / t["launches_per_step"]
+            avg_ms = t["ms_per_step"] / t["launches_per_step"]
+            traffic = profiled_traffic("wproj_tiles_fp32_c2.txt") if default_workload else None
             roof = {"kernel": "k_tiles (W-projection)", "bound": "hbm", "achieved": ach,
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                    "traffic": None, "bytes_per_unit": per_vis, "unit_name": "visibility"}
+                    "traffic": traffic, "bytes_per_unit": per_vis, "unit_name": "visibility",
+                    "avg_launch_ms": avg_ms,
+                    "measured_GBps": (traffic / (avg_ms * 1e-3) / 1e9) if traffic else None}
     else:
         timers = head["timers"]
         dom = max((k for k in timers if k.startswith("tower")), key=lambda k: timers[k]["ms_per_step"])
@@ -208,9 +230,11 @@ def main():
         per_task = {"tower_cols": 3 * n2 * 8, "tower_rows": 2 * n2 * 8, "tower_tiles": n2 * 8}
         if dom in per_task:
             t = timers[dom]
+            prof = {"tower_tiles": "tower_bands_fp32_c2.txt", "tower_cols": "tower_cols_fp32_c2.txt"}
+            traffic = profiled_traffic(prof[dom]) if (default_workload and dom in prof) else None
             ach = per_task[dom] * nt / (t["ms_per_step"] * 1e-3) / 1e9
             roof = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                    "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
                     "bytes_per_unit": per_task[dom], "unit_name": "(tower, floor) task",
                     "avg_launch_ms": t["ms_per_step"] / t["launches_per_step"]}
 
