@@ -91,7 +91,8 @@ typedef struct rg_grid_params {
     int32_t tile;           /* batched W-projection: tile edge in cells (32 or 48, 0 = 32);
                                batched W-towers: band height in rows (8 or 16, 0 = 8) */
     int32_t vis_per_item;   /* batched visibilities per work item (0 = default) */
-    int32_t compensated;    /* batched FP32: 0 plain, 1 two-sum in the tile (default) */
+    int32_t compensated;    /* batched FP32: 0 plain FP32 tile sums (default), 1 two-sum
+                               compensated tile sums (worth it only into an RG_C128 grid) */
     int32_t floor_batch;    /* batched W-towers: (tower, floor) tasks per batch (0 = default) */
 } rg_grid_params;
 

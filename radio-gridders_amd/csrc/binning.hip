@@ -33,8 +33,18 @@ __device__ inline uint32_t items_of(uint32_t c, uint32_t per_item)
     return (c + per_item - 1) / per_item;
 }
 
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const uint32_t *count, int64_t nkeys,
-                                                            uint32_t per_item, uint2 *block_sums)
+// total count of key k over the replicas
+__device__ inline uint32_t key_count(const uint32_t *count, Replicas rp, int64_t k)
+{
+    uint32_t c = 0;
+    for (uint32_t r = 0; r <= rp.mask; ++r)
+        c += count[r * rp.stride + k];
+    return c;
+}
+
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const uint32_t *count, Replicas rp,
+                                                            int64_t nkeys, uint32_t per_item,
+                                                            uint2 *block_sums)
 {
     __shared__ uint2 lds[SCAN_BLOCK];
     const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
@@ -42,7 +52,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const uint32_t *coun
     for (int j = 0; j < SCAN_PER_THREAD; ++j) {
         const int64_t k = base + (int64_t)j * SCAN_BLOCK + threadIdx.x;
         if (k < nkeys) {
-            const uint32_t c = count[k];
+            const uint32_t c = key_count(count, rp, k);
             acc.x += c;
             acc.y += items_of(c, per_item);
         }
@@ -79,13 +89,13 @@ __global__ __launch_bounds__(1024) void k_scan_blocks(uint2 *block_sums, int nbl
     }
     if (threadIdx.x == 0) {
         block_sums[nblocks] = total;
-        stats[CNT_GRIDDED] = total.x;
+        stats[CNT_RECORDS] = total.x;
         stats[CNT_ITEMS] = total.y;
     }
 }
 
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t *count, int64_t nkeys,
-                                                          uint32_t per_item,
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t *count, Replicas rp,
+                                                          int64_t nkeys, uint32_t per_item,
                                                           const uint2 *block_sums,
                                                           uint32_t *vis_start,
                                                           uint32_t *item_start, uint32_t *cursor,
@@ -96,7 +106,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t *count,
     const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
     for (int j = 0; j < SCAN_PER_THREAD; ++j) {
         const int e = j * SCAN_BLOCK + threadIdx.x;
-        cnt[e] = (base + e < nkeys) ? count[base + e] : 0u;
+        cnt[e] = (base + e < nkeys) ? key_count(count, rp, base + e) : 0u;
     }
     __syncthreads();
     const int e0 = threadIdx.x * SCAN_PER_THREAD;
@@ -118,7 +128,11 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t *count,
         const uint32_t c = cnt[e0 + j];
         vis_start[k] = run.x;
         item_start[k] = run.y;
-        cursor[k] = run.x;
+        uint32_t cur = run.x;
+        for (uint32_t r = 0; r <= rp.mask; ++r) {
+            cursor[r * rp.stride + k] = cur;
+            cur += count[r * rp.stride + k];
+        }
         const uint32_t ni = items_of(c, per_item);
         for (uint32_t q = 0; q < ni; ++q) {
             Item it;

@@ -1,14 +1,24 @@
 // binning.hpp -- device-side counting sort of visibilities into work items.
 //
-// A "keyer" maps visibility i to (key, packed tile position, kernel-tap index).
-// Keys are dense in [0, nkeys).  The sort is:
-//   count   : wave-aggregated atomics on count[key]
-//   scan    : three-phase exclusive scan of (count, ceil(count / K))
-//   items   : one work item per K visibilities of a key
-//   scatter : wave-aggregated atomics on cursor[key], record written in place
+// A "keyer" maps visibility i to (key, nkey, packed tile position,
+// kernel-tap index): the visibility produces one record under each of the
+// keys key .. key + nkey - 1 (W-projection: one tile key; W-towers: every
+// subgrid band its footprint reaches).  Keys are dense in [0, nkeys).  The
+// sort is:
+//   count   : wave-aggregated atomics on count[rep][key]
+//   scan    : three-phase exclusive scan of (count, ceil(count / K)), where
+//             count[key] is the sum over the replicas
+//   items   : one work item per K records of a key
+//   scatter : wave-aggregated atomics on cursor[rep][key], record written in
+//             place
+// Block b uses counter replica rep = b mod NREP.  Device-scope atomics on one
+// address serialise, and a few keys (the short-baseline tiles at the uv
+// centre) take thousands of wavefront updates, so the counters of a key are
+// spread over NREP separate arrays; replica rep's records of a key follow
+// those of replicas 0..rep-1.
 // Consecutive visibilities of a wavefront that share a key keep their stream
 // order, so baseline-track runs stay contiguous inside a key (the locality
-// the Romein accumulation relies on).
+// the W-projection Romein accumulation relies on).
 #pragma once
 
 #include "rg_internal.hpp"
@@ -37,39 +47,56 @@ template <typename T> struct Rec {
 
 enum VisStatus { VIS_OK = 0, VIS_DROPPED = 1, VIS_BAD_PLANE = 2 };
 
-// Returns, for the calling lane, old counter value + rank among the lanes
-// of the wavefront that have the same key.  Must be called by the whole
-// wavefront in uniform control flow.
-__device__ inline uint32_t wave_agg_add(uint32_t *counters, uint32_t key, bool active)
+// Returns, for the calling lane, old counter value + rank inside its run (when
+// `want` is set).  A run is a maximal stretch of consecutive active lanes with
+// the same key; its first lane adds the run length to the key's counter with
+// one atomic.  All runs of the wavefront issue their atomics in one
+// instruction and the results are gathered once, so a wavefront costs one
+// atomic round trip whatever its keys (visibility streams are baseline-track
+// ordered, so neighbouring lanes mostly share keys).  Must be called by the
+// whole wavefront in uniform control flow.
+__device__ inline uint32_t wave_agg_add(uint32_t *counters, uint32_t key, bool active, bool want)
 {
     const int lane = __lane_id();
-    unsigned long long todo = __ballot(active);
-    uint32_t result = 0;
-    while (todo) {
-        const int leader = __ffsll((long long)todo) - 1;
-        const uint32_t lkey = __shfl(key, leader);
-        const unsigned long long same = __ballot(active && key == lkey) & todo;
-        uint32_t base = 0;
-        if (lane == leader)
-            base = atomicAdd(&counters[lkey], (uint32_t)__popcll(same));
-        base = __shfl(base, leader);
-        if ((same >> lane) & 1ull)
-            result = base + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
-        todo &= ~same;
+    const unsigned long long act = __ballot(active);
+    const uint32_t prev = (uint32_t)__shfl_up((int)key, 1);
+    const bool head = active && (lane == 0 || !((act >> (lane - 1)) & 1ull) || prev != key);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long ends = heads | ~act; // a run stops at the next head or idle lane
+    uint32_t base = 0;
+    if (head) {
+        const unsigned long long nxt = lane == 63 ? 0ull : (ends >> (lane + 1));
+        const uint32_t len = nxt ? (uint32_t)__ffsll((long long)nxt) : (uint32_t)(64 - lane);
+        base = atomicAdd(&counters[key], len);
     }
-    return result;
+    if (!want)
+        return 0;
+    const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const unsigned long long hm = heads & upto;
+    const int my_head = hm ? 63 - __clzll((long long)hm) : 0;
+    return (uint32_t)__shfl((int)base, my_head) + (uint32_t)(lane - my_head);
 }
+
+// Counter replicas: NREP arrays of `stride` entries (stride >= nkeys + 1).
+struct Replicas {
+    uint32_t mask;   // NREP - 1, NREP a power of two
+    int64_t stride;
+};
 
 template <typename Keyer>
 __global__ __launch_bounds__(256) void k_bin_count(Keyer kf, int64_t n, uint32_t *count,
-                                                   unsigned long long *stats)
+                                                   Replicas rp, unsigned long long *stats)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t key = 0, pos = 0, kidx = 0;
+    int nkey = 0;
     int st = VIS_DROPPED;
     if (i < n)
-        st = kf.classify(i, key, pos, kidx);
-    wave_agg_add(count, key, i < n && st == VIS_OK);
+        st = kf.classify(i, key, nkey, pos, kidx);
+    const bool ok = i < n && st == VIS_OK;
+    uint32_t *cnt = count + (blockIdx.x & rp.mask) * rp.stride;
+    for (int q = 0; q < kf.max_keys(); ++q)
+        wave_agg_add(cnt, key + (uint32_t)q, ok && q < nkey, false);
     // per-wave status counts
     const unsigned long long dropped = __ballot(i < n && st == VIS_DROPPED);
     const unsigned long long badp = __ballot(i < n && st == VIS_BAD_PLANE);
@@ -83,33 +110,39 @@ __global__ __launch_bounds__(256) void k_bin_count(Keyer kf, int64_t n, uint32_t
 
 template <typename Keyer, typename T>
 __global__ __launch_bounds__(256) void k_bin_scatter(Keyer kf, int64_t n, uint32_t *cursor,
-                                                     Rec<T> *recs)
+                                                     Replicas rp, Rec<T> *recs)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t key = 0, pos = 0, kidx = 0;
+    int nkey = 0;
     int st = VIS_DROPPED;
     if (i < n)
-        st = kf.classify(i, key, pos, kidx);
+        st = kf.classify(i, key, nkey, pos, kidx);
     const bool ok = i < n && st == VIS_OK;
-    const uint32_t dst = wave_agg_add(cursor, key, ok);
-    if (ok) {
-        Rec<T> r;
-        r.pos = pos;
-        r.kidx = kidx;
+    Rec<T> r;
+    r.pos = pos;
+    r.kidx = kidx;
+    if (ok)
         r.vis = kf.template vis_value<T>(i);
-        recs[dst] = r;
+    uint32_t *cur = cursor + (blockIdx.x & rp.mask) * rp.stride;
+    for (int q = 0; q < kf.max_keys(); ++q) {
+        const bool act = ok && q < nkey;
+        const uint32_t dst = wave_agg_add(cur, key + (uint32_t)q, act, true);
+        if (act)
+            recs[dst] = r;
     }
 }
 
 // Phase 1: per-block totals of (count, items).
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const uint32_t *count, int64_t nkeys,
-                                                            uint32_t per_item, uint2 *block_sums);
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_reduce(const uint32_t *count, Replicas rp,
+                                                            int64_t nkeys, uint32_t per_item,
+                                                            uint2 *block_sums);
 // Phase 2: exclusive scan of block totals (single block), totals to stats.
 __global__ __launch_bounds__(1024) void k_scan_blocks(uint2 *block_sums, int nblocks,
                                                       unsigned long long *stats);
-// Phase 3: per-key starts, work items and scatter cursors.
-__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t *count, int64_t nkeys,
-                                                          uint32_t per_item,
+// Phase 3: per-key starts, work items and the scatter cursors of every replica.
+__global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t *count, Replicas rp,
+                                                          int64_t nkeys, uint32_t per_item,
                                                           const uint2 *block_sums,
                                                           uint32_t *vis_start,
                                                           uint32_t *item_start, uint32_t *cursor,

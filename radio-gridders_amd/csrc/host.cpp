@@ -296,9 +296,9 @@ int rg_context_stats(rg_context *ctx, rg_grid_stats *out)
     out->n_items = (int64_t)c[CNT_ITEMS];
     out->n_tasks = (int64_t)c[CNT_TASKS];
     out->n_towers = (int64_t)c[CNT_TOWERS];
-    out->n_gridded = ctx->last_kind == 1 && c[CNT_GRIDDED] == 0 && c[CNT_ITEMS] == 0
-                         ? ctx->last_nvis - out->n_dropped - out->n_bad_plane
-                         : (int64_t)c[CNT_GRIDDED];
+    // every visibility is gridded, dropped (off the grid / tower set) or has
+    // no W-plane
+    out->n_gridded = ctx->last_nvis - out->n_dropped - out->n_bad_plane;
     return RG_OK;
 }
 

@@ -128,7 +128,7 @@ enum Counter {
     CNT_ITEMS = 2,
     CNT_TASKS = 3,
     CNT_TOWERS = 4,
-    CNT_GRIDDED = 5,
+    CNT_RECORDS = 5, // records written by the sort (one per key a visibility reaches)
     CNT_MAX_ITEMS = 6,
     CNT__N = 8
 };

@@ -36,7 +36,6 @@ struct TileSink {
     int ntiles;              // tiles per group
     const int32_t *group_map; // optional: group -> rank (slot of the target array)
     int32_t group_offset;     // rank - offset = target slot
-    uint32_t *row_flags;      // optional: bit per target row, gsize bits per slot
 };
 
 template <typename T, int S, int TILE, bool COMP, typename GT>
@@ -56,7 +55,6 @@ __global__ __launch_bounds__(S *S) void k_tiles(const Rec<T> *__restrict__ recs,
     __shared__ T s_lre[LC];
     __shared__ T s_lim[LC];
     __shared__ Rec<T> s_rec[NT];
-    __shared__ uint32_t s_rowmask[(RT + 31) / 32];
 
     const uint32_t end = stats ? (uint32_t)stats[CNT_ITEMS] : item_end;
     const uint32_t it = item0 + blockIdx.x;
@@ -72,8 +70,6 @@ __global__ __launch_bounds__(S *S) void k_tiles(const Rec<T> *__restrict__ recs,
             s_lim[c] = T(0);
         }
     }
-    if (threadIdx.x < (RT + 31) / 32)
-        s_rowmask[threadIdx.x] = 0;
 
     const int myU = threadIdx.x % S, myV = threadIdx.x / S;
     int cur = -1;
@@ -168,19 +164,6 @@ __global__ __launch_bounds__(S *S) void k_tiles(const Rec<T> *__restrict__ recs,
                 GT *dst = g + 2 * ((size_t)gy * G + gx);
                 atomicAdd(dst, re);
                 atomicAdd(dst + 1, im);
-                if (sink.row_flags)
-                    atomicOr(&s_rowmask[ry >> 5], 1u << (ry & 31));
-            }
-        }
-    }
-    if (sink.row_flags) {
-        __syncthreads();
-        if (threadIdx.x < RT) {
-            const int ry = threadIdx.x;
-            if ((s_rowmask[ry >> 5] >> (ry & 31)) & 1u) {
-                const int gy = oy + ry;
-                uint32_t *f = sink.row_flags + (size_t)slot * ((G + 31) / 32);
-                atomicOr(&f[gy >> 5], 1u << (gy & 31));
             }
         }
     }

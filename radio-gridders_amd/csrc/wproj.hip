@@ -155,9 +155,13 @@ struct WprojKeyer {
         return lo;
     }
 
-    __device__ int classify(int64_t i, uint32_t &key, uint32_t &pos, uint32_t &kidx) const
+    __host__ __device__ int max_keys() const { return 1; }
+
+    __device__ int classify(int64_t i, uint32_t &key, int &nkey, uint32_t &pos,
+                            uint32_t &kidx) const
     {
         int gx, gy, xf, yf;
+        nkey = 1;
         frac_coord(N, S, R, theta, u[i], v[i], 0.0, 0.0, gx, gy, xf, yf);
         if (gx <= -S || gx >= N || gy <= -S || gy >= N)
             return VIS_DROPPED;
@@ -200,7 +204,6 @@ static int launch_tiles(const BinResult &br, rg_context *ctx, const rg_wkern *wk
     sink.ntiles = ntiles;
     sink.group_map = nullptr;
     sink.group_offset = 0;
-    sink.row_flags = nullptr;
     KernelTimer kt(ctx, RG_TIMER_WPROJ_TILES, s);
     if (br.max_items > 0)
         hipLaunchKernelGGL((k_tiles<T, S, TILE, COMP, GT>), dim3((unsigned)br.max_items),

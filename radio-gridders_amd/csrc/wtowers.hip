@@ -28,10 +28,6 @@
 
 namespace rg {
 
-template <typename T2, bool INV>
-int fft_rows_dispatch(T2 *data, int n, int64_t nrows, const T2 *tw, const uint32_t *flags,
-                      hipStream_t s);
-
 // ---- binning -------------------------------------------------------------------------
 
 struct TowerKeyer {
@@ -43,11 +39,17 @@ struct TowerKeyer {
     int n, chunk, cc;
     double winc;
     int wp_min, nwp;
-    // band_h == 0: key = task; else key = task * nk + (gy + S) / band_h.
+    // band_h == 0: one key per visibility, key = task.  Otherwise keys are
+    // task * nk + b for the nk = n / band_h bands of band_h subgrid rows, and a
+    // visibility has one record under every band its footprint rows
+    // [gy, gy + S) reach inside [0, n).
     // pos is always the raw subgrid corner + S: (gx + S) | (gy + S) << 16.
-    int band_h, nk;
+    int band_h, nk, mk;
 
-    __device__ int classify(int64_t i, uint32_t &key, uint32_t &pos, uint32_t &kidx) const
+    __host__ __device__ int max_keys() const { return mk; }
+
+    __device__ int classify(int64_t i, uint32_t &key, int &nkey, uint32_t &pos,
+                            uint32_t &kidx) const
     {
         const double ui = u[i], vi = v[i], wi = w[i];
         const int cx = tower_index(ui, theta, N, chunk);
@@ -70,7 +72,14 @@ struct TowerKeyer {
         if (p < 0 || p >= P)
             return VIS_BAD_PLANE;
         const uint32_t task = (uint32_t)((cy * cc + cx) * nwp + (wp - wp_min));
-        key = band_h > 0 ? task * (uint32_t)nk + (uint32_t)((gy + S) / band_h) : task;
+        if (band_h > 0) {
+            const int b0 = max(gy, 0) / band_h, b1 = min(gy + S - 1, n - 1) / band_h;
+            key = task * (uint32_t)nk + (uint32_t)b0;
+            nkey = b1 - b0 + 1;
+        } else {
+            key = task;
+            nkey = 1;
+        }
         pos = (uint32_t)(gx + S) | ((uint32_t)(gy + S) << 16);
         kidx = (uint32_t)((p * R + yf) * R + xf);
         return VIS_OK;
@@ -175,59 +184,144 @@ template <> struct Trig<double> {
     }
 };
 
+// ---- record order inside band items -------------------------------------------------------
+// A band lane's cell depends only on the footprint column gx (its row is fixed),
+// so records sorted by gx keep each lane on one cell for long runs and the
+// Romein register sum rarely flushes.  Counting sort of every item's records
+// by gx + S (bins < n + S <= SORT_BINS) from `in` into `out` (same offsets);
+// the order inside a bin follows LDS atomics.
+constexpr int SORT_BINS = 1024;
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_sort_items(const Rec<T> *__restrict__ in,
+                                                    Rec<T> *__restrict__ out,
+                                                    const Item *__restrict__ items,
+                                                    const unsigned long long *__restrict__ stats,
+                                                    int nbins)
+{
+    __shared__ uint32_t bins[SORT_BINS];
+    __shared__ uint32_t wsum[4];
+    const uint32_t it = blockIdx.x;
+    if (it >= (uint32_t)stats[CNT_ITEMS])
+        return;
+    const Item item = items[it];
+    const Rec<T> *src = in + item.start;
+    Rec<T> *dst = out + item.start;
+    if (item.count <= 32) {
+        if (threadIdx.x < item.count)
+            dst[threadIdx.x] = src[threadIdx.x];
+        return;
+    }
+    for (int b = threadIdx.x; b < SORT_BINS; b += 256)
+        bins[b] = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < item.count; i += 256)
+        atomicAdd(&bins[min(src[i].pos & 0xffffu, (uint32_t)nbins - 1)], 1u);
+    __syncthreads();
+    // exclusive scan of the bins: 4 per thread, wavefront scans, wave totals
+    uint32_t v[4], tot = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        v[q] = bins[threadIdx.x * 4 + q];
+        tot += v[q];
+    }
+    const int lane = __lane_id(), wv = threadIdx.x / 64;
+    uint32_t inc = tot;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_up((int)inc, o);
+        if (lane >= o)
+            inc += y;
+    }
+    if (lane == 63)
+        wsum[wv] = inc;
+    __syncthreads();
+    uint32_t run = inc - tot;
+    for (int w2 = 0; w2 < wv; ++w2)
+        run += wsum[w2];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        bins[threadIdx.x * 4 + q] = run;
+        run += v[q];
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < item.count; i += 256) {
+        const Rec<T> r = src[i];
+        const uint32_t d = atomicAdd(&bins[min(r.pos & 0xffffu, (uint32_t)nbins - 1)], 1u);
+        dst[d] = r;
+    }
+}
+
 // ---- batched band pass -------------------------------------------------------------------
-// Block (b, j): band b (subgrid rows [H b, H b + H)) of task t0 + j of the batch.
-// The records of keys [b, b + S/H] of the task -- every footprint that reaches
-// into the band -- are accumulated Romein-style into the band in LDS: thread
-// (myU, myV) keeps the register sum of its current footprint cell and adds it
-// into the band when the cell changes; it only ever owns cells congruent to
-// (myU, myV) modulo S, so the LDS updates need no atomics.  The band's rows are
-// then inverse-FFT'd in LDS and the band is written once to the task's subgrid
-// with plain stores; band_flag says whether the band holds data.
-template <typename T, int S, int H, int LOGN>
-__global__ __launch_bounds__(S *S) void k_tower_bands(const Rec<T> *__restrict__ recs,
-                                                      const uint32_t *__restrict__ vis_start, int nk,
-                                                      const int32_t *__restrict__ task_slot, int t0,
-                                                      const typename Vec2<T>::type *__restrict__ kern,
-                                                      const typename Vec2<T>::type *__restrict__ twg,
-                                                      typename Vec2<T>::type *G, uint8_t *band_flag)
+// One workgroup per work item: up to K records of one band key (task, b), i.e.
+// visibilities whose footprint reaches subgrid rows [H b, H b + H).  The
+// workgroup is GRP groups of S x H lanes; the groups take alternate runs of the
+// item's records.  Lane (myU, myV) owns band row myV and the columns congruent
+// to myU modulo S: a footprint covering that row has exactly one cell there.
+// The lane keeps the register sum of its current cell and adds it into the
+// LDS band when the cell changes (LDS float atomics when GRP > 1, since lanes
+// of different groups own the same cells).  The band's rows are then inverse
+// FFT'd in LDS.  A key of one item stores the band straight into the task's
+// subgrid; a key split over several items stores each part in a spill slot
+// and k_band_combine sums the parts in item order.
+template <typename T, int S, int H, int LOGN, int GRP>
+__global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
+    const Rec<T> *__restrict__ recs, const Item *__restrict__ items,
+    const uint32_t *__restrict__ item_start, uint32_t key_begin, uint32_t key_end,
+    const int32_t *__restrict__ rank_of_slot, int t0,
+    const typename Vec2<T>::type *__restrict__ kern, const typename Vec2<T>::type *__restrict__ twg,
+    typename Vec2<T>::type *G, typename Vec2<T>::type *spill, uint32_t spill_cap,
+    uint32_t *spill_ctr, uint32_t *item_spill)
 {
     using T2 = typename Vec2<T>::type;
     constexpr int n = 1 << LOGN;
-    constexpr int NT = S * S;
+    constexpr int NL = S * H;
+    constexpr int NT = NL * GRP;
     constexpr int NB = n / H;
     __shared__ T2 band[H * n];
     __shared__ T2 tw[n];
     __shared__ Rec<T> s_rec[NT];
-    const int b = blockIdx.x, j = blockIdx.y;
-    const int slot = task_slot[t0 + j];
-    const uint32_t r0 = vis_start[(int64_t)slot * nk + b];
-    const uint32_t r1 = vis_start[(int64_t)slot * nk + min(b + S / H + 1, nk)];
-    if (r0 == r1) {
-        if (threadIdx.x == 0)
-            band_flag[j * NB + b] = 0;
+    __shared__ uint32_t s_slot;
+    const uint32_t ib = item_start[key_begin], ie = item_start[key_end];
+    const uint32_t it = ib + blockIdx.x;
+    if (it >= ie)
         return;
-    }
+    const Item item = items[it];
+    const uint32_t tslot = item.key / (uint32_t)NB;
+    const int b = (int)(item.key - tslot * (uint32_t)NB);
+    const int j = rank_of_slot[tslot] - t0;
+
     for (int e = threadIdx.x; e < H * n; e += NT) {
         band[e].x = T(0);
         band[e].y = T(0);
     }
     for (int e = threadIdx.x; e < n; e += NT)
         tw[e] = twg[e];
-    const int myU = threadIdx.x % S, myV = threadIdx.x / S;
+    const int lane = threadIdx.x % NL, grp = threadIdx.x / NL;
+    const int myU = lane % S, myV = lane / S;
+    const int row = H * b + myV;
+    T *band_f = reinterpret_cast<T *>(band);
     int cur = -1;
     T sr = T(0), si = T(0);
+    auto flush = [&]() {
+        if (GRP > 1) {
+            atomicAdd(&band_f[2 * cur], sr);
+            atomicAdd(&band_f[2 * cur + 1], si);
+        } else {
+            band[cur].x += sr;
+            band[cur].y += si;
+        }
+    };
     // U records at a time: U tap loads in flight, then the U updates in order.
     // Padding records lie outside every band.
     constexpr int U = 8;
-    static_assert(NT % U == 0, "record batch must be a multiple of the unroll");
-    for (uint32_t base = r0; base < r1; base += NT) {
-        const uint32_t nloc = min((uint32_t)NT, r1 - base);
-        const uint32_t nlocU = (nloc + U - 1) / U * U;
+    static_assert(NL % U == 0, "record batch must be a multiple of the unroll");
+    for (uint32_t base = 0; base < item.count; base += NT) {
+        const uint32_t nloc = min((uint32_t)NT, item.count - base);
         __syncthreads();
         if (threadIdx.x < nloc) {
-            s_rec[threadIdx.x] = recs[base + threadIdx.x];
-        } else if (threadIdx.x < nlocU) {
+            s_rec[threadIdx.x] = recs[item.start + base + threadIdx.x];
+        } else {
             Rec<T> z;
             z.pos = 0xffffffffu;
             z.kidx = 0;
@@ -236,30 +330,31 @@ __global__ __launch_bounds__(S *S) void k_tower_bands(const Rec<T> *__restrict__
             s_rec[threadIdx.x] = z;
         }
         __syncthreads();
-        using B = RecBatch<T, U>;
-        auto fetch = [&](uint32_t j, B &bt) {
+        const int ng = min(NL, max(0, (int)nloc - grp * NL));
+        const Rec<T> *gr = s_rec + grp * NL;
+        using Bt = RecBatch<T, U>;
+        auto fetch = [&](uint32_t q0, Bt &bt) {
 #pragma unroll
             for (int q = 0; q < U; ++q) {
-                const Rec<T> r = s_rec[j + q];
+                const Rec<T> r = gr[q0 + q];
                 const int gx = (int)(r.pos & 0xffffu) - S, gy = (int)(r.pos >> 16) - S;
-                const int cx = (myU - gx) & (S - 1), cy = (myV - gy) & (S - 1);
-                const int col = gx + cx, row = gy + cy - H * b;
-                const bool in = (unsigned)row < (unsigned)H && (unsigned)col < (unsigned)n;
-                bt.cell[q] = in ? row * n + col : -1;
+                const int cy = row - gy;
+                const int cx = (myU - gx) & (S - 1);
+                const int col = gx + cx;
+                const bool in = (unsigned)cy < (unsigned)S && (unsigned)col < (unsigned)n;
+                bt.cell[q] = in ? myV * n + col : -1;
                 bt.v[q] = r.vis;
-                bt.k[q] = kern[in ? r.kidx * (uint32_t)NT + (uint32_t)(cy * S + cx) : 0u];
+                bt.k[q] = kern[in ? r.kidx * (uint32_t)(S * S) + (uint32_t)(cy * S + cx) : 0u];
             }
         };
-        // lanes whose cell lies outside the band skip the record (no flush)
-        auto update = [&](const B &bt) {
+        // lanes whose cell lies outside the footprint skip the record (no flush)
+        auto update = [&](const Bt &bt) {
 #pragma unroll
             for (int q = 0; q < U; ++q) {
                 if (bt.cell[q] >= 0) {
                     if (bt.cell[q] != cur) {
-                        if (cur >= 0) {
-                            band[cur].x += sr;
-                            band[cur].y += si;
-                        }
+                        if (cur >= 0)
+                            flush();
                         cur = bt.cell[q];
                         sr = T(0);
                         si = T(0);
@@ -269,31 +364,87 @@ __global__ __launch_bounds__(S *S) void k_tower_bands(const Rec<T> *__restrict__
                 }
             }
         };
-        pipelined_records<U, B>(nlocU, fetch, update);
+        pipelined_records<U, Bt>((uint32_t)((ng + U - 1) / U * U), fetch, update);
     }
-    if (cur >= 0) {
-        band[cur].x += sr;
-        band[cur].y += si;
-    }
+    if (cur >= 0)
+        flush();
     __syncthreads();
     fft_lines_lm<T2, LOGN, NT, n, true>(band, tw, H);
-    T2 *g = G + (size_t)j * n * n + (size_t)b * H * n;
+
+    const uint32_t k0 = item_start[item.key], k1 = item_start[item.key + 1];
+    T2 *dst;
+    if (k1 - k0 == 1) {
+        dst = G + (size_t)j * n * n + (size_t)b * H * n;
+    } else {
+        if (threadIdx.x == 0) {
+            const uint32_t sl = atomicAdd(spill_ctr, 1u);
+            item_spill[it - ib] = sl;
+            s_slot = sl;
+        }
+        __syncthreads();
+        // the capacity bounds every batch (at most 2 R / K parts); the check
+        // only keeps the stores inside the buffer
+        if (s_slot >= spill_cap)
+            return;
+        dst = spill + (size_t)s_slot * H * n;
+    }
     for (int e = threadIdx.x; e < H * n; e += NT)
-        g[e] = band[e];
-    if (threadIdx.x == 0)
-        band_flag[j * NB + b] = 1;
+        dst[e] = band[e];
+}
+
+// Sums the spilled parts of every key split over several items (in item
+// order) into the task's subgrid band.  Launched with the grid of
+// k_tower_band_items; the first item of each split key does the work.
+template <typename T2, int H, int LOGN>
+__global__ __launch_bounds__(256) void k_band_combine(const Item *__restrict__ items,
+                                                      const uint32_t *__restrict__ item_start,
+                                                      uint32_t key_begin, uint32_t key_end,
+                                                      const int32_t *__restrict__ rank_of_slot,
+                                                      int t0, T2 *G, const T2 *__restrict__ spill,
+                                                      uint32_t spill_cap,
+                                                      const uint32_t *__restrict__ item_spill)
+{
+    constexpr int n = 1 << LOGN;
+    constexpr int NB = n / H;
+    const uint32_t ib = item_start[key_begin], ie = item_start[key_end];
+    const uint32_t it = ib + blockIdx.x;
+    if (it >= ie)
+        return;
+    const uint32_t key = items[it].key;
+    const uint32_t k0 = item_start[key], k1 = item_start[key + 1];
+    if (it != k0 || k1 - k0 < 2)
+        return;
+    const uint32_t tslot = key / (uint32_t)NB;
+    const int b = (int)(key - tslot * (uint32_t)NB);
+    const int j = rank_of_slot[tslot] - t0;
+    T2 *dst = G + (size_t)j * n * n + (size_t)b * H * n;
+    for (int e = threadIdx.x; e < H * n; e += blockDim.x) {
+        T2 a;
+        a.x = 0;
+        a.y = 0;
+        for (uint32_t q = k0; q < k1; ++q) {
+            const uint32_t sl = item_spill[q - ib];
+            if (sl < spill_cap) {
+                const T2 v = spill[(size_t)sl * H * n + e];
+                a.x += v.x;
+                a.y += v.y;
+            }
+        }
+        dst[e] = a;
+    }
 }
 
 // ---- batched column pass -------------------------------------------------------------------
 // Block (cb, c): column block cb of chunk c (up to a few consecutive tasks of
-// one tower).  For every task: load the column block of its subgrid (rows of
-// empty bands are zero and are not read), inverse FFT in LDS, multiply by
+// one tower).  For every task: load the column block of its subgrid (bands
+// without records are zero and are not read), inverse FFT in LDS, multiply by
 // exp(-2 pi i wp winc base), accumulate in registers; write the chunk's sum
 // to its partial image.
 template <typename T, int LOGN, int CB, int H>
 __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
-    const typename Vec2<T>::type *__restrict__ G, const uint8_t *__restrict__ band_flag,
-    typename Vec2<T>::type *P, const int32_t *__restrict__ chunk_first,
+    const typename Vec2<T>::type *__restrict__ G, const uint32_t *__restrict__ key_start,
+    const int32_t *__restrict__ task_slot, typename Vec2<T>::type *P,
+    const int32_t *__restrict__ chunk_first,
     const int32_t *__restrict__ chunk_count, int chunk_base, int32_t t0,
     const int32_t *__restrict__ task_wp, const typename Vec2<T>::type *__restrict__ twg,
     double theta, double winc)
@@ -321,23 +472,32 @@ __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
         acc[q].x = T(0);
         acc[q].y = T(0);
     }
-    for (int k = first; k < first + count; ++k) {
-        const int slot = k - t0;
-        const T2 *g = G + (size_t)slot * n * n + c0;
-        const uint8_t *f = band_flag + (size_t)slot * NB;
-        __syncthreads();
+    // The column block of task k + 1 is loaded into registers while task k is
+    // transformed in LDS.
+    T2 nxt[PER];
+    auto load = [&](int k) {
+        const T2 *g = G + (size_t)(k - t0) * n * n + c0;
+        // band b of the task holds data iff its key has records
+        const uint32_t *f = key_start + (size_t)task_slot[k] * NB;
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const int e = threadIdx.x + q * NT;
             const int row = e / CB, col = e - row * CB;
-            T2 val;
-            val.x = T(0);
-            val.y = T(0);
-            if (f[row / H])
-                val = g[(size_t)row * n + col];
-            lds[e] = val;
+            nxt[q].x = T(0);
+            nxt[q].y = T(0);
+            if (f[row / H + 1] > f[row / H])
+                nxt[q] = g[(size_t)row * n + col];
         }
+    };
+    load(first);
+    for (int k = first; k < first + count; ++k) {
         __syncthreads();
+#pragma unroll
+        for (int q = 0; q < PER; ++q)
+            lds[threadIdx.x + q * NT] = nxt[q];
+        __syncthreads();
+        if (k + 1 < first + count)
+            load(k + 1);
         fft_block<T2, LOGN, CB, CB, true>(lds, tw);
         const double ph0 = -(double)task_wp[k] * winc;
 #pragma unroll
@@ -519,38 +679,51 @@ static int band_height_of(const rg_grid_params *p, bool fp32)
     return 8;
 }
 
+struct BandLaunch {
+    const void *recs;
+    const Item *items;
+    const uint32_t *item_start;
+    uint32_t key_begin, key_end; // keys of the batch's tasks
+    const int32_t *rank_of_slot;
+    int t0;
+    const void *kern, *tw;
+    void *G, *spill;
+    uint32_t spill_cap;
+    uint32_t *spill_ctr, *item_spill;
+    uint32_t blocks; // bound on the batch's items
+};
+
 template <typename T, int S, int H, int LOGN>
-static int launch_bands_h(const Rec<T> *recs, const uint32_t *vis_start, int nk,
-                          const int32_t *task_slot, int t0, int ntask, const void *kern,
-                          const void *tw, void *G, uint8_t *flags, hipStream_t s)
+static int launch_bands_h(const BandLaunch &a, hipStream_t s)
 {
     using T2 = typename Vec2<T>::type;
-    constexpr int n = 1 << LOGN;
-    hipLaunchKernelGGL((k_tower_bands<T, S, H, LOGN>), dim3(n / H, ntask), dim3(S * S), 0, s, recs,
-                       vis_start, nk, task_slot, t0, (const T2 *)kern, (const T2 *)tw, (T2 *)G,
-                       flags);
+    constexpr int GRP = S * H >= 256 ? 1 : 256 / (S * H);
+    hipLaunchKernelGGL((k_tower_band_items<T, S, H, LOGN, GRP>), dim3(a.blocks), dim3(S * H * GRP),
+                       0, s, (const Rec<T> *)a.recs, a.items, a.item_start, a.key_begin,
+                       a.key_end, a.rank_of_slot, a.t0, (const T2 *)a.kern, (const T2 *)a.tw,
+                       (T2 *)a.G, (T2 *)a.spill, a.spill_cap, a.spill_ctr, a.item_spill);
+    RG_HIP(hipGetLastError());
+    hipLaunchKernelGGL((k_band_combine<T2, H, LOGN>), dim3(a.blocks), dim3(256), 0, s, a.items,
+                       a.item_start, a.key_begin, a.key_end, a.rank_of_slot, a.t0, (T2 *)a.G,
+                       (const T2 *)a.spill, a.spill_cap, (const uint32_t *)a.item_spill);
     RG_HIP(hipGetLastError());
     return RG_OK;
 }
 
 template <typename T, int S, int LOGN>
-static int launch_bands_t(int H, const Rec<T> *recs, const uint32_t *vis_start, int nk,
-                          const int32_t *task_slot, int t0, int ntask, const void *kern,
-                          const void *tw, void *G, uint8_t *flags, hipStream_t s)
+static int launch_bands_t(int H, const BandLaunch &a, hipStream_t s)
 {
     if constexpr (sizeof(T) == 4) {
         if (H == 16)
-            return launch_bands_h<T, S, 16, LOGN>(recs, vis_start, nk, task_slot, t0, ntask, kern,
-                                                  tw, G, flags, s);
+            return launch_bands_h<T, S, 16, LOGN>(a, s);
     }
-    return launch_bands_h<T, S, 8, LOGN>(recs, vis_start, nk, task_slot, t0, ntask, kern, tw, G,
-                                         flags, s);
+    return launch_bands_h<T, S, 8, LOGN>(a, s);
 }
 
 template <typename T, int LOGN, int H>
-static int launch_cols_h(const void *G, const uint8_t *flags, void *P, const int32_t *cf,
-                         const int32_t *cc, int chunk_base, int nchunks, int32_t t0,
-                         const int32_t *twp, const void *tw, double theta, double winc,
+static int launch_cols_h(const void *G, const uint32_t *ks, const int32_t *tslot, void *P,
+                         const int32_t *cf, const int32_t *cc, int chunk_base, int nchunks,
+                         int32_t t0, const int32_t *twp, const void *tw, double theta, double winc,
                          hipStream_t s)
 {
     using T2 = typename Vec2<T>::type;
@@ -558,50 +731,48 @@ static int launch_cols_h(const void *G, const uint8_t *flags, void *P, const int
     constexpr int CB0 = sizeof(T) == 4 ? 16 : 8;
     constexpr int CB = CB0 > n ? n : CB0;
     hipLaunchKernelGGL((k_tower_cols<T, LOGN, CB, H>), dim3(n / CB, nchunks), dim3(CB * (n / 8)),
-                       0, s, (const T2 *)G, flags, (T2 *)P, cf, cc, chunk_base, t0, twp,
+                       0, s, (const T2 *)G, ks, tslot, (T2 *)P, cf, cc, chunk_base, t0, twp,
                        (const T2 *)tw, theta, winc);
     RG_HIP(hipGetLastError());
     return RG_OK;
 }
 
 template <typename T, int LOGN>
-static int launch_cols_t(int H, const void *G, const uint8_t *flags, void *P, const int32_t *cf,
-                         const int32_t *cc, int chunk_base, int nchunks, int32_t t0,
-                         const int32_t *twp, const void *tw, double theta, double winc,
+static int launch_cols_t(int H, const void *G, const uint32_t *ks, const int32_t *tslot, void *P,
+                         const int32_t *cf, const int32_t *cc, int chunk_base, int nchunks,
+                         int32_t t0, const int32_t *twp, const void *tw, double theta, double winc,
                          hipStream_t s)
 {
     if (H == 16)
-        return launch_cols_h<T, LOGN, 16>(G, flags, P, cf, cc, chunk_base, nchunks, t0, twp, tw,
-                                          theta, winc, s);
-    return launch_cols_h<T, LOGN, 8>(G, flags, P, cf, cc, chunk_base, nchunks, t0, twp, tw, theta,
-                                     winc, s);
+        return launch_cols_h<T, LOGN, 16>(G, ks, tslot, P, cf, cc, chunk_base, nchunks, t0, twp,
+                                          tw, theta, winc, s);
+    return launch_cols_h<T, LOGN, 8>(G, ks, tslot, P, cf, cc, chunk_base, nchunks, t0, twp, tw,
+                                     theta, winc, s);
 }
 
 template <typename T, int S>
-static int launch_bands(int n, int H, const Rec<T> *recs, const uint32_t *vis_start, int nk,
-                        const int32_t *task_slot, int t0, int ntask, const void *kern,
-                        const void *tw, void *G, uint8_t *flags, hipStream_t s)
+static int launch_bands(int n, int H, const BandLaunch &a, hipStream_t s)
 {
     switch (n) {
-    case 64: return launch_bands_t<T, S, 6>(H, recs, vis_start, nk, task_slot, t0, ntask, kern, tw, G, flags, s);
-    case 128: return launch_bands_t<T, S, 7>(H, recs, vis_start, nk, task_slot, t0, ntask, kern, tw, G, flags, s);
-    case 256: return launch_bands_t<T, S, 8>(H, recs, vis_start, nk, task_slot, t0, ntask, kern, tw, G, flags, s);
-    case 512: return launch_bands_t<T, S, 9>(H, recs, vis_start, nk, task_slot, t0, ntask, kern, tw, G, flags, s);
+    case 64: return launch_bands_t<T, S, 6>(H, a, s);
+    case 128: return launch_bands_t<T, S, 7>(H, a, s);
+    case 256: return launch_bands_t<T, S, 8>(H, a, s);
+    case 512: return launch_bands_t<T, S, 9>(H, a, s);
     default: return fail(RG_ERR_UNSUPPORTED, "batched W-towers supports subgrid sizes 64..512 (got %d)", n);
     }
 }
 
 template <typename T>
-static int launch_cols(int n, int H, const void *G, const uint8_t *flags, void *P, const int32_t *cf,
-                       const int32_t *cc, int chunk_base, int nchunks, int32_t t0,
-                       const int32_t *twp, const void *tw, double theta, double winc,
+static int launch_cols(int n, int H, const void *G, const uint32_t *ks, const int32_t *tslot,
+                       void *P, const int32_t *cf, const int32_t *cc, int chunk_base, int nchunks,
+                       int32_t t0, const int32_t *twp, const void *tw, double theta, double winc,
                        hipStream_t s)
 {
     switch (n) {
-    case 64: return launch_cols_t<T, 6>(H, G, flags, P, cf, cc, chunk_base, nchunks, t0, twp, tw, theta, winc, s);
-    case 128: return launch_cols_t<T, 7>(H, G, flags, P, cf, cc, chunk_base, nchunks, t0, twp, tw, theta, winc, s);
-    case 256: return launch_cols_t<T, 8>(H, G, flags, P, cf, cc, chunk_base, nchunks, t0, twp, tw, theta, winc, s);
-    case 512: return launch_cols_t<T, 9>(H, G, flags, P, cf, cc, chunk_base, nchunks, t0, twp, tw, theta, winc, s);
+    case 64: return launch_cols_t<T, 6>(H, G, ks, tslot, P, cf, cc, chunk_base, nchunks, t0, twp, tw, theta, winc, s);
+    case 128: return launch_cols_t<T, 7>(H, G, ks, tslot, P, cf, cc, chunk_base, nchunks, t0, twp, tw, theta, winc, s);
+    case 256: return launch_cols_t<T, 8>(H, G, ks, tslot, P, cf, cc, chunk_base, nchunks, t0, twp, tw, theta, winc, s);
+    case 512: return launch_cols_t<T, 9>(H, G, ks, tslot, P, cf, cc, chunk_base, nchunks, t0, twp, tw, theta, winc, s);
     default: return fail(RG_ERR_UNSUPPORTED, "batched W-towers supports subgrid sizes 64..512 (got %d)", n);
     }
 }
@@ -622,6 +793,7 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
         return fail(RG_ERR_UNSUPPORTED, "batched W-towers supports subgrid sizes 64..512 (got %d)", n);
     if (n + 2 * S >= 65536)
         return fail(RG_ERR_UNSUPPORTED, "subgrid too large");
+    static_assert(SORT_BINS >= 512 + 16, "sort bins must cover n + S");
 
     RG_TRY(zero_fill(grid, sizeof(double2) * (size_t)N * N, s));
     if (vis->n == 0)
@@ -672,11 +844,15 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
     kf.wp_min = wp_min;
     kf.nwp = nwp;
     kf.band_h = H;
-    kf.nk = H > 0 ? (n + S - 1) / H + 1 : 1;
+    kf.nk = H > 0 ? n / H : 1;
+    kf.mk = H > 0 ? (S - 2) / H + 2 : 1; // bands an S-row footprint can reach
     const int64_t nslots = (int64_t)cc * cc * nwp;
     const int64_t nkeys = nslots * kf.nk;
-    // work items are not used by this path: one item per key keeps the list small
-    const uint32_t per_item = 0x40000000u;
+    // plain mode reads whole tasks and takes no work items: one item per key
+    // keeps the list small.  Batched mode: items of at most K records per band.
+    const uint32_t K = plain ? 0x40000000u
+                             : (p->vis_per_item > 0 ? (uint32_t)p->vis_per_item : 2048u);
+    const uint32_t per_item = K;
 
     BinResult br;
     RG_TRY((bin_visibilities<TowerKeyer, T>(ctx, kf, vis->n, nkeys, per_item, s, br)));
@@ -764,8 +940,14 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
     const size_t o_rot = 0, o_slot = o_rot + (size_t)cc * cc, o_wp = o_slot + ntasks,
                  o_cf = o_wp + ntasks, o_cc = o_cf + chunk_first.size(),
                  o_rt = o_cc + chunk_count.size(), o_rc0 = o_rt + run_tower.size(),
-                 o_rnc = o_rc0 + run_chunk0.size(), o_end = o_rnc + run_nchunk.size();
+                 o_rnc = o_rc0 + run_chunk0.size(), o_ros = o_rnc + run_nchunk.size(),
+                 o_end = o_ros + (plain ? 0 : (size_t)nslots);
     tables.resize(o_end);
+    if (!plain) {
+        std::fill(tables.begin() + o_ros, tables.end(), -1);
+        for (int k = 0; k < ntasks; ++k)
+            tables[o_ros + tl.slot[k]] = k;
+    }
     std::copy(tl.rank_of_tower.begin(), tl.rank_of_tower.end(), tables.begin() + o_rot);
     std::copy(tl.slot.begin(), tl.slot.end(), tables.begin() + o_slot);
     std::copy(tl.wp.begin(), tl.wp.end(), tables.begin() + o_wp);
@@ -826,30 +1008,80 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
         }
     } else {
         const int NB = n / H;
+        const int nbatch = (ntasks + Q - 1) / Q;
+        // per batch: bound on its work items (grid of the band launch) and
+        // spill slots (parts of split keys: at most 2 R / K for R records)
+        std::vector<uint32_t> blocks(nbatch);
+        uint64_t max_blocks = 1, max_spill = 1;
+        for (int bi = 0; bi < nbatch; ++bi) {
+            uint64_t nb = 0, rec = 0;
+            for (int k = bi * Q; k < std::min(ntasks, (bi + 1) * Q); ++k) {
+                nb += std::min<uint64_t>((uint64_t)NB, tl.nvis[k]) + tl.nvis[k] / K;
+                rec += tl.nvis[k];
+            }
+            blocks[bi] = (uint32_t)std::min<uint64_t>(nb + 1, 0x7fffffffu);
+            max_blocks = std::max<uint64_t>(max_blocks, nb + 1);
+            max_spill = std::max<uint64_t>(max_spill, 2 * rec / K + 1);
+        }
+        const size_t band_bytes = sizeof(T2) * (size_t)H * n;
         const size_t g_bytes = sizeof(T2) * nn * Q;
         const size_t p_bytes = sizeof(T2) * nn * (size_t)max_chunks;
-        RG_TRY(ctx->scratch[7].ensure(g_bytes + p_bytes + (size_t)Q * NB + 256));
-        T2 *G = ctx->scratch[7].as<T2>();
-        T2 *Pp = reinterpret_cast<T2 *>(ctx->scratch[7].as<char>() + g_bytes);
-        uint8_t *flags = reinterpret_cast<uint8_t *>(ctx->scratch[7].as<char>() + g_bytes + p_bytes);
-        const Rec<T> *recs = (const Rec<T> *)br.recs;
+        const size_t sp_bytes = band_bytes * max_spill;
+        const size_t is_bytes = (sizeof(uint32_t) * max_blocks + 255) / 256 * 256;
+        const size_t ctr_bytes = (sizeof(uint32_t) * nbatch + 255) / 256 * 256;
+        RG_TRY(ctx->scratch[7].ensure(g_bytes + p_bytes + sp_bytes + is_bytes + ctr_bytes));
+        char *base7 = ctx->scratch[7].as<char>();
+        T2 *G = reinterpret_cast<T2 *>(base7);
+        T2 *Pp = reinterpret_cast<T2 *>(base7 + g_bytes);
+        T2 *spill = reinterpret_cast<T2 *>(base7 + g_bytes + p_bytes);
+        uint32_t *item_spill = reinterpret_cast<uint32_t *>(base7 + g_bytes + p_bytes + sp_bytes);
+        uint32_t *spill_ctr =
+            reinterpret_cast<uint32_t *>(base7 + g_bytes + p_bytes + sp_bytes + is_bytes);
+        RG_TRY(zero_fill(spill_ctr, sizeof(uint32_t) * nbatch, s));
+        // records of every band item sorted by footprint column
+        const size_t nrec = (size_t)vis->n * kf.mk;
+        RG_TRY(ctx->scratch[11].ensure(sizeof(Rec<T>) * (nrec > 0 ? nrec : 1)));
+        Rec<T> *sorted = ctx->scratch[11].as<Rec<T>>();
+        {
+            KernelTimer kt(ctx, RG_TIMER_BINNING, s);
+            if (br.max_items > 0)
+                hipLaunchKernelGGL((k_sort_items<T>), dim3((unsigned)br.max_items), dim3(256), 0, s,
+                                   (const Rec<T> *)br.recs, sorted, (const Item *)br.items,
+                                   (const unsigned long long *)ctx->counters.p, n + S);
+            RG_HIP(hipGetLastError());
+        }
+        BandLaunch bl;
+        bl.recs = sorted;
+        bl.items = br.items;
+        bl.item_start = br.item_start;
+        bl.rank_of_slot = d_tab + o_ros;
+        bl.kern = wk->data;
+        bl.tw = tw_inv;
+        bl.G = G;
+        bl.spill = spill;
+        bl.spill_cap = (uint32_t)max_spill;
+        bl.item_spill = item_spill;
         int b = 0;
         for (int t0 = 0; t0 < ntasks; t0 += Q, ++b) {
             const int t1 = std::min(ntasks, t0 + Q);
+            bl.key_begin = (uint32_t)((int64_t)tl.slot[t0] * NB);
+            bl.key_end = (uint32_t)((int64_t)(tl.slot[t1 - 1] + 1) * NB);
+            bl.t0 = t0;
+            bl.spill_ctr = spill_ctr + b;
+            bl.blocks = blocks[b];
             {
                 KernelTimer kt(ctx, RG_TIMER_TOWER_TILES, s);
                 if (S == 16)
-                    RG_TRY((launch_bands<T, 16>(n, H, recs, br.vis_start, kf.nk, d_tab + o_slot, t0,
-                                               t1 - t0, wk->data, tw_inv, G, flags, s)));
+                    RG_TRY((launch_bands<T, 16>(n, H, bl, s)));
                 else
-                    RG_TRY((launch_bands<T, 8>(n, H, recs, br.vis_start, kf.nk, d_tab + o_slot, t0,
-                                              t1 - t0, wk->data, tw_inv, G, flags, s)));
+                    RG_TRY((launch_bands<T, 8>(n, H, bl, s)));
             }
             const int cb0 = batch_chunk0[b], ncb = batch_chunk0[b + 1] - batch_chunk0[b];
             {
                 KernelTimer kt(ctx, RG_TIMER_TOWER_COLS, s);
-                RG_TRY(launch_cols<T>(n, H, G, flags, Pp, d_tab + o_cf, d_tab + o_cc, cb0, ncb, t0,
-                                      d_tab + o_wp, tw_inv, p->theta, p->winc, s));
+                RG_TRY(launch_cols<T>(n, H, G, br.vis_start, d_tab + o_slot, Pp, d_tab + o_cf,
+                                      d_tab + o_cc, cb0, ncb, t0, d_tab + o_wp, tw_inv, p->theta,
+                                      p->winc, s));
             }
             const int rb = batch_run0[b], nr = batch_run0[b + 1] - batch_run0[b];
             {

@@ -170,7 +170,7 @@ def wkern_desc(table, w_min: float, w_step: float) -> WKern:
 
 
 def params(grid_size, theta, mode=RG_MODE_BATCHED, subgrid=0, margin=0, winc=0.0, tile=0,
-           vis_per_item=0, compensated=1, floor_batch=0) -> GridParams:
+           vis_per_item=0, compensated=0, floor_batch=0) -> GridParams:
     return GridParams(int(grid_size), int(mode), float(theta), int(subgrid), int(margin),
                       float(winc), int(tile), int(vis_per_item), int(compensated), int(floor_batch))
 

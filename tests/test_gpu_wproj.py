@@ -143,3 +143,26 @@ def test_c1_full_size_fp64(ctx):
                  vis_dtype=torch.complex64, k_dtype=torch.complex64)
     img32, img64 = orc.grid_to_image(g32), orc.grid_to_image(ref)
     assert rel_max(img32, img64) < 1e-5 and rel_rms(img32, img64) < 1e-5
+
+
+def test_hot_tile_across_replicas(ctx):
+    """60 000 visibilities in one 32 x 32 tile (235 blocks of the sort, so every
+    counter replica holds part of the key) plus a sparse background: batched
+    FP64 equals the oracle."""
+    N, theta, S, R = 256, 0.1, 8, 4
+    rng = np.random.default_rng(21)
+    P = 5
+    table = rng.standard_normal((P, R, R, S, S)) + 1j * rng.standard_normal((P, R, R, S, S))
+    n_hot, n_bg = 60000, 5000
+    u = np.concatenate([rng.uniform(-100.0, 100.0, n_hot), rng.uniform(-1200.0, 1200.0, n_bg)])
+    v = np.concatenate([rng.uniform(-100.0, 100.0, n_hot), rng.uniform(-1200.0, 1200.0, n_bg)])
+    w = rng.uniform(-20.0, 20.0, n_hot + n_bg)
+    vis = rng.standard_normal(n_hot + n_bg) + 1j * rng.standard_normal(n_hot + n_bg)
+    ref, _, bad = orc.grid_wprojection(N, theta, u, v, w, vis, table, -20.0, 10.0)
+    assert bad == 0
+    obs = synth.Observation("hot", u, v, w, None, None, None, theta, N / theta, 0, 0, None)
+    for per_item in (0, 333):
+        g, st = run(ctx, obs, vis, table, -20.0, 10.0, rgamd.RG_MODE_BATCHED,
+                    vis_per_item=per_item)
+        assert rel_max(g, ref) < 1e-12
+        assert st.n_gridded == n_hot + n_bg and st.n_dropped == 0

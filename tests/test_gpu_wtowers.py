@@ -52,11 +52,16 @@ def test_batched_fp64_matches_oracle(ctx, case, floor_batch):
     assert st.n_gridded == obs.n_vis and st.n_dropped == 0
 
 
-def test_batched_fp32_within_tolerance(ctx, case):
+@pytest.mark.parametrize("band,per_item", [(8, 0), (8, 37), (16, 0), (16, 37)])
+def test_batched_fp32_within_tolerance(ctx, case, band, per_item):
+    """FP32 bands of 8 / 16 rows; per_item 37 splits most band keys over
+    several work items (spilled parts summed by the combine pass)."""
     obs, vis, table, ref = case
-    g, _ = run(ctx, obs, vis, table, rgamd.RG_MODE_BATCHED, dt=torch.complex64)
+    g, st = run(ctx, obs, vis, table, rgamd.RG_MODE_BATCHED, dt=torch.complex64, tile=band,
+                vis_per_item=per_item)
     img, img_ref = orc.grid_to_image(g), orc.grid_to_image(ref)
     assert rel_max(img, img_ref) < 1e-5 and rel_rms(img, img_ref) < 1e-5
+    assert st.n_gridded == obs.n_vis
 
 
 def test_task_count_matches_binning(ctx, case):
