@@ -152,4 +152,27 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t *count,
     }
 }
 
+int items_from_lengths(rg_context *ctx, const uint32_t *len, int64_t nkeys, uint32_t per_item,
+                       uint32_t *prefix, uint32_t *item_start, Item *items, hipStream_t s)
+{
+    const int64_t nblocks_scan = (nkeys + SCAN_TILE - 1) / SCAN_TILE;
+    if (nkeys <= 0 || nblocks_scan > 1024 * 64)
+        return fail(RG_ERR_PARAM, "invalid key space %lld", (long long)nkeys);
+    RG_TRY(ensure_counters(ctx));
+    RG_TRY(ctx->scratch[6].ensure(sizeof(uint2) * (nblocks_scan + 1)));
+    Replicas one;
+    one.mask = 0;
+    one.stride = nkeys + 1;
+    uint2 *bsums = ctx->scratch[6].as<uint2>();
+    auto *stats = ctx->counters.as<unsigned long long>();
+    hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nblocks_scan), dim3(SCAN_BLOCK), 0, s, len, one,
+                       nkeys, per_item, bsums);
+    hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, bsums, (int)nblocks_scan, stats);
+    // the single cursor array coincides with the prefix (same values)
+    hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nblocks_scan), dim3(SCAN_BLOCK), 0, s, len, one,
+                       nkeys, per_item, (const uint2 *)bsums, prefix, item_start, prefix, items);
+    RG_HIP(hipGetLastError());
+    return RG_OK;
+}
+
 } // namespace rg

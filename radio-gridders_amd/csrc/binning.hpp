@@ -162,4 +162,12 @@ template <typename Keyer, typename T>
 int bin_visibilities(rg_context *ctx, const Keyer &kf, int64_t n, int64_t nkeys, uint32_t per_item,
                      hipStream_t s, BinResult &out);
 
+// Work items over per-key lengths len[0..nkeys): prefix[k] = exclusive sum of
+// len, item_start[k] = first item of key k (both nkeys + 1 entries), items of
+// at most per_item entries with start = prefix[key] + q * per_item (callers
+// shift them to their own offsets).  Writes the item count to the context's
+// counters; uses scratch slot 6.
+int items_from_lengths(rg_context *ctx, const uint32_t *len, int64_t nkeys, uint32_t per_item,
+                       uint32_t *prefix, uint32_t *item_start, Item *items, hipStream_t s);
+
 } // namespace rg

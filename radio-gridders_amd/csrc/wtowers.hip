@@ -39,14 +39,14 @@ struct TowerKeyer {
     int n, chunk, cc;
     double winc;
     int wp_min, nwp;
-    // band_h == 0: one key per visibility, key = task.  Otherwise keys are
-    // task * nk + b for the nk = n / band_h bands of band_h subgrid rows, and a
-    // visibility has one record under every band its footprint rows
-    // [gy, gy + S) reach inside [0, n).
+    // band_h == 0: key = task.  Otherwise keys are task * nk + b for the
+    // nk = n / band_h bands of band_h subgrid rows, and a visibility's key is
+    // the first band its footprint rows [gy, gy + S) reach inside [0, n); the
+    // footprint reaches at most mk consecutive bands.
     // pos is always the raw subgrid corner + S: (gx + S) | (gy + S) << 16.
     int band_h, nk, mk;
 
-    __host__ __device__ int max_keys() const { return mk; }
+    __host__ __device__ int max_keys() const { return 1; }
 
     __device__ int classify(int64_t i, uint32_t &key, int &nkey, uint32_t &pos,
                             uint32_t &kidx) const
@@ -72,14 +72,8 @@ struct TowerKeyer {
         if (p < 0 || p >= P)
             return VIS_BAD_PLANE;
         const uint32_t task = (uint32_t)((cy * cc + cx) * nwp + (wp - wp_min));
-        if (band_h > 0) {
-            const int b0 = max(gy, 0) / band_h, b1 = min(gy + S - 1, n - 1) / band_h;
-            key = task * (uint32_t)nk + (uint32_t)b0;
-            nkey = b1 - b0 + 1;
-        } else {
-            key = task;
-            nkey = 1;
-        }
+        key = band_h > 0 ? task * (uint32_t)nk + (uint32_t)(max(gy, 0) / band_h) : task;
+        nkey = 1;
         pos = (uint32_t)(gx + S) | ((uint32_t)(gy + S) << 16);
         kidx = (uint32_t)((p * R + yf) * R + xf);
         return VIS_OK;
@@ -187,9 +181,11 @@ template <> struct Trig<double> {
 // ---- record order inside band items -------------------------------------------------------
 // A band lane's cell depends only on the footprint column gx (its row is fixed),
 // so records sorted by gx keep each lane on one cell for long runs and the
-// Romein register sum rarely flushes.  Counting sort of every item's records
-// by gx + S (bins < n + S <= SORT_BINS) from `in` into `out` (same offsets);
-// the order inside a bin follows LDS atomics.
+// Romein register sum rarely flushes.  Every band work item (window records
+// src = wstart[key] + (start - prefix[key]) .. + count of the binned array) is
+// counting-sorted by gx + S (bins < n + S <= SORT_BINS) into its own slot
+// [start, start + count) of the expanded array; the order inside a bin
+// follows LDS atomics.  Workgroups loop over the items.
 constexpr int SORT_BINS = 1024;
 
 template <typename T>
@@ -197,64 +193,86 @@ __global__ __launch_bounds__(256) void k_sort_items(const Rec<T> *__restrict__ i
                                                     Rec<T> *__restrict__ out,
                                                     const Item *__restrict__ items,
                                                     const unsigned long long *__restrict__ stats,
-                                                    int nbins)
+                                                    const uint32_t *__restrict__ prefix,
+                                                    const uint32_t *__restrict__ wstart, int nbins)
 {
     __shared__ uint32_t bins[SORT_BINS];
     __shared__ uint32_t wsum[4];
-    const uint32_t it = blockIdx.x;
-    if (it >= (uint32_t)stats[CNT_ITEMS])
-        return;
-    const Item item = items[it];
-    const Rec<T> *src = in + item.start;
-    Rec<T> *dst = out + item.start;
-    if (item.count <= 32) {
-        if (threadIdx.x < item.count)
-            dst[threadIdx.x] = src[threadIdx.x];
-        return;
-    }
-    for (int b = threadIdx.x; b < SORT_BINS; b += 256)
-        bins[b] = 0;
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < item.count; i += 256)
-        atomicAdd(&bins[min(src[i].pos & 0xffffu, (uint32_t)nbins - 1)], 1u);
-    __syncthreads();
-    // exclusive scan of the bins: 4 per thread, wavefront scans, wave totals
-    uint32_t v[4], tot = 0;
+    const uint32_t nitems = (uint32_t)stats[CNT_ITEMS];
+    for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const Item item = items[it];
+        const Rec<T> *src = in + (item.start - prefix[item.key] + wstart[item.key]);
+        Rec<T> *dst = out + item.start;
+        if (item.count <= 32) {
+            if (threadIdx.x < item.count)
+                dst[threadIdx.x] = src[threadIdx.x];
+            continue;
+        }
+        __syncthreads(); // bins of the previous item are no longer read
+        for (int b = threadIdx.x; b < SORT_BINS; b += 256)
+            bins[b] = 0;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < item.count; i += 256)
+            atomicAdd(&bins[min(src[i].pos & 0xffffu, (uint32_t)nbins - 1)], 1u);
+        __syncthreads();
+        // exclusive scan of the bins: 4 per thread, wavefront scans, wave totals
+        uint32_t v[4], tot = 0;
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        v[q] = bins[threadIdx.x * 4 + q];
-        tot += v[q];
-    }
-    const int lane = __lane_id(), wv = threadIdx.x / 64;
-    uint32_t inc = tot;
+        for (int q = 0; q < 4; ++q) {
+            v[q] = bins[threadIdx.x * 4 + q];
+            tot += v[q];
+        }
+        const int lane = __lane_id(), wv = threadIdx.x / 64;
+        uint32_t inc = tot;
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = (uint32_t)__shfl_up((int)inc, o);
-        if (lane >= o)
-            inc += y;
-    }
-    if (lane == 63)
-        wsum[wv] = inc;
-    __syncthreads();
-    uint32_t run = inc - tot;
-    for (int w2 = 0; w2 < wv; ++w2)
-        run += wsum[w2];
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_up((int)inc, o);
+            if (lane >= o)
+                inc += y;
+        }
+        if (lane == 63)
+            wsum[wv] = inc;
+        __syncthreads();
+        uint32_t run = inc - tot;
+        for (int w2 = 0; w2 < wv; ++w2)
+            run += wsum[w2];
+        __syncthreads(); // every thread has read wsum before the next item
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        bins[threadIdx.x * 4 + q] = run;
-        run += v[q];
-    }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < item.count; i += 256) {
-        const Rec<T> r = src[i];
-        const uint32_t d = atomicAdd(&bins[min(r.pos & 0xffffu, (uint32_t)nbins - 1)], 1u);
-        dst[d] = r;
+        for (int q = 0; q < 4; ++q) {
+            bins[threadIdx.x * 4 + q] = run;
+            run += v[q];
+        }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < item.count; i += 256) {
+            const Rec<T> r = src[i];
+            const uint32_t d = atomicAdd(&bins[min(r.pos & 0xffffu, (uint32_t)nbins - 1)], 1u);
+            dst[d] = r;
+        }
     }
 }
 
+// ---- band windows ------------------------------------------------------------------------
+// Band b of a task takes the records of keys b - mk + 1 .. b (every footprint
+// that can reach rows [H b, H b + H)); they are contiguous in the binned
+// record array.  Per band key: window length and start.
+__global__ __launch_bounds__(256) void k_window_len(const uint32_t *__restrict__ key_start,
+                                                    int64_t nkeys, int nb, int mk, uint32_t *len,
+                                                    uint32_t *wstart)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nkeys)
+        return;
+    const int b = (int)(k % nb);
+    const int64_t k0 = k - min(b, mk - 1);
+    const uint32_t w0 = key_start[k0];
+    len[k] = key_start[k + 1] - w0;
+    wstart[k] = w0;
+}
+
 // ---- batched band pass -------------------------------------------------------------------
-// One workgroup per work item: up to K records of one band key (task, b), i.e.
-// visibilities whose footprint reaches subgrid rows [H b, H b + H).  The
+// One workgroup per work item: up to K records of the window of band b of a
+// task -- records of keys b - mk + 1 .. b, sorted by footprint column --
+// accumulated into subgrid rows [H b, H b + H).  The
 // workgroup is GRP groups of S x H lanes; the groups take alternate runs of the
 // item's records.  Lane (myU, myV) owns band row myV and the columns congruent
 // to myU modulo S: a footprint covering that row has exactly one cell there.
@@ -472,32 +490,23 @@ __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
         acc[q].x = T(0);
         acc[q].y = T(0);
     }
-    // The column block of task k + 1 is loaded into registers while task k is
-    // transformed in LDS.
-    T2 nxt[PER];
-    auto load = [&](int k) {
+    for (int k = first; k < first + count; ++k) {
         const T2 *g = G + (size_t)(k - t0) * n * n + c0;
-        // band b of the task holds data iff its key has records
+        // band b of the task was written iff it has work items
         const uint32_t *f = key_start + (size_t)task_slot[k] * NB;
+        __syncthreads();
 #pragma unroll
         for (int q = 0; q < PER; ++q) {
             const int e = threadIdx.x + q * NT;
             const int row = e / CB, col = e - row * CB;
-            nxt[q].x = T(0);
-            nxt[q].y = T(0);
+            T2 val;
+            val.x = T(0);
+            val.y = T(0);
             if (f[row / H + 1] > f[row / H])
-                nxt[q] = g[(size_t)row * n + col];
+                val = g[(size_t)row * n + col];
+            lds[e] = val;
         }
-    };
-    load(first);
-    for (int k = first; k < first + count; ++k) {
         __syncthreads();
-#pragma unroll
-        for (int q = 0; q < PER; ++q)
-            lds[threadIdx.x + q * NT] = nxt[q];
-        __syncthreads();
-        if (k + 1 < first + count)
-            load(k + 1);
         fft_block<T2, LOGN, CB, CB, true>(lds, tw);
         const double ph0 = -(double)task_wp[k] * winc;
 #pragma unroll
@@ -728,7 +737,7 @@ static int launch_cols_h(const void *G, const uint32_t *ks, const int32_t *tslot
 {
     using T2 = typename Vec2<T>::type;
     constexpr int n = 1 << LOGN;
-    constexpr int CB0 = sizeof(T) == 4 ? 16 : 8;
+    constexpr int CB0 = 8; // columns per workgroup (512 threads at n = 512)
     constexpr int CB = CB0 > n ? n : CB0;
     hipLaunchKernelGGL((k_tower_cols<T, LOGN, CB, H>), dim3(n / CB, nchunks), dim3(CB * (n / 8)),
                        0, s, (const T2 *)G, ks, tslot, (T2 *)P, cf, cc, chunk_base, t0, twp,
@@ -848,19 +857,21 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
     kf.mk = H > 0 ? (S - 2) / H + 2 : 1; // bands an S-row footprint can reach
     const int64_t nslots = (int64_t)cc * cc * nwp;
     const int64_t nkeys = nslots * kf.nk;
-    // plain mode reads whole tasks and takes no work items: one item per key
-    // keeps the list small.  Batched mode: items of at most K records per band.
-    const uint32_t K = plain ? 0x40000000u
-                             : (p->vis_per_item > 0 ? (uint32_t)p->vis_per_item : 2048u);
-    const uint32_t per_item = K;
+    // The sort makes one item per key (plain mode reads whole tasks).  Batched
+    // band work items of at most K records per band window are built after it.
+    const uint32_t K = p->vis_per_item > 0 ? (uint32_t)p->vis_per_item : 2048u;
+    const int64_t max_items_w = nkeys + (int64_t)kf.mk * vis->n / K + 1;
+    if (!plain)
+        RG_TRY(ctx->scratch[4].ensure(sizeof(Item) * std::max<int64_t>(nkeys + 1, max_items_w)));
 
     BinResult br;
-    RG_TRY((bin_visibilities<TowerKeyer, T>(ctx, kf, vis->n, nkeys, per_item, s, br)));
+    RG_TRY((bin_visibilities<TowerKeyer, T>(ctx, kf, vis->n, nkeys, 0x40000000u, s, br)));
 
     // visibilities per task slot (read back)
-    RG_TRY(ctx->scratch[9].ensure(sizeof(uint32_t) * 2 * nslots));
+    RG_TRY(ctx->scratch[9].ensure(sizeof(uint32_t) * (2 * nslots + nkeys + 1)));
     uint32_t *d_nvis = ctx->scratch[9].as<uint32_t>();
     uint32_t *d_item0 = d_nvis + nslots;
+    uint32_t *d_wstart = d_nvis + 2 * nslots;
     hipLaunchKernelGGL(k_task_info, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s,
                        (const uint32_t *)br.vis_start, (const uint32_t *)br.item_start, nslots,
                        kf.nk, d_nvis, d_item0);
@@ -907,7 +918,7 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
 
     // Batches of Q tasks; inside a batch each tower's run of tasks is cut into
     // chunks of at most TPB tasks (one column-pass workgroup each).
-    const int Q = plain ? 1 : (p->floor_batch > 0 ? p->floor_batch : 256);
+    const int Q = plain ? 1 : (p->floor_batch > 0 ? p->floor_batch : 512);
     const int TPB = 8;
     std::vector<int32_t> chunk_first, chunk_count, batch_chunk0, run_tower, run_chunk0, run_nchunk,
         batch_run0;
@@ -1016,8 +1027,9 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
         for (int bi = 0; bi < nbatch; ++bi) {
             uint64_t nb = 0, rec = 0;
             for (int k = bi * Q; k < std::min(ntasks, (bi + 1) * Q); ++k) {
-                nb += std::min<uint64_t>((uint64_t)NB, tl.nvis[k]) + tl.nvis[k] / K;
-                rec += tl.nvis[k];
+                const uint64_t wrec = (uint64_t)kf.mk * tl.nvis[k]; // window records
+                nb += std::min<uint64_t>((uint64_t)NB, wrec) + wrec / K;
+                rec += wrec;
             }
             blocks[bi] = (uint32_t)std::min<uint64_t>(nb + 1, 0x7fffffffu);
             max_blocks = std::max<uint64_t>(max_blocks, nb + 1);
@@ -1038,22 +1050,34 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
         uint32_t *spill_ctr =
             reinterpret_cast<uint32_t *>(base7 + g_bytes + p_bytes + sp_bytes + is_bytes);
         RG_TRY(zero_fill(spill_ctr, sizeof(uint32_t) * nbatch, s));
-        // records of every band item sorted by footprint column
-        const size_t nrec = (size_t)vis->n * kf.mk;
-        RG_TRY(ctx->scratch[11].ensure(sizeof(Rec<T>) * (nrec > 0 ? nrec : 1)));
+        // band window work items (item_start_w in the binning's item_start
+        // slot, window lengths and prefix in the count / cursor slots), then
+        // every item's records sorted by footprint column into the expanded
+        // array (window records laid out item after item)
+        const int64_t nexp = (int64_t)kf.mk * vis->n;
+        RG_TRY(ctx->scratch[11].ensure(sizeof(Rec<T>) * (size_t)std::max<int64_t>(nexp, 1)));
         Rec<T> *sorted = ctx->scratch[11].as<Rec<T>>();
+        uint32_t *item_start_w = br.item_start;
+        Item *items_w = br.items;
         {
             KernelTimer kt(ctx, RG_TIMER_BINNING, s);
-            if (br.max_items > 0)
-                hipLaunchKernelGGL((k_sort_items<T>), dim3((unsigned)br.max_items), dim3(256), 0, s,
-                                   (const Rec<T> *)br.recs, sorted, (const Item *)br.items,
-                                   (const unsigned long long *)ctx->counters.p, n + S);
+            uint32_t *len = ctx->scratch[0].as<uint32_t>();
+            uint32_t *prefix = ctx->scratch[3].as<uint32_t>();
+            hipLaunchKernelGGL(k_window_len, dim3((unsigned)((nkeys + 255) / 256)), dim3(256), 0, s,
+                               (const uint32_t *)br.vis_start, nkeys, NB, kf.mk, len, d_wstart);
+            RG_HIP(hipGetLastError());
+            RG_TRY(items_from_lengths(ctx, len, nkeys, K, prefix, item_start_w, items_w, s));
+            hipLaunchKernelGGL((k_sort_items<T>),
+                               dim3((unsigned)std::min<int64_t>(max_items_w, 8192)), dim3(256), 0, s,
+                               (const Rec<T> *)br.recs, sorted, (const Item *)items_w,
+                               (const unsigned long long *)ctx->counters.p, (const uint32_t *)prefix,
+                               (const uint32_t *)d_wstart, n + S);
             RG_HIP(hipGetLastError());
         }
         BandLaunch bl;
         bl.recs = sorted;
-        bl.items = br.items;
-        bl.item_start = br.item_start;
+        bl.items = items_w;
+        bl.item_start = item_start_w;
         bl.rank_of_slot = d_tab + o_ros;
         bl.kern = wk->data;
         bl.tw = tw_inv;
@@ -1079,7 +1103,7 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
             const int cb0 = batch_chunk0[b], ncb = batch_chunk0[b + 1] - batch_chunk0[b];
             {
                 KernelTimer kt(ctx, RG_TIMER_TOWER_COLS, s);
-                RG_TRY(launch_cols<T>(n, H, G, br.vis_start, d_tab + o_slot, Pp, d_tab + o_cf,
+                RG_TRY(launch_cols<T>(n, H, G, item_start_w, d_tab + o_slot, Pp, d_tab + o_cf,
                                       d_tab + o_cc, cb0, ncb, t0, d_tab + o_wp, tw_inv, p->theta,
                                       p->winc, s));
             }
