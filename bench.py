@@ -10,7 +10,8 @@ the timed region; a step is one full gridding pass (sort, scatter, subgrid
 FFTs and w-screens, fold) over all visibilities.  The final N^2 image FFT is
 not part of a step.
 
-Prints ONE JSON line.  value = FP32 batched W-projection Mvis/s; the W-towers
+Prints ONE JSON line.  value = FP32 batched W-towers Mvis/s (the workload's
+own gridder), compared with the FP64 plain W-towers gridder; the W-projection
 numbers of the same stream are reported alongside under results.
 """
 import argparse
@@ -200,47 +201,23 @@ def main():
     if args.dft_pixels > 0 and dft_images:
         dft = dft_errors(ctx, fv128, N, obs.lam, theta, S, dft_images, args.dft_pixels)
 
-    # headline: FP32 batched W-projection (W-towers when only that ran)
-    head_kind = "wproj" if "wproj" in res else "wtowers"
+    # headline: FP32 batched W-towers (W-projection when only that ran)
+    head_kind = "wtowers" if "wtowers" in res else "wproj"
     head = res[head_kind]["fp32_batched"]
     plain = res[head_kind].get("fp64_plain")
-    # dominant kernel roofline (algorithmic bytes per launch / event-timed duration)
     nv = obs.n_vis
-    roof = None
     # the counter summaries were taken on the default 512-station, 64-dump workload
     default_workload = (args.stations, args.dumps) == (512, 64) and not args.tower_band \
         and not args.floor_batch
-    if head_kind == "wproj":
-        t = head["timers"].get("wproj_tiles")
-        per_vis = 24 + 8 + S * S * 8
-        byt = nv * per_vis
-        if t:
-            ach = byt / (t["ms_per_step"] / t["launches_per_step"] * 1e-3) / 1e9 / t["launches_per_step"]
-            avg_ms = t["ms_per_step"] / t["launches_per_step"]
-            traffic = profiled_traffic("wproj_tiles_fp32_c2.txt") if default_workload else None
-            roof = {"kernel": "k_tiles (W-projection)", "bound": "hbm", "achieved": ach,
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                    "traffic": traffic, "bytes_per_unit": per_vis, "unit_name": "visibility",
-                    "avg_launch_ms": avg_ms,
-                    "measured_GBps": (traffic / (avg_ms * 1e-3) / 1e9) if traffic else None}
-    else:
-        timers = head["timers"]
-        dom = max((k for k in timers if k.startswith("tower")), key=lambda k: timers[k]["ms_per_step"])
-        nt, n2 = head["tasks"], SUB * SUB
-        per_task = {"tower_cols": 3 * n2 * 8, "tower_rows": 2 * n2 * 8, "tower_tiles": n2 * 8}
-        if dom in per_task:
-            t = timers[dom]
-            prof = {"tower_tiles": "tower_bands_fp32_c2.txt", "tower_cols": "tower_cols_fp32_c2.txt"}
-            traffic = profiled_traffic(prof[dom]) if (default_workload and dom in prof) else None
-            ach = per_task[dom] * nt / (t["ms_per_step"] * 1e-3) / 1e9
-            roof = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
-                    "bytes_per_unit": per_task[dom], "unit_name": "(tower, floor) task",
-                    "avg_launch_ms": t["ms_per_step"] / t["launches_per_step"]}
+    roof, job = roofline(head_kind, head, nv, S, N, SUB, default_workload,
+                         table_bytes=(wp_planes if head_kind == "wproj" else 11) * 64 * S * S * 8)
 
     cpu = None
     if not args.skip_cpu:
-        cpu = cpu_baseline(obs, vis128, tab_p, wp_min, wp_step, args.cpu_seconds)
+        if head_kind == "wtowers":
+            cpu = cpu_baseline_wtowers(obs, vis128, tab_t, SUB, MARGIN, WINC, head["towers"])
+        else:
+            cpu = cpu_baseline(obs, vis128, tab_p, wp_min, wp_step, args.cpu_seconds)
 
     out = {
         "metric": METRIC,
@@ -265,10 +242,55 @@ def main():
         "error_vs_fp64": res[head_kind]["error_fp32_batched_vs_fp64"],
         "error_vs_dft": dft,
         "roofline": roof,
+        "job_roofline": job,
         "cpu_baseline": cpu,
         "results": res,
     }
     print(json.dumps(out), flush=True)
+
+
+def roofline(kind, head, nv, S, N, n, default_workload, table_bytes):
+    """Dominant-kernel and whole-step rooflines under the algorithmic count:
+    per visibility 24 (uvw) + 8 (vis) + S^2 x 8 (taps) bytes and 8 S^2 flop;
+    W-towers per (tower, floor) task 6 n^2 x 8 bytes (scatter write, row pass
+    read + write, column pass read + subimage read + write) and
+    5 n^2 log2(n^2) + 8 n^2 flop, per tower 4 n^2 x 8 bytes and the same
+    flop; per run the N^2 grid written once and the kernel table read once.
+    The band kernel (scatter + row pass) is charged the visibility bytes and
+    3 n^2 x 8 per task, the column pass 3 n^2 x 8 per task."""
+    per_vis = 24 + 8 + S * S * 8
+    timers = head["timers"]
+    if kind == "wproj":
+        name, tkey, prof = "k_tiles (W-projection)", "wproj_tiles", "wproj_tiles_fp32_c2.txt"
+        byt = nv * per_vis
+        unit = {"visibility": per_vis}
+        job_b = nv * per_vis + N * N * 8 + table_bytes
+        job_f = nv * 8 * S * S
+    else:
+        nt, ntow, n2 = head["tasks"], head["towers"], n * n
+        name, tkey, prof = ("k_tower_band_items (W-towers scatter + row IFFT)", "tower_tiles",
+                            "tower_bands_fp32_c2.txt")
+        byt = nv * per_vis + nt * 3 * n2 * 8
+        unit = {"visibility": per_vis, "(tower, floor) task": 3 * n2 * 8}
+        fft = 5 * n2 * math.log2(n2) + 8 * n2
+        job_b = nv * per_vis + nt * 6 * n2 * 8 + ntow * 4 * n2 * 8 + N * N * 16 + table_bytes
+        job_f = nv * 8 * S * S + (nt + ntow) * fft
+    t = timers.get(tkey)
+    roof = None
+    if t:
+        launches = t["launches_per_step"]
+        avg_ms = t["ms_per_step"] / launches
+        ach = byt / launches / (avg_ms * 1e-3) / 1e9
+        traffic = profiled_traffic(prof) if default_workload else None
+        roof = {"kernel": name, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                "bytes_per_launch": byt / launches, "bytes_per_unit": unit,
+                "launches_per_step": launches, "avg_launch_ms": avg_ms,
+                "measured_GBps": (traffic / (avg_ms * 1e-3) / 1e9) if traffic else None}
+    t_roof = max(job_b / (HBM_PEAK_GBS * 1e9), job_f / 157.3e12)
+    job = {"bytes": job_b, "flop": job_f, "t_roof_ms": t_roof * 1e3,
+           "frac": t_roof * 1e3 / head["ms_per_step"]}
+    return roof, job
 
 
 def dft_errors(ctx, fv, N, lam, theta, S, images, npix, seed=4):
@@ -331,6 +353,31 @@ def cpu_baseline(obs, vis128, tab_p, w_min, w_step, seconds):
     return {"value": ns / dt / 1e6, "unit": "Mvis/s", "cores": threads, "kind": "port",
             "sample": f"first {ns} visibilities of the stream, W-projection (OpenMP, row-partitioned, "
                       f"-O3 -ffast-math), 2048^2 grid",
+            "cpu": _cpu_model()}
+
+
+def cpu_baseline_wtowers(obs, vis128, tab_t, n, margin, winc, ntowers):
+    """The oracle's OpenMP W-towers (fast build, OpenMP over towers, as
+    grid.c:414) on host cores over the whole visibility stream (W-towers time
+    is dominated by the per-(tower, floor) FFTs, so a prefix of the stream is
+    not representative).  At most one thread per active tower does work."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import pyoracle as orc
+
+    threads = min(16, os.cpu_count() or 1)
+    ns = min(obs.n_vis, 8_000_000)
+    vis = vis128[:ns].cpu().numpy()
+    table = tab_t.cpu().numpy()
+    t = time.perf_counter()
+    orc.grid_wtowers(obs.grid_size, obs.theta, obs.u[:ns], obs.v[:ns], obs.w[:ns], vis, table,
+                     -5.0, 1.0, n, margin, winc, nthreads=threads, fast=True)
+    dt = time.perf_counter() - t
+    log(f"[bench] CPU W-towers: {ns} vis in {dt:.2f}s on {threads} threads")
+    busy = max(1, min(threads, ntowers))
+    return {"value": ns / dt / 1e6, "unit": "Mvis/s", "cores": busy, "kind": "port",
+            "sample": (f"{'all' if ns == obs.n_vis else 'first'} {ns} visibilities of the workload, "
+                       f"W-towers (OpenMP over towers: {threads} threads, {ntowers} active towers; "
+                       "-O3 -ffast-math, iterative radix-2 FFT), 2048^2 grid"),
             "cpu": _cpu_model()}
 
 

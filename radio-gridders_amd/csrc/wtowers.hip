@@ -296,9 +296,14 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
     constexpr int NL = S * H;
     constexpr int NT = NL * GRP;
     constexpr int NB = n / H;
+    // records staged RS at a time (RS / GRP per group): 128 keeps the FP32
+    // 8-row band at four workgroups per CU
+    constexpr int RS = 128;
+    constexpr int RG = RS / GRP;
+    static_assert(RS % GRP == 0 && RS <= NT, "record staging");
     __shared__ T2 band[H * n];
     __shared__ T2 tw[n];
-    __shared__ Rec<T> s_rec[NT];
+    __shared__ Rec<T> s_rec[RS];
     __shared__ uint32_t s_slot;
     const uint32_t ib = item_start[key_begin], ie = item_start[key_end];
     const uint32_t it = ib + blockIdx.x;
@@ -333,13 +338,13 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
     // U records at a time: U tap loads in flight, then the U updates in order.
     // Padding records lie outside every band.
     constexpr int U = 8;
-    static_assert(NL % U == 0, "record batch must be a multiple of the unroll");
-    for (uint32_t base = 0; base < item.count; base += NT) {
-        const uint32_t nloc = min((uint32_t)NT, item.count - base);
+    static_assert(RG % U == 0, "record batch must be a multiple of the unroll");
+    for (uint32_t base = 0; base < item.count; base += RS) {
+        const uint32_t nloc = min((uint32_t)RS, item.count - base);
         __syncthreads();
         if (threadIdx.x < nloc) {
             s_rec[threadIdx.x] = recs[item.start + base + threadIdx.x];
-        } else {
+        } else if (threadIdx.x < RS) {
             Rec<T> z;
             z.pos = 0xffffffffu;
             z.kidx = 0;
@@ -348,8 +353,8 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
             s_rec[threadIdx.x] = z;
         }
         __syncthreads();
-        const int ng = min(NL, max(0, (int)nloc - grp * NL));
-        const Rec<T> *gr = s_rec + grp * NL;
+        const int ng = min(RG, max(0, (int)nloc - grp * RG));
+        const Rec<T> *gr = s_rec + grp * RG;
         using Bt = RecBatch<T, U>;
         auto fetch = [&](uint32_t q0, Bt &bt) {
 #pragma unroll
