@@ -50,6 +50,35 @@ template <bool INV, typename T2> __device__ inline T2 mul_si(T2 a)
     return r;
 }
 
+// cos and sin of 2 pi t, t in turns (FP64).  t is split in FP64 into quarter
+// turns q + r (q integer, |r| <= 1/2, exact), sin and cos of (pi/2) r are
+// their Taylor series to degree 9 and 10 (truncation below 2e-9 for
+// |r| <= 1/2) evaluated in FP32, and q mod 4 rotates the pair by q right
+// angles.  Error about 1e-7, like an FP32 sincos of the reduced argument.
+__device__ inline void turn_sincos_f32(double t, float &c, float &s)
+{
+    const double t4 = 4.0 * t;
+    const double q = rint(t4);
+    const float r = (float)(t4 - q);
+    const int quad = (int)((long long)q & 3);
+    const float r2 = r * r;
+    // (pi/2)^k / k! with alternating signs
+    const float sn = r * (1.5707963267948966f +
+                          r2 * (-0.6459640975062462f +
+                                r2 * (0.07969262624616703f +
+                                      r2 * (-0.004681754135318687f +
+                                            r2 * 0.00016044118478735975f))));
+    const float cs = 1.0f + r2 * (-1.2337005501361697f +
+                                  r2 * (0.253669507901048f +
+                                        r2 * (-0.020863480763352957f +
+                                              r2 * (0.0009192602748394263f +
+                                                    r2 * -2.5202042373060596e-05f))));
+    const bool odd = quad & 1, neg_c = quad == 1 || quad == 2, neg_s = quad >= 2;
+    const float c0 = odd ? sn : cs, s0 = odd ? cs : sn;
+    c = neg_c ? -c0 : c0;
+    s = neg_s ? -s0 : s0;
+}
+
 template <bool INV, typename T2> __device__ inline void dft2(T2 &a, T2 &b)
 {
     const T2 t = a;
@@ -131,9 +160,11 @@ __device__ inline void fft_stage(T2 *lds, const T2 *tw, int logns)
         const int j = lt + q * TPL;
         const int k = j & (Ns - 1);
         const int tstep = k * (n >> (logns + (R == 8 ? 3 : (R == 4 ? 2 : 1))));
+        if (logns > 0) { // the first stage's twiddles are all 1
 #pragma unroll
-        for (int r = 1; r < R; ++r)
-            v[q * R + r] = cmul(v[q * R + r], tw[r * tstep]);
+            for (int r = 1; r < R; ++r)
+                v[q * R + r] = cmul(v[q * R + r], tw[r * tstep]);
+        }
         if (R == 8)
             dft8<INV>(&v[q * R]);
         else if (R == 4)
@@ -151,9 +182,11 @@ __device__ inline void fft_stage(T2 *lds, const T2 *tw, int logns)
 // ---- line-major variant ---------------------------------------------------------
 // Lines stored contiguously (line l at base + l * LSTR); thread t of a round
 // works on line t / TPL, position t % TPL.  `active` threads do the work, all
-// threads reach the barriers.
+// threads reach the barriers.  A stage given `out` writes its results there
+// (same layout as `line`) instead of back into LDS.
 template <typename T2, int LOGN, int R, bool INV>
-__device__ inline void fft_stage_lm(T2 *line, const T2 *tw, int logns, int lt, bool active)
+__device__ inline void fft_stage_lm(T2 *line, const T2 *tw, int logns, int lt, bool active,
+                                    T2 *out = nullptr)
 {
     constexpr int n = 1 << LOGN;
     constexpr int TPL = n / 8;
@@ -177,9 +210,11 @@ __device__ inline void fft_stage_lm(T2 *line, const T2 *tw, int logns, int lt, b
             const int j = lt + q * TPL;
             const int k = j & (Ns - 1);
             const int tstep = k * (n >> (logns + (R == 8 ? 3 : (R == 4 ? 2 : 1))));
+            if (logns > 0) { // the first stage's twiddles are all 1
 #pragma unroll
-            for (int r = 1; r < R; ++r)
-                v[q * R + r] = cmul(v[q * R + r], tw[r * tstep]);
+                for (int r = 1; r < R; ++r)
+                    v[q * R + r] = cmul(v[q * R + r], tw[r * tstep]);
+            }
             if (R == 8)
                 dft8<INV>(&v[q * R]);
             else if (R == 4)
@@ -187,18 +222,20 @@ __device__ inline void fft_stage_lm(T2 *line, const T2 *tw, int logns, int lt, b
             else
                 dft2<INV>(v[q * R], v[q * R + 1]);
             const int d = (j >> logns) * (Ns * R) + k;
+            T2 *dst = out ? out : line;
 #pragma unroll
             for (int r = 0; r < R; ++r)
-                line[d + r * Ns] = v[q * R + r];
+                dst[d + r * Ns] = v[q * R + r];
         }
     }
     __syncthreads();
 }
 
 // Transforms nl contiguous lines of length n = 2^LOGN (line stride LSTR) with
-// NT threads, B = NT / (n / 8) lines per round.
+// NT threads, B = NT / (n / 8) lines per round.  With `out` (line l at
+// out + l * n) the last stage stores the result there, not in LDS.
 template <typename T2, int LOGN, int NT, int LSTR, bool INV>
-__device__ inline void fft_lines_lm(T2 *lds, const T2 *tw, int nl)
+__device__ inline void fft_lines_lm(T2 *lds, const T2 *tw, int nl, T2 *out = nullptr)
 {
     constexpr int n = 1 << LOGN;
     constexpr int TPL = n / 8;
@@ -209,6 +246,7 @@ __device__ inline void fft_lines_lm(T2 *lds, const T2 *tw, int nl)
         const int l = l0 + (int)threadIdx.x / TPL;
         const bool active = (int)threadIdx.x < B * TPL && l < nl;
         T2 *line = lds + (size_t)(active ? l : 0) * LSTR;
+        T2 *oline = out ? out + (size_t)(active ? l : 0) * n : nullptr;
         int logns = 0;
 #pragma unroll
         for (int st = 0; st < 8; ++st) {
@@ -216,13 +254,15 @@ __device__ inline void fft_lines_lm(T2 *lds, const T2 *tw, int nl)
                 break;
             const int rem = LOGN - logns;
             if (rem >= 3) {
-                fft_stage_lm<T2, LOGN, 8, INV>(line, tw, logns, lt, active);
+                fft_stage_lm<T2, LOGN, 8, INV>(line, tw, logns, lt, active,
+                                               logns + 3 >= LOGN ? oline : nullptr);
                 logns += 3;
             } else if (rem == 2) {
-                fft_stage_lm<T2, LOGN, 4, INV>(line, tw, logns, lt, active);
+                fft_stage_lm<T2, LOGN, 4, INV>(line, tw, logns, lt, active,
+                                               logns + 2 >= LOGN ? oline : nullptr);
                 logns += 2;
             } else {
-                fft_stage_lm<T2, LOGN, 2, INV>(line, tw, logns, lt, active);
+                fft_stage_lm<T2, LOGN, 2, INV>(line, tw, logns, lt, active, oline);
                 logns += 1;
             }
         }

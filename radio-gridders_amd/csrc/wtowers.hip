@@ -164,11 +164,7 @@ __device__ inline double screen_base(int x, int y, int n, double theta)
 
 template <typename T> struct Trig;
 template <> struct Trig<float> {
-    __device__ static void rot(double rev, float &c, float &s)
-    {
-        const float f = (float)(rev - rint(rev));
-        sincospif(2.0f * f, &s, &c);
-    }
+    __device__ static void rot(double rev, float &c, float &s) { turn_sincos_f32(rev, c, s); }
 };
 template <> struct Trig<double> {
     __device__ static void rot(double rev, double &c, double &s)
@@ -391,9 +387,8 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
     }
     if (cur >= 0)
         flush();
-    __syncthreads();
-    fft_lines_lm<T2, LOGN, NT, n, true>(band, tw, H);
 
+    // destination: the task's subgrid band, or a spill slot for a split key
     const uint32_t k0 = item_start[item.key], k1 = item_start[item.key + 1];
     T2 *dst;
     if (k1 - k0 == 1) {
@@ -411,8 +406,9 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
             return;
         dst = spill + (size_t)s_slot * H * n;
     }
-    for (int e = threadIdx.x; e < H * n; e += NT)
-        dst[e] = band[e];
+    __syncthreads();
+    // row IFFT; the last stage stores straight to dst
+    fft_lines_lm<T2, LOGN, NT, n, true>(band, tw, H, dst);
 }
 
 // Sums the spilled parts of every key split over several items (in item
