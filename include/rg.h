@@ -93,7 +93,8 @@ typedef struct rg_grid_params {
     int32_t vis_per_item;   /* batched visibilities per work item (0 = default) */
     int32_t compensated;    /* batched FP32: 0 plain FP32 tile sums (default), 1 two-sum
                                compensated tile sums (worth it only into an RG_C128 grid) */
-    int32_t floor_batch;    /* batched W-towers: (tower, floor) tasks per batch (0 = default) */
+    int32_t floor_batch;    /* batched W-towers: (tower, floor) tasks per batch (0 = as many as
+                               fit 8 GiB of subgrids) */
 } rg_grid_params;
 
 /* Counters of the last rg_grid call (read back synchronously). */

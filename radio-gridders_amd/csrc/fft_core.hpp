@@ -182,11 +182,15 @@ __device__ inline void fft_stage(T2 *lds, const T2 *tw, int logns)
 // ---- line-major variant ---------------------------------------------------------
 // Lines stored contiguously (line l at base + l * LSTR); thread t of a round
 // works on line t / TPL, position t % TPL.  `active` threads do the work, all
-// threads reach the barriers.  A stage given `out` writes its results there
-// (same layout as `line`) instead of back into LDS.
-template <typename T2, int LOGN, int R, bool INV>
+// threads reach the barriers.  A stage given a store functor hands it its
+// results (position in the line, value) instead of writing them back to LDS.
+struct NoStore {
+    template <typename... A> __device__ void operator()(A &&...) const {}
+};
+
+template <typename T2, int LOGN, int R, bool INV, typename Store = NoStore>
 __device__ inline void fft_stage_lm(T2 *line, const T2 *tw, int logns, int lt, bool active,
-                                    T2 *out = nullptr)
+                                    const Store *out = nullptr)
 {
     constexpr int n = 1 << LOGN;
     constexpr int TPL = n / 8;
@@ -222,20 +226,23 @@ __device__ inline void fft_stage_lm(T2 *line, const T2 *tw, int logns, int lt, b
             else
                 dft2<INV>(v[q * R], v[q * R + 1]);
             const int d = (j >> logns) * (Ns * R) + k;
-            T2 *dst = out ? out : line;
 #pragma unroll
-            for (int r = 0; r < R; ++r)
-                dst[d + r * Ns] = v[q * R + r];
+            for (int r = 0; r < R; ++r) {
+                if (out)
+                    (*out)(d + r * Ns, v[q * R + r]);
+                else
+                    line[d + r * Ns] = v[q * R + r];
+            }
         }
     }
     __syncthreads();
 }
 
 // Transforms nl contiguous lines of length n = 2^LOGN (line stride LSTR) with
-// NT threads, B = NT / (n / 8) lines per round.  With `out` (line l at
-// out + l * n) the last stage stores the result there, not in LDS.
-template <typename T2, int LOGN, int NT, int LSTR, bool INV>
-__device__ inline void fft_lines_lm(T2 *lds, const T2 *tw, int nl, T2 *out = nullptr)
+// NT threads, B = NT / (n / 8) lines per round.  With `out`, the last stage
+// calls out(line, position, value) instead of storing the result in LDS.
+template <typename T2, int LOGN, int NT, int LSTR, bool INV, typename Out = NoStore>
+__device__ inline void fft_lines_lm(T2 *lds, const T2 *tw, int nl, const Out *out = nullptr)
 {
     constexpr int n = 1 << LOGN;
     constexpr int TPL = n / 8;
@@ -246,7 +253,10 @@ __device__ inline void fft_lines_lm(T2 *lds, const T2 *tw, int nl, T2 *out = nul
         const int l = l0 + (int)threadIdx.x / TPL;
         const bool active = (int)threadIdx.x < B * TPL && l < nl;
         T2 *line = lds + (size_t)(active ? l : 0) * LSTR;
-        T2 *oline = out ? out + (size_t)(active ? l : 0) * n : nullptr;
+        const int lo = active ? l : 0;
+        auto st_line = [&](int pos, const T2 &val) { (*out)(lo, pos, val); };
+        using SL = decltype(st_line);
+        const SL *oline = out ? &st_line : nullptr;
         int logns = 0;
 #pragma unroll
         for (int st = 0; st < 8; ++st) {

@@ -126,8 +126,11 @@ __global__ __launch_bounds__(S *S) void k_tiles(const Rec<T> *__restrict__ recs,
                     sr = T(0);
                     si = T(0);
                 }
-                sr += bt.k[q].x * bt.v[q].x + bt.k[q].y * bt.v[q].y;
-                si += bt.k[q].x * bt.v[q].y - bt.k[q].y * bt.v[q].x;
+                // conj(k) * v
+                sr = fma(bt.k[q].x, bt.v[q].x, sr);
+                sr = fma(bt.k[q].y, bt.v[q].y, sr);
+                si = fma(bt.k[q].x, bt.v[q].y, si);
+                si = fma(-bt.k[q].y, bt.v[q].x, si);
             }
         };
         pipelined_records<U, B>(nlocU, fetch, update);

@@ -174,6 +174,17 @@ template <> struct Trig<double> {
     }
 };
 
+// ---- subgrid layout in the batch buffer ---------------------------------------------------
+// A task's n x n subgrid is stored in column blocks of GCB columns: element
+// (row, col) at (col / GCB) * n * GCB + row * GCB + col % GCB, so the column
+// pass reads each of its blocks as one contiguous run.
+constexpr int GCB = 8;
+
+__device__ inline size_t gcb_index(int row, int col, int n)
+{
+    return (size_t)(col / GCB) * n * GCB + (size_t)row * GCB + (col % GCB);
+}
+
 // ---- record order inside band items -------------------------------------------------------
 // A band lane's cell depends only on the footprint column gx (its row is fixed),
 // so records sorted by gx keep each lane on one cell for long runs and the
@@ -297,7 +308,10 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
     constexpr int RS = 128;
     constexpr int RG = RS / GRP;
     static_assert(RS % GRP == 0 && RS <= NT, "record staging");
-    __shared__ T2 band[H * n];
+    // band rows padded to BS = n + 8 elements: the column-block copy-out
+    // reads 8 rows at once without all of them hitting the same LDS banks
+    constexpr int BS = n + 8;
+    __shared__ T2 band[H * BS];
     __shared__ T2 tw[n];
     __shared__ Rec<T> s_rec[RS];
     __shared__ uint32_t s_slot;
@@ -310,7 +324,7 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
     const int b = (int)(item.key - tslot * (uint32_t)NB);
     const int j = rank_of_slot[tslot] - t0;
 
-    for (int e = threadIdx.x; e < H * n; e += NT) {
+    for (int e = threadIdx.x; e < H * BS; e += NT) {
         band[e].x = T(0);
         band[e].y = T(0);
     }
@@ -320,7 +334,7 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
     const int myU = lane % S, myV = lane / S;
     const int row = H * b + myV;
     T *band_f = reinterpret_cast<T *>(band);
-    int cur = -1;
+    int cur = -1, carry = -1;
     T sr = T(0), si = T(0);
     auto flush = [&]() {
         if (GRP > 1) {
@@ -352,6 +366,9 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
         const int ng = min(RG, max(0, (int)nloc - grp * RG));
         const Rec<T> *gr = s_rec + grp * RG;
         using Bt = RecBatch<T, U>;
+        // A lane whose row the footprint misses (or whose column is off the
+        // subgrid) takes a zero tap and keeps its previous cell, so it adds
+        // nothing and never flushes: the update needs no second branch.
         auto fetch = [&](uint32_t q0, Bt &bt) {
 #pragma unroll
             for (int q = 0; q < U; ++q) {
@@ -361,26 +378,30 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
                 const int cx = (myU - gx) & (S - 1);
                 const int col = gx + cx;
                 const bool in = (unsigned)cy < (unsigned)S && (unsigned)col < (unsigned)n;
-                bt.cell[q] = in ? myV * n + col : -1;
+                carry = in ? myV * BS + col : carry;
+                bt.cell[q] = carry;
                 bt.v[q] = r.vis;
-                bt.k[q] = kern[in ? r.kidx * (uint32_t)(S * S) + (uint32_t)(cy * S + cx) : 0u];
+                T2 k = kern[in ? r.kidx * (uint32_t)(S * S) + (uint32_t)(cy * S + cx) : 0u];
+                k.x = in ? k.x : T(0);
+                k.y = in ? k.y : T(0);
+                bt.k[q] = k;
             }
         };
-        // lanes whose cell lies outside the footprint skip the record (no flush)
         auto update = [&](const Bt &bt) {
 #pragma unroll
             for (int q = 0; q < U; ++q) {
-                if (bt.cell[q] >= 0) {
-                    if (bt.cell[q] != cur) {
-                        if (cur >= 0)
-                            flush();
-                        cur = bt.cell[q];
-                        sr = T(0);
-                        si = T(0);
-                    }
-                    sr += bt.k[q].x * bt.v[q].x + bt.k[q].y * bt.v[q].y;
-                    si += bt.k[q].x * bt.v[q].y - bt.k[q].y * bt.v[q].x;
+                if (bt.cell[q] != cur) {
+                    if (cur >= 0)
+                        flush();
+                    cur = bt.cell[q];
+                    sr = T(0);
+                    si = T(0);
                 }
+                // conj(k) * v
+                sr = fma(bt.k[q].x, bt.v[q].x, sr);
+                sr = fma(bt.k[q].y, bt.v[q].y, sr);
+                si = fma(bt.k[q].x, bt.v[q].y, si);
+                si = fma(-bt.k[q].y, bt.v[q].x, si);
             }
         };
         pipelined_records<U, Bt>((uint32_t)((ng + U - 1) / U * U), fetch, update);
@@ -388,11 +409,13 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
     if (cur >= 0)
         flush();
 
-    // destination: the task's subgrid band, or a spill slot for a split key
+    // destination: the task's subgrid band (column blocks), or a spill slot
+    // (H rows of n) for a split key
     const uint32_t k0 = item_start[item.key], k1 = item_start[item.key + 1];
     T2 *dst;
-    if (k1 - k0 == 1) {
-        dst = G + (size_t)j * n * n + (size_t)b * H * n;
+    const bool whole = k1 - k0 == 1;
+    if (whole) {
+        dst = G + (size_t)j * n * n;
     } else {
         if (threadIdx.x == 0) {
             const uint32_t sl = atomicAdd(spill_ctr, 1u);
@@ -407,8 +430,21 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
         dst = spill + (size_t)s_slot * H * n;
     }
     __syncthreads();
-    // row IFFT; the last stage stores straight to dst
-    fft_lines_lm<T2, LOGN, NT, n, true>(band, tw, H, dst);
+    fft_lines_lm<T2, LOGN, NT, BS, true>(band, tw, H);
+    if (whole) {
+        // rows [H b, H b + H) of column block cb are H * GCB contiguous elements
+        T2 *d0 = dst + (size_t)H * b * GCB;
+        for (int e = threadIdx.x; e < H * n; e += NT) {
+            const int cb = e / (H * GCB), w = e - cb * (H * GCB);
+            const int l = w / GCB, c = cb * GCB + (w - l * GCB);
+            d0[(size_t)cb * n * GCB + w] = band[l * BS + c];
+        }
+    } else {
+        for (int e = threadIdx.x; e < H * n; e += NT) {
+            const int l = e / n, c = e - l * n;
+            dst[e] = band[l * BS + c];
+        }
+    }
 }
 
 // Sums the spilled parts of every key split over several items (in item
@@ -436,7 +472,7 @@ __global__ __launch_bounds__(256) void k_band_combine(const Item *__restrict__ i
     const uint32_t tslot = key / (uint32_t)NB;
     const int b = (int)(key - tslot * (uint32_t)NB);
     const int j = rank_of_slot[tslot] - t0;
-    T2 *dst = G + (size_t)j * n * n + (size_t)b * H * n;
+    T2 *dst = G + (size_t)j * n * n;
     for (int e = threadIdx.x; e < H * n; e += blockDim.x) {
         T2 a;
         a.x = 0;
@@ -449,7 +485,7 @@ __global__ __launch_bounds__(256) void k_band_combine(const Item *__restrict__ i
                 a.y += v.y;
             }
         }
-        dst[e] = a;
+        dst[gcb_index(H * b + e / n, e % n, n)] = a;
     }
 }
 
@@ -491,8 +527,9 @@ __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
         acc[q].x = T(0);
         acc[q].y = T(0);
     }
+    static_assert(CB == GCB || (1 << LOGN) < GCB, "column pass reads whole column blocks");
     for (int k = first; k < first + count; ++k) {
-        const T2 *g = G + (size_t)(k - t0) * n * n + c0;
+        const T2 *g = G + (size_t)(k - t0) * n * n + (size_t)blockIdx.x * n * CB;
         // band b of the task was written iff it has work items
         const uint32_t *f = key_start + (size_t)task_slot[k] * NB;
         __syncthreads();
@@ -504,7 +541,7 @@ __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
             val.x = T(0);
             val.y = T(0);
             if (f[row / H + 1] > f[row / H])
-                val = g[(size_t)row * n + col];
+                val = g[(size_t)row * CB + col];
             lds[e] = val;
         }
         __syncthreads();
@@ -918,8 +955,12 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
     RG_TRY(get_twiddles(ctx, n, sizeof(T) == 4 ? RG_C64 : RG_C128, +1, &tw_inv));
 
     // Batches of Q tasks; inside a batch each tower's run of tasks is cut into
-    // chunks of at most TPB tasks (one column-pass workgroup each).
-    const int Q = plain ? 1 : (p->floor_batch > 0 ? p->floor_batch : 512);
+    // chunks of at most TPB tasks (one column-pass workgroup each).  By
+    // default one batch holds every task while its subgrids fit 8 GiB of HBM
+    // (fewer, fuller launches).
+    constexpr size_t BATCH_BYTES = (size_t)8 << 30;
+    const int q_fit = (int)std::max<size_t>(1, BATCH_BYTES / (sizeof(T2) * nn));
+    const int Q = plain ? 1 : (p->floor_batch > 0 ? p->floor_batch : std::min(ntasks, q_fit));
     const int TPB = 8;
     std::vector<int32_t> chunk_first, chunk_count, batch_chunk0, run_tower, run_chunk0, run_nchunk,
         batch_run0;
