@@ -269,7 +269,7 @@ def roofline(kind, head, nv, S, N, n, default_workload, table_bytes):
     else:
         nt, ntow, n2 = head["tasks"], head["towers"], n * n
         name, tkey, prof = ("k_tower_band_items (W-towers scatter + row IFFT)", "tower_tiles",
-                            "tower_bands_fp32_c2.txt")
+                            "tower_band_items_fp32_c2.txt")
         byt = nv * per_vis + nt * 3 * n2 * 8
         unit = {"visibility": per_vis, "(tower, floor) task": 3 * n2 * 8}
         fft = 5 * n2 * math.log2(n2) + 8 * n2

@@ -112,7 +112,7 @@ struct rg_context {
     int last_kind = 0; // 0 none, 1 wproj, 2 wtowers
     int64_t last_nvis = 0;
     // generic scratch arrays
-    rg::DevBuf scratch[12];
+    rg::DevBuf scratch[16];
     std::map<rg::TwKey, rg::DevBuf> twiddles;
     // kernel timers (RG_TIMER_*): recorded (start, stop) event pairs
     bool timing = false;
