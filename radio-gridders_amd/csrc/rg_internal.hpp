@@ -113,6 +113,9 @@ struct rg_context {
     int64_t last_nvis = 0;
     // generic scratch arrays
     rg::DevBuf scratch[16];
+    // channel offsets resident in scratch[13] (uploaded only when they change,
+    // so a repeated multi-channel call issues no copy and can be captured)
+    std::vector<int64_t> chan_off;
     std::map<rg::TwKey, rg::DevBuf> twiddles;
     // kernel timers (RG_TIMER_*): recorded (start, stop) event pairs
     bool timing = false;

@@ -10,6 +10,8 @@
 // footprint (binning.hpp), coordinates computed once per visibility during
 // the sort (FP64, same formulas), then accumulated per tile in LDS by the
 // Romein kernel of tiles.hpp.
+#include <algorithm>
+
 #include "binning_impl.hpp"
 #include "tiles.hpp"
 
@@ -253,10 +255,18 @@ static int wproj_batched_t(rg_context *ctx, const rg_grid_params *p, const rg_fl
 
     const int64_t *d_off = nullptr;
     if (n_chan > 1) {
-        RG_TRY(ctx->scratch[7].ensure(sizeof(int64_t) * (n_chan + 1)));
-        RG_HIP(hipMemcpyAsync(ctx->scratch[7].p, offsets, sizeof(int64_t) * (n_chan + 1),
-                              hipMemcpyHostToDevice, s));
-        d_off = ctx->scratch[7].as<int64_t>();
+        // resident copy of the offsets: a call with the offsets of the previous
+        // multi-channel call issues no copy (and so can be graph-captured)
+        const bool same = ctx->chan_off.size() == (size_t)n_chan + 1 &&
+                          std::equal(ctx->chan_off.begin(), ctx->chan_off.end(), offsets);
+        if (!same) {
+            RG_TRY(ctx->scratch[13].ensure(sizeof(int64_t) * (n_chan + 1)));
+            RG_HIP(hipMemcpyAsync(ctx->scratch[13].p, offsets, sizeof(int64_t) * (n_chan + 1),
+                                  hipMemcpyHostToDevice, s));
+            RG_HIP(hipStreamSynchronize(s)); // the host array may go away after the call
+            ctx->chan_off.assign(offsets, offsets + n_chan + 1);
+        }
+        d_off = ctx->scratch[13].as<int64_t>();
     }
     const size_t gbytes = (size_t)(grid_type == RG_C128 ? 16 : 8) * N * N * n_chan;
     RG_TRY(zero_fill(grid, gbytes, s));

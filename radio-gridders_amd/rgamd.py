@@ -330,3 +330,42 @@ def write_wkern_hdf5(path, theta, table, w_min, w_step):
     P, R, _, S, _ = t.shape
     check(lib().rg_write_wkern_hdf5(path.encode(), float(theta), P, R, S, float(w_min),
                                     float(w_step), t.ctypes.data))
+
+
+def autotune(ctx, base: GridParams, fv: FlatVis, wk: WKern, grid, candidates, offsets=None,
+             reps: int = 3):
+    """Times every candidate (a dict of GridParams fields over `base`, e.g.
+    {"tile": 48, "vis_per_item": 1024}) on the caller's own data and returns
+    (best GridParams, [(candidate, ms)]).  Each candidate runs once to warm up
+    (scratch memory, caches), then `reps` times between device events; the
+    grid is overwritten."""
+    import torch
+
+    timings = []
+    best, best_ms = None, float("inf")
+    for cand in candidates:
+        p = GridParams()
+        for name, _ in GridParams._fields_:
+            setattr(p, name, getattr(base, name))
+        for k, v in cand.items():
+            setattr(p, k, v)
+
+        def run():
+            if offsets is None:
+                ctx.grid(p, fv, wk, grid)
+            else:
+                ctx.grid_channels(p, fv, offsets, wk, grid)
+
+        run()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            run()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        timings.append((dict(cand), ms))
+        if ms < best_ms:
+            best, best_ms = p, ms
+    return best, timings
+

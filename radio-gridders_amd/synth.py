@@ -161,6 +161,40 @@ def config_c2(seed: int = 1, n_st: int = 512, n_dumps: int = 64) -> Observation:
     return observe("C2", pos, n_dumps, 4.0, math.radians(-30.0), [FREQ_HZ], 50, 2, 0.1, 20480.0)
 
 
+def config_c4(seed: int = 1, n_st: int = 256, n_chan: int = 256, n_dumps: int = 64,
+              median: float = 40_000.0, sigma: float = 1.0, count_seed: int = 3,
+              f_lo: float = 100e6, f_hi: float = 200e6) -> Observation:
+    """Many channels with uneven counts: n_st stations (radius ~ Exp(3 km)), n_dumps
+    over 4 h at dec -30 deg, bl_max 10 km (first sample), n_chan channels linear in
+    [f_lo, f_hi], channel c keeps round(LogNormal(ln median, sigma)) (seed
+    count_seed) of its (baseline, dump) samples, drawn without replacement and kept
+    in baseline-track order.  The stream is channel-major (chan_offsets).  10
+    sources, theta 0.05, lambda 20480 (1024^2 grids)."""
+    pos = layout_radial(n_st, 3_000.0, np.random.default_rng(seed))
+    a1, a2, enu = baselines(pos)
+    uvw = uvw_tracks(enu, hour_angles(n_dumps, 4.0), math.radians(-30.0))
+    uvw = uvw[filter_first_sample(uvw, 0.0, 10_000.0)]
+    flat = uvw.reshape(-1, 3)  # baseline -> time
+    freqs = np.linspace(f_lo, f_hi, n_chan)
+    rng = np.random.default_rng(count_seed)
+    counts = np.minimum(np.rint(rng.lognormal(math.log(median), sigma, n_chan)).astype(np.int64),
+                        len(flat))
+    us, vs, ws, offs = [], [], [], [0]
+    for c in range(n_chan):
+        pick = np.sort(rng.choice(len(flat), size=int(counts[c]), replace=False))
+        sc = freqs[c] / C_LIGHT
+        us.append(flat[pick, 0] * sc)
+        vs.append(flat[pick, 1] * sc)
+        ws.append(flat[pick, 2] * sc)
+        offs.append(offs[-1] + int(counts[c]))
+    l, m, flux = sources(10, np.random.default_rng(2))
+    return Observation("C4", np.concatenate(us), np.concatenate(vs), np.concatenate(ws), l, m, flux,
+                       0.05, 20480.0, uvw.shape[0], n_dumps, freqs,
+                       chan_offsets=np.asarray(offs, np.int64),
+                       info={"stations": n_st, "channels": n_chan, "median": median,
+                             "sigma": sigma, "count_seed": count_seed})
+
+
 def config_small(seed: int = 1, n_st: int = 8, n_dumps: int = 16, arm_m: float = 600.0,
                  theta: float = 0.1, lam: float = 640.0, n_src: int = 3) -> Observation:
     """Small Y-layout observation for parity tests (N = theta * lambda)."""
