@@ -58,7 +58,9 @@ def main():
     ap.add_argument("--dumps", type=int, default=64)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true", help="skip the FP64 plain runs")
-    ap.add_argument("--only", choices=["all", "wproj", "wtowers"], default="all")
+    ap.add_argument("--only", choices=["all", "wproj", "wtowers", "channels"], default="all")
+    ap.add_argument("--skip-channels", action="store_true",
+                    help="skip the 256-channel batched W-projection workload")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--calibrate", action="store_true", help="run a 1 GiB copy for counter scaling")
     ap.add_argument("--dft-pixels", type=int, default=65536,
@@ -197,11 +199,21 @@ def main():
         dft_images["wtowers_fp32_batched"] = i32
         res["wtowers"] = r
 
+    if args.only in ("all", "channels") and not args.skip_channels:
+        try:
+            res["channels"] = bench_channels(ctx, K, W)
+        except Exception as e:  # reported, never hides the headline
+            log(f"[bench] channels workload failed: {e!r}")
+            res["channels"] = {"error": repr(e)}
+
     dft = None
     if args.dft_pixels > 0 and dft_images:
         dft = dft_errors(ctx, fv128, N, obs.lam, theta, S, dft_images, args.dft_pixels)
 
     # headline: FP32 batched W-towers (W-projection when only that ran)
+    if "wtowers" not in res and "wproj" not in res:
+        print(json.dumps({"metric": METRIC, "value": None, "results": res}), flush=True)
+        return
     head_kind = "wtowers" if "wtowers" in res else "wproj"
     head = res[head_kind]["fp32_batched"]
     plain = res[head_kind].get("fp64_plain")
@@ -247,6 +259,106 @@ def main():
         "results": res,
     }
     print(json.dumps(out), flush=True)
+
+
+def bench_channels(ctx, steps, warmup):
+    """256 channels (100-200 MHz) with lognormal counts (median 40 000, sigma 1)
+    from 256 stations, theta 0.05 (1024^2 FP32 grid per channel): one batched
+    FP32 call over all channels (eager and replayed from a captured graph)
+    against 256 plain FP64 per-channel calls; parameters from the autotuner."""
+    import torch
+
+    import rgamd
+    import synth
+
+    t0 = time.time()
+    obs = synth.config_c4()
+    off = np.ascontiguousarray(obs.chan_offsets, dtype=np.int64)
+    nc, N, nv = len(off) - 1, obs.grid_size, obs.n_vis
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    u, v, w = dev(obs.u), dev(obs.v), dev(obs.w)
+    vis128 = torch.empty(nv, dtype=torch.complex128, device="cuda")
+    rgamd.predict_points(u, v, w, dev(obs.l), dev(obs.m), dev(obs.flux), vis128)
+    vis64 = vis128.to(torch.complex64)
+    fv128, fv64 = rgamd.flat_vis(u, v, w, vis128), rgamd.flat_vis(u, v, w, vis64)
+    w_step = 10.0
+    w_min, P = synth.kernel_range(obs.w, w_step)
+    tab = dev(rgamd.make_wkernels(P, 8, 16, obs.theta, w_min, w_step))
+    tab32 = tab.to(torch.complex64)
+    k128, k64 = rgamd.wkern_desc(tab, w_min, w_step), rgamd.wkern_desc(tab32, w_min, w_step)
+    log(f"[bench] channels: {nv} visibilities in {nc} channels, {N}^2 grids, {P} W-planes "
+        f"({time.time() - t0:.1f}s)")
+    grids = torch.empty((nc, N, N), dtype=torch.complex64, device="cuda")
+    base = rgamd.params(N, obs.theta)
+    cands = [{"tile": t, "vis_per_item": k} for t in (32, 48) for k in (1024, 2048, 4096)]
+    best, table = rgamd.autotune(ctx, base, fv64, k64, grids, cands, offsets=off, reps=2)
+    for _ in range(warmup):
+        ctx.grid_channels(best, fv64, off, k64, grids)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        ctx.grid_channels(best, fv64, off, k64, grids)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    # the same call captured once and replayed
+    graph = torch.cuda.CUDAGraph()
+    gs = torch.cuda.Stream()
+    gs.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(gs):
+        with torch.cuda.graph(graph, stream=gs):
+            ctx.grid_channels(best, fv64, off, k64, grids, stream=gs.cuda_stream)
+    torch.cuda.current_stream().wait_stream(gs)
+    graph.replay()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        graph.replay()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_graph = e0.elapsed_time(e1) / steps
+    # plain FP64, one call per channel
+    g64 = torch.empty((N, N), dtype=torch.complex128, device="cuda")
+    plain = rgamd.params(N, obs.theta, rgamd.RG_MODE_PLAIN)
+    errs = []
+    check = set(np.linspace(0, nc - 1, 4).astype(int).tolist())
+    torch.cuda.synchronize()
+    t_plain = 0.0
+    for c in range(nc):
+        sub = rgamd.flat_vis(u[off[c]:off[c + 1]], v[off[c]:off[c + 1]], w[off[c]:off[c + 1]],
+                             vis128[off[c]:off[c + 1]])
+        e0.record()
+        ctx.grid(plain, sub, k128, g64)
+        e1.record()
+        torch.cuda.synchronize()
+        t_plain += e0.elapsed_time(e1)
+        if c in check:
+            a = grids[c].to(torch.complex128).clone()
+            b = g64.clone()
+            ctx.grid_to_image(a)
+            ctx.grid_to_image(b)
+            d = (a - b).abs()
+            errs.append({"channel": c,
+                         "max_rel": float(d.max() / b.abs().max()),
+                         "rms_rel": float(torch.sqrt((d ** 2).mean()) /
+                                          torch.sqrt((b.abs() ** 2).mean()))})
+    del graph, grids
+    out = {"n_vis": nv, "channels": nc, "grid": N, "w_planes": P,
+           "counts": {"min": int(np.diff(off).min()), "median": float(np.median(np.diff(off))),
+                      "max": int(np.diff(off).max())},
+           "autotune": {"chosen": {"tile": best.tile, "vis_per_item": best.vis_per_item},
+                        "ms": [{"tile": c["tile"], "vis_per_item": c["vis_per_item"], "ms": t}
+                               for c, t in table]},
+           "fp32_batched": {"ms_per_step": ms, "mvis_s": nv / ms / 1e3},
+           "fp32_batched_graph": {"ms_per_step": ms_graph, "mvis_s": nv / ms_graph / 1e3},
+           "fp64_plain_per_channel": {"ms_per_step": t_plain, "mvis_s": nv / t_plain / 1e3,
+                                      "launches": nc},
+           "speedup_vs_fp64_plain": t_plain / ms,
+           "error_fp32_batched_vs_fp64_plain": errs}
+    log(f"[bench] channels: batched {ms:.3f} ms ({nv / ms / 1e3:.1f} Mvis/s), graph replay "
+        f"{ms_graph:.3f} ms, plain per channel {t_plain:.2f} ms; autotune {out['autotune']}")
+    return out
 
 
 def roofline(kind, head, nv, S, N, n, default_workload, table_bytes):
