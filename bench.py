@@ -61,6 +61,9 @@ def main():
     ap.add_argument("--only", choices=["all", "wproj", "wtowers", "channels"], default="all")
     ap.add_argument("--skip-channels", action="store_true",
                     help="skip the 256-channel batched W-projection workload")
+    ap.add_argument("--c5", action="store_true",
+                    help="also run the stress workload (~260 M visibilities, 4096^2, W-towers "
+                         "subgrid 256); adds about a minute")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--calibrate", action="store_true", help="run a 1 GiB copy for counter scaling")
     ap.add_argument("--dft-pixels", type=int, default=65536,
@@ -205,6 +208,13 @@ def main():
         except Exception as e:  # reported, never hides the headline
             log(f"[bench] channels workload failed: {e!r}")
             res["channels"] = {"error": repr(e)}
+
+    if args.c5:
+        try:
+            res["stress"] = bench_stress(ctx, K, W)
+        except Exception as e:  # reported, never hides the headline
+            log(f"[bench] stress workload failed: {e!r}")
+            res["stress"] = {"error": repr(e)}
 
     dft = None
     if args.dft_pixels > 0 and dft_images:
@@ -361,6 +371,82 @@ def bench_channels(ctx, steps, warmup):
     return out
 
 
+def bench_stress(ctx, steps, warmup, npix=4096):
+    """Stress workload: 512 stations x 128 dumps x 16 channels (~260 M complex64
+    visibilities), phase centre at zenith angle 60 deg, 4096^2 master grid
+    (theta 0.2), W-towers subgrid 256, margin 16, winc 10.  FP32 batched timed;
+    FP64 batched once for the FP32 error gate; the FP32 image against the FP64
+    direct DFT on `npix` seeded pixels of the inner field.  W-kernels: 41
+    residual-w planes, step 1/4 wavelength."""
+    import torch
+
+    import rgamd
+    import synth
+
+    t0 = time.time()
+    obs = synth.config_c5()
+    nv, N = obs.n_vis, obs.grid_size
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    u, v, w = dev(obs.u), dev(obs.v), dev(obs.w)
+    del obs.u, obs.v
+    vis = torch.empty(nv, dtype=torch.complex64, device="cuda")
+    chunk = 1 << 25
+    for i in range(0, nv, chunk):  # predict in FP64, store complex64
+        j = min(nv, i + chunk)
+        tmp = torch.empty(j - i, dtype=torch.complex128, device="cuda")
+        rgamd.predict_points(u[i:j], v[i:j], w[i:j], dev(obs.l), dev(obs.m), dev(obs.flux), tmp)
+        vis[i:j] = tmp.to(torch.complex64)
+        del tmp
+    fv64 = rgamd.flat_vis(u, v, w, vis)
+    # residual-w table over [-winc/2, winc/2] in steps of 1/4 wavelength: at
+    # theta 0.2 the field edge has |n - 1| ~ 5e-3, so the plane quantisation
+    # (half a step) is kept to ~1e-3 turns
+    tab = dev(rgamd.make_wkernels(41, 8, 16, obs.theta, -5.0, 0.25))
+    k128 = rgamd.wkern_desc(tab, -5.0, 0.25)
+    tab32 = tab.to(torch.complex64)
+    k64 = rgamd.wkern_desc(tab32, -5.0, 0.25)
+    log(f"[bench] stress: {nv} visibilities, {N}^2 ({time.time() - t0:.1f}s)")
+    p = rgamd.params(N, obs.theta, rgamd.RG_MODE_BATCHED, subgrid=256, margin=16, winc=10.0)
+    g32 = torch.empty((N, N), dtype=torch.complex128, device="cuda")
+    for _ in range(warmup):
+        ctx.grid(p, fv64, k64, g32)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        ctx.grid(p, fv64, k64, g32)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    st = ctx.stats()
+    # FP64 batched reference on the same (complex64-rounded) visibilities
+    vis128 = vis.to(torch.complex128)
+    fv128 = rgamd.flat_vis(u, v, w, vis128)
+    g64 = torch.empty_like(g32)
+    e0.record()
+    ctx.grid(p, fv128, k128, g64)
+    e1.record()
+    torch.cuda.synchronize()
+    ms64 = e0.elapsed_time(e1)
+    i32, i64 = g32.clone(), g64.clone()
+    del g32, g64, vis128
+    ctx.grid_to_image(i32)
+    ctx.grid_to_image(i64)
+    d = (i32 - i64).abs()
+    err = {"max_rel": float(d.max() / i64.abs().max()),
+           "rms_rel": float(torch.sqrt((d ** 2).mean()) / torch.sqrt((i64.abs() ** 2).mean()))}
+    del d, i64
+    dft = dft_errors(ctx, fv64, N, obs.lam, obs.theta, 16, {"wtowers_fp32_batched": i32}, npix,
+                     inner_only=True)
+    out = {"n_vis": nv, "grid": N, "subgrid": 256, "tasks": st.n_tasks, "towers": st.n_towers,
+           "fp32_batched": {"ms_per_step": ms, "mvis_s": nv / ms / 1e3},
+           "fp64_batched": {"ms_per_step": ms64, "mvis_s": nv / ms64 / 1e3},
+           "error_fp32_vs_fp64": err, "error_vs_dft": dft}
+    log(f"[bench] stress: FP32 batched {ms:.1f} ms ({nv / ms / 1e3:.1f} Mvis/s), FP64 batched "
+        f"{ms64:.1f} ms, {st.n_tasks} tasks, {st.n_towers} towers, err {err}")
+    return out
+
+
 def roofline(kind, head, nv, S, N, n, default_workload, table_bytes):
     """Dominant-kernel and whole-step rooflines under the algorithmic count:
     per visibility 24 (uvw) + 8 (vis) + S^2 x 8 (taps) bytes and 8 S^2 flop;
@@ -405,18 +491,25 @@ def roofline(kind, head, nv, S, N, n, default_workload, table_bytes):
     return roof, job
 
 
-def dft_errors(ctx, fv, N, lam, theta, S, images, npix, seed=4):
-    """Images against the FP64 direct DFT on `npix` seeded pixels, after the
-    layout map (DFT pixel (x, y) is gridder pixel ((N - x) % N, (N - y) % N))
-    and division by the taper A(l) A(m).  Reported for the whole image and the
-    inner field |l|, |m| < theta / 4."""
+def dft_errors(ctx, fv, N, lam, theta, S, images, npix, seed=4, inner_only=False):
+    """Images against the FP64 direct DFT on `npix` seeded pixels (of the whole
+    image, or of the inner field only), after the layout map (DFT pixel (x, y)
+    is gridder pixel ((N - x) % N, (N - y) % N)) and division by the taper
+    A(l) A(m).  Reported for the whole sample and its inner-field part
+    |l|, |m| < theta / 4."""
     import torch
 
     import rgamd
 
     rng = np.random.default_rng(seed)
-    idx = rng.choice(N * N, size=min(npix, N * N), replace=False)
-    ys, xs = np.divmod(idx, N)
+    if inner_only:
+        side = N // 4 - 1  # pixels with |x - N/2| < N/8
+        idx = rng.choice(side * side, size=min(npix, side * side), replace=False)
+        ys, xs = np.divmod(idx, side)
+        ys, xs = ys + N // 2 - N // 8 + 1, xs + N // 2 - N // 8 + 1
+    else:
+        idx = rng.choice(N * N, size=min(npix, N * N), replace=False)
+        ys, xs = np.divmod(idx, N)
     px = torch.from_numpy(xs.astype(np.int32)).cuda()
     py = torch.from_numpy(ys.astype(np.int32)).cuda()
     out = torch.empty(len(idx), dtype=torch.complex128, device="cuda")

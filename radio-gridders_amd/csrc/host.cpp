@@ -126,8 +126,8 @@ KernelTimer::~KernelTimer()
     ctx->timer_events[which].emplace_back(start, stop);
 }
 
-int dft_run(const rg_flat_vis *vis, const int32_t *px, const int32_t *py, int64_t npix, int N,
-            double lambda, void *out, int mode, hipStream_t s);
+int dft_run(rg_context *ctx, const rg_flat_vis *vis, const int32_t *px, const int32_t *py,
+            int64_t npix, int N, double lambda, void *out, int mode, hipStream_t s);
 int predict_run(const double *u, const double *v, const double *w, int64_t n, const double *l,
                 const double *m, const double *flux, int nsrc, void *out, int type, hipStream_t s);
 
@@ -406,7 +406,7 @@ int rg_dft(rg_context *ctx, const rg_flat_vis *vis, const int32_t *px, const int
     if (mode != RG_DFT_FP64 && mode != RG_DFT_MIXED)
         return fail(RG_ERR_PARAM, "unknown DFT mode %d", mode);
     RG_HIP(hipSetDevice(ctx->device));
-    return dft_run(vis, px, py, npix, grid_size, lambda, out, mode, (hipStream_t)stream);
+    return dft_run(ctx, vis, px, py, npix, grid_size, lambda, out, mode, (hipStream_t)stream);
 }
 
 int rg_dft_image(rg_context *ctx, const rg_flat_vis *vis, int32_t grid_size, double lambda,
@@ -424,8 +424,8 @@ int rg_dft_image(rg_context *ctx, const rg_flat_vis *vis, int32_t grid_size, dou
     if (mode != RG_DFT_FP64 && mode != RG_DFT_MIXED)
         return fail(RG_ERR_PARAM, "unknown DFT mode %d", mode);
     RG_HIP(hipSetDevice(ctx->device));
-    return dft_run(vis, nullptr, nullptr, (int64_t)grid_size * grid_size, grid_size, lambda, out,
-                   mode, (hipStream_t)stream);
+    return dft_run(ctx, vis, nullptr, nullptr, (int64_t)grid_size * grid_size, grid_size, lambda,
+                   out, mode, (hipStream_t)stream);
 }
 
 int rg_predict_points(const double *u, const double *v, const double *w, int64_t n,

@@ -195,6 +195,18 @@ def config_c4(seed: int = 1, n_st: int = 256, n_chan: int = 256, n_dumps: int = 
                              "sigma": sigma, "count_seed": count_seed})
 
 
+def config_c5(seed: int = 1, n_st: int = 512, n_chan: int = 16, n_dumps: int = 128,
+              za_deg: float = 60.0, chan_hz: float = 1e6) -> Observation:
+    """Stress workload: n_st stations (radius ~ Exp(3 km)), n_dumps over 4 h with the
+    phase centre transiting at zenith angle za_deg (|w| up to ~5000 wavelengths),
+    bl_max 10 km, n_chan channels chan_hz apart centred on 150 MHz, stream order
+    baseline -> time -> channel, 50 sources, theta 0.2, lambda 20480 (4096^2)."""
+    pos = layout_radial(n_st, 3_000.0, np.random.default_rng(seed))
+    freqs = FREQ_HZ + chan_hz * (np.arange(n_chan) - (n_chan - 1) / 2)
+    return observe("C5", pos, n_dumps, 4.0, dec_for_zenith_angle(za_deg), freqs, 50, 2, 0.2,
+                   20480.0)
+
+
 def config_small(seed: int = 1, n_st: int = 8, n_dumps: int = 16, arm_m: float = 600.0,
                  theta: float = 0.1, lam: float = 640.0, n_src: int = 3) -> Observation:
     """Small Y-layout observation for parity tests (N = theta * lambda)."""

@@ -80,3 +80,29 @@ def test_fp64_required_for_plain(ctx, case):
     with pytest.raises(rgamd.RGError) as e:
         run(ctx, obs, vis, table, rgamd.RG_MODE_PLAIN, dt=torch.complex64)
     assert e.value.code == rgamd.RG_ERR_PARAM
+
+
+def test_stress_shape_matches_oracle(ctx):
+    """The stress-workload shape at reduced size: 4096^2 grid (theta 0.2),
+    subgrid 256, margin 16, winc 10, phase centre at zenith angle 60 deg
+    (|w| up to thousands of wavelengths, hundreds of floors), three channels."""
+    obs = synth.config_c5(n_st=20, n_chan=3, n_dumps=24)
+    vis = synth.predict_numpy(obs.u, obs.v, obs.w, obs.l, obs.m, obs.flux)
+    sub, margin, winc = 256, 16, 10.0
+    table = rgamd.make_wkernels(11, 8, 16, obs.theta, -5.0, 1.0)
+    ref, fl, bad = orc.grid_wtowers(obs.grid_size, obs.theta, obs.u, obs.v, obs.w, vis, table,
+                                    -5.0, 1.0, sub, margin, winc, nthreads=8)
+    assert bad == 0 and obs.grid_size == 4096
+    N = obs.grid_size
+    for dt, tol in ((torch.complex128, 1e-11), (torch.complex64, 1e-5)):
+        _, fv = stage(obs.u, obs.v, obs.w, vis, dt)
+        kt, wk = kern(table, -5.0, 1.0, dt)
+        g = torch.empty((N, N), dtype=torch.complex128, device="cuda")
+        ctx.grid(rgamd.params(N, obs.theta, mode=rgamd.RG_MODE_BATCHED, subgrid=sub,
+                              margin=margin, winc=winc), fv, wk, g)
+        st = ctx.stats()
+        torch.cuda.synchronize()
+        got = g.cpu().numpy()
+        assert rel_max(got, ref) < tol, dt
+        assert st.n_gridded == obs.n_vis and st.n_tasks > 50
+        del g
