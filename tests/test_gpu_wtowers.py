@@ -106,3 +106,24 @@ def test_stress_shape_matches_oracle(ctx):
         assert rel_max(got, ref) < tol, dt
         assert st.n_gridded == obs.n_vis and st.n_tasks > 50
         del g
+
+
+@pytest.mark.parametrize("sub,margin,S", [(128, 16, 8), (128, 8, 8), (256, 16, 16)])
+def test_other_shapes_match_oracle(ctx, sub, margin, S):
+    """Kernel support 8 and subgrids 128 / 256 through the batched path."""
+    obs, vis = small_obs(n_st=14, n_dumps=24, arm_m=900.0, lam=5120.0)  # N = 512
+    table = rgamd.make_wkernels(11, 4, S, obs.theta, -5.0, 1.0)
+    ref, _, bad = orc.grid_wtowers(obs.grid_size, obs.theta, obs.u, obs.v, obs.w, vis, table,
+                                   -5.0, 1.0, sub, margin, WINC, nthreads=8)
+    assert bad == 0
+    N = obs.grid_size
+    for dt, tol in ((torch.complex128, 1e-11), (torch.complex64, 1e-5)):
+        _, fv = stage(obs.u, obs.v, obs.w, vis, dt)
+        kt, wk = kern(table, -5.0, 1.0, dt)
+        g = torch.empty((N, N), dtype=torch.complex128, device="cuda")
+        ctx.grid(rgamd.params(N, obs.theta, mode=rgamd.RG_MODE_BATCHED, subgrid=sub,
+                              margin=margin, winc=WINC, vis_per_item=97), fv, wk, g)
+        st = ctx.stats()
+        torch.cuda.synchronize()
+        assert rel_max(g.cpu().numpy(), ref) < tol, (dt, sub, S)
+        assert st.n_gridded == obs.n_vis
