@@ -1,16 +1,15 @@
 // binning.hpp -- device-side counting sort of visibilities into work items.
 //
-// A "keyer" maps visibility i to (key, nkey, packed tile position,
-// kernel-tap index): the visibility produces one record under each of the
-// keys key .. key + nkey - 1 (W-projection: one tile key; W-towers: every
-// subgrid band its footprint reaches).  Keys are dense in [0, nkeys).  The
-// sort is:
-//   count   : wave-aggregated atomics on count[rep][key]
+// A "keyer" maps visibility i to (key, packed tile position, kernel-tap
+// index).  Keys are dense in [0, nkeys).  The sort is:
+//   count   : wave-aggregated atomics on count[rep][key]; each visibility
+//             keeps the counter value it got (its rank inside the replica's
+//             share of the key) with its key, position and tap index
 //   scan    : three-phase exclusive scan of (count, ceil(count / K)), where
-//             count[key] is the sum over the replicas
+//             count[key] is the sum over the replicas, then the start of
+//             every replica's share of every key
 //   items   : one work item per K records of a key
-//   scatter : wave-aggregated atomics on cursor[rep][key], record written in
-//             place
+//   scatter : record i written at start[rep][key] + rank (no atomics)
 // Block b uses counter replica rep = b mod NREP.  Device-scope atomics on one
 // address serialise, and a few keys (the short-baseline tiles at the uv
 // centre) take thousands of wavefront updates, so the counters of a key are
@@ -83,20 +82,25 @@ struct Replicas {
     int64_t stride;
 };
 
+// Per visibility after the count pass: key (KEY_NONE if not gridded), rank,
+// packed position, tap index.
+constexpr uint32_t KEY_NONE = 0xffffffffu;
+
 template <typename Keyer>
 __global__ __launch_bounds__(256) void k_bin_count(Keyer kf, int64_t n, uint32_t *count,
-                                                   Replicas rp, unsigned long long *stats)
+                                                   Replicas rp, uint4 *krp,
+                                                   unsigned long long *stats)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t key = 0, pos = 0, kidx = 0;
-    int nkey = 0;
     int st = VIS_DROPPED;
     if (i < n)
-        st = kf.classify(i, key, nkey, pos, kidx);
+        st = kf.classify(i, key, pos, kidx);
     const bool ok = i < n && st == VIS_OK;
-    uint32_t *cnt = count + (blockIdx.x & rp.mask) * rp.stride;
-    for (int q = 0; q < kf.max_keys(); ++q)
-        wave_agg_add(cnt, key + (uint32_t)q, ok && q < nkey, false);
+    const uint32_t rank =
+        wave_agg_add(count + (blockIdx.x & rp.mask) * rp.stride, key, ok, true);
+    if (i < n)
+        krp[i] = make_uint4(ok ? key : KEY_NONE, rank, pos, kidx);
     // per-wave status counts
     const unsigned long long dropped = __ballot(i < n && st == VIS_DROPPED);
     const unsigned long long badp = __ballot(i < n && st == VIS_BAD_PLANE);
@@ -108,29 +112,23 @@ __global__ __launch_bounds__(256) void k_bin_count(Keyer kf, int64_t n, uint32_t
     }
 }
 
+// Same grid as k_bin_count (block b, replica b mod NREP).
 template <typename Keyer, typename T>
-__global__ __launch_bounds__(256) void k_bin_scatter(Keyer kf, int64_t n, uint32_t *cursor,
-                                                     Replicas rp, Rec<T> *recs)
+__global__ __launch_bounds__(256) void k_bin_scatter(Keyer kf, int64_t n, const uint32_t *start,
+                                                     Replicas rp, const uint4 *__restrict__ krp,
+                                                     Rec<T> *recs)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t key = 0, pos = 0, kidx = 0;
-    int nkey = 0;
-    int st = VIS_DROPPED;
-    if (i < n)
-        st = kf.classify(i, key, nkey, pos, kidx);
-    const bool ok = i < n && st == VIS_OK;
+    if (i >= n)
+        return;
+    const uint4 e = krp[i];
+    if (e.x == KEY_NONE)
+        return;
     Rec<T> r;
-    r.pos = pos;
-    r.kidx = kidx;
-    if (ok)
-        r.vis = kf.template vis_value<T>(i);
-    uint32_t *cur = cursor + (blockIdx.x & rp.mask) * rp.stride;
-    for (int q = 0; q < kf.max_keys(); ++q) {
-        const bool act = ok && q < nkey;
-        const uint32_t dst = wave_agg_add(cur, key + (uint32_t)q, act, true);
-        if (act)
-            recs[dst] = r;
-    }
+    r.pos = e.z;
+    r.kidx = e.w;
+    r.vis = kf.template vis_value<T>(i);
+    recs[start[(blockIdx.x & rp.mask) * rp.stride + e.x] + e.y] = r;
 }
 
 // Phase 1: per-block totals of (count, items).

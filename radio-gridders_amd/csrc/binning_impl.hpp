@@ -10,8 +10,8 @@ template <typename Keyer, typename T>
 int bin_visibilities(rg_context *ctx, const Keyer &kf, int64_t n, int64_t nkeys, uint32_t per_item,
                      hipStream_t s, BinResult &out)
 {
-    // a visibility writes up to max_keys records; record indices are 32-bit
-    const int64_t nrec = n * (int64_t)kf.max_keys();
+    // record indices are 32-bit
+    const int64_t nrec = n;
     if (nrec >= (int64_t)0x7fffffff)
         return fail(RG_ERR_PARAM, "visibility stream too long for one call (%lld)", (long long)n);
     if (nkeys <= 0 || nkeys >= (int64_t)0x7fffffff)
@@ -36,6 +36,7 @@ int bin_visibilities(rg_context *ctx, const Keyer &kf, int64_t n, int64_t nkeys,
     RG_TRY(ctx->scratch[4].ensure(sizeof(Item) * max_items));              // items
     RG_TRY(ctx->scratch[5].ensure(sizeof(Rec<T>) * (nrec > 0 ? nrec : 1))); // records
     RG_TRY(ctx->scratch[6].ensure(sizeof(uint2) * (nblocks_scan + 1)));   // block sums
+    RG_TRY(ctx->scratch[15].ensure(sizeof(uint4) * (n > 0 ? n : 1)));     // key, rank, pos, kidx
 
     uint32_t *count = ctx->scratch[0].as<uint32_t>();
     KernelTimer kt(ctx, RG_TIMER_BINNING, s);
@@ -46,7 +47,7 @@ int bin_visibilities(rg_context *ctx, const Keyer &kf, int64_t n, int64_t nkeys,
     const int64_t nb = (n + tb - 1) / tb;
     if (nb > 0)
         hipLaunchKernelGGL((k_bin_count<Keyer>), dim3((unsigned)nb), dim3(tb), 0, s, kf, n, count,
-                           rp, stats);
+                           rp, ctx->scratch[15].as<uint4>(), stats);
     RG_HIP(hipGetLastError());
     uint2 *bsums = ctx->scratch[6].as<uint2>();
     hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nblocks_scan), dim3(SCAN_BLOCK), 0, s,
@@ -59,7 +60,9 @@ int bin_visibilities(rg_context *ctx, const Keyer &kf, int64_t n, int64_t nkeys,
     RG_HIP(hipGetLastError());
     if (nb > 0)
         hipLaunchKernelGGL((k_bin_scatter<Keyer, T>), dim3((unsigned)nb), dim3(tb), 0, s, kf, n,
-                           ctx->scratch[3].as<uint32_t>(), rp, ctx->scratch[5].as<Rec<T>>());
+                           (const uint32_t *)ctx->scratch[3].as<uint32_t>(), rp,
+                           (const uint4 *)ctx->scratch[15].as<uint4>(),
+                           ctx->scratch[5].as<Rec<T>>());
     RG_HIP(hipGetLastError());
 
     out.vis_start = ctx->scratch[1].as<uint32_t>();

@@ -157,13 +157,9 @@ struct WprojKeyer {
         return lo;
     }
 
-    __host__ __device__ int max_keys() const { return 1; }
-
-    __device__ int classify(int64_t i, uint32_t &key, int &nkey, uint32_t &pos,
-                            uint32_t &kidx) const
+    __device__ int classify(int64_t i, uint32_t &key, uint32_t &pos, uint32_t &kidx) const
     {
         int gx, gy, xf, yf;
-        nkey = 1;
         frac_coord(N, S, R, theta, u[i], v[i], 0.0, 0.0, gx, gy, xf, yf);
         if (gx <= -S || gx >= N || gy <= -S || gy >= N)
             return VIS_DROPPED;

@@ -46,10 +46,7 @@ struct TowerKeyer {
     // pos is always the raw subgrid corner + S: (gx + S) | (gy + S) << 16.
     int band_h, nk, mk;
 
-    __host__ __device__ int max_keys() const { return 1; }
-
-    __device__ int classify(int64_t i, uint32_t &key, int &nkey, uint32_t &pos,
-                            uint32_t &kidx) const
+    __device__ int classify(int64_t i, uint32_t &key, uint32_t &pos, uint32_t &kidx) const
     {
         const double ui = u[i], vi = v[i], wi = w[i];
         const int cx = tower_index(ui, theta, N, chunk);
@@ -73,7 +70,6 @@ struct TowerKeyer {
             return VIS_BAD_PLANE;
         const uint32_t task = (uint32_t)((cy * cc + cx) * nwp + (wp - wp_min));
         key = band_h > 0 ? task * (uint32_t)nk + (uint32_t)(max(gy, 0) / band_h) : task;
-        nkey = 1;
         pos = (uint32_t)(gx + S) | ((uint32_t)(gy + S) << 16);
         kidx = (uint32_t)((p * R + yf) * R + xf);
         return VIS_OK;
