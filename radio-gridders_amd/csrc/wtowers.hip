@@ -486,7 +486,7 @@ __global__ __launch_bounds__(256) void k_band_combine(const Item *__restrict__ i
 }
 
 // ---- batched column pass -------------------------------------------------------------------
-// Block (cb, c): column block cb of chunk c (up to a few consecutive tasks of
+// Block (cb, c): column block cb of chunk c (up to 32 consecutive tasks of
 // one tower).  For every task: load the column block of its subgrid (bands
 // without records are zero and are not read), inverse FFT in LDS, multiply by
 // exp(-2 pi i wp winc base), accumulate in registers; write the chunk's sum
@@ -847,9 +847,11 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
     int *d_range = ctx->scratch[8].as<int>();
     hipLaunchKernelGGL(k_range_init, dim3(1), dim3(1), 0, s, d_range);
     {
+        // few workgroups: each ends with one atomicMin / atomicMax on the
+        // same two words, which serialise
         int64_t blocks = (vis->n + 255) / 256;
-        if (blocks > 4096)
-            blocks = 4096;
+        if (blocks > 1024)
+            blocks = 1024;
         hipLaunchKernelGGL(k_floor_range, dim3((unsigned)blocks), dim3(256), 0, s, vis->u, vis->v,
                            vis->w, vis->n, p->theta, N, chunk, cc, p->winc, d_range);
         RG_HIP(hipGetLastError());
@@ -957,7 +959,7 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
     constexpr size_t BATCH_BYTES = (size_t)8 << 30;
     const int q_fit = (int)std::max<size_t>(1, BATCH_BYTES / (sizeof(T2) * nn));
     const int Q = plain ? 1 : (p->floor_batch > 0 ? p->floor_batch : std::min(ntasks, q_fit));
-    const int TPB = 8;
+    const int TPB = 32;
     std::vector<int32_t> chunk_first, chunk_count, batch_chunk0, run_tower, run_chunk0, run_nchunk,
         batch_run0;
     for (int t0 = 0; t0 < ntasks; t0 += Q) {
