@@ -179,6 +179,93 @@ __device__ inline void fft_stage(T2 *lds, const T2 *tw, int logns)
     __syncthreads();
 }
 
+// ---- block transform with register input and output --------------------------------
+// Same stage sequence and thread mapping as fft_block, but the first stage
+// takes its inputs from load(slot, idx) (no LDS store of the input, no barrier
+// before it) and the last stage hands its outputs to emit(slot, idx, value)
+// instead of storing them.  slot = q * R + r numbers a thread's values in a
+// stage (compile-time after unrolling), idx is the position in the line
+// (line = threadIdx.x % B).  The caller synchronises before the next use of
+// lds (the last stage reads it).  Needs at least two stages (n >= 16).
+template <int LOGN> __host__ __device__ constexpr int last_radix()
+{
+    return (LOGN % 3) == 0 ? 8 : ((LOGN % 3) == 2 ? 4 : 2);
+}
+
+// position in the line of the last stage's output `slot` for this thread
+template <int LOGN, int B> __device__ inline int fft_out_index(int slot)
+{
+    constexpr int n = 1 << LOGN;
+    constexpr int R = last_radix<LOGN>();
+    const int lt = threadIdx.x / B;
+    return lt + (slot / R) * (n / 8) + (slot % R) * (n / R);
+}
+
+template <typename T2, int LOGN, int B, int LS, int R, bool INV, int MODE, typename Load,
+          typename Emit>
+__device__ inline void fft_stage_io(T2 *lds, const T2 *tw, int logns, Load &load, Emit &emit)
+{
+    // MODE 1: inputs from load, outputs to lds.  MODE 2: inputs from lds,
+    // outputs to emit.
+    constexpr int n = 1 << LOGN;
+    constexpr int TPL = n / 8;
+    constexpr int NBUT = n / R;
+    constexpr int PER = NBUT / TPL;
+    const int line = threadIdx.x % B;
+    const int lt = threadIdx.x / B;
+    const int Ns = 1 << logns;
+    T2 v[8];
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int j = lt + q * TPL;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            v[q * R + r] = MODE == 1 ? load(q * R + r, j + r * NBUT) : lds[(j + r * NBUT) * LS + line];
+    }
+#pragma unroll
+    for (int q = 0; q < PER; ++q) {
+        const int j = lt + q * TPL;
+        const int k = j & (Ns - 1);
+        const int tstep = k * (n >> (logns + (R == 8 ? 3 : (R == 4 ? 2 : 1))));
+        if (logns > 0) {
+#pragma unroll
+            for (int r = 1; r < R; ++r)
+                v[q * R + r] = cmul(v[q * R + r], tw[r * tstep]);
+        }
+        if (R == 8)
+            dft8<INV>(&v[q * R]);
+        else if (R == 4)
+            dft4<INV>(v[q * R], v[q * R + 1], v[q * R + 2], v[q * R + 3]);
+        else
+            dft2<INV>(v[q * R], v[q * R + 1]);
+        const int d = (j >> logns) * (Ns * R) + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (MODE == 1)
+                lds[(d + r * Ns) * LS + line] = v[q * R + r];
+            else
+                emit(q * R + r, d + r * Ns, v[q * R + r]);
+        }
+    }
+    if (MODE == 1)
+        __syncthreads();
+}
+
+template <typename T2, int LOGN, int B, int LS, bool INV, typename Load, typename Emit>
+__device__ inline void fft_block_io(T2 *lds, const T2 *tw, Load &&load, Emit &&emit)
+{
+    static_assert(LOGN >= 4, "two or more stages");
+    constexpr int NST = LOGN / 3 + (LOGN % 3 ? 1 : 0);
+    fft_stage_io<T2, LOGN, B, LS, 8, INV, 1>(lds, tw, 0, load, emit);
+    int logns = 3;
+#pragma unroll
+    for (int st = 1; st < NST - 1; ++st) {
+        fft_stage<T2, LOGN, B, LS, 8, INV>(lds, tw, logns);
+        logns += 3;
+    }
+    fft_stage_io<T2, LOGN, B, LS, last_radix<LOGN>(), INV, 2>(lds, tw, logns, load, emit);
+}
+
 // ---- line-major variant ---------------------------------------------------------
 // Lines stored contiguously (line l at base + l * LSTR); thread t of a round
 // works on line t / TPL, position t % TPL.  `active` threads do the work, all

@@ -181,34 +181,85 @@ __device__ inline size_t gcb_index(int row, int col, int n)
     return (size_t)(col / GCB) * n * GCB + (size_t)row * GCB + (col % GCB);
 }
 
+// ---- band records ------------------------------------------------------------------------
+// A visibility as band b of its task sees it, with everything a band lane
+// needs precomputed (z = sizeof(T2), the element size):
+//   xy = gz | cy0 << 16, gz = (gx + S) z the footprint's first column in the
+//        band's row layout (S elements of left pad), cy0 = H b - gy (signed)
+//        the footprint row that band row 0 meets;
+//   kb = (kidx S^2 + cy0 S) z, byte offset of that footprint row's first tap
+//        in the kernel table (negative values are never dereferenced: a lane
+//        whose row the footprint misses reads no tap).
+template <typename T> struct BandRec {
+    uint32_t xy;
+    int32_t kb;
+    typename Vec2<T>::type vis;
+};
+static_assert(sizeof(BandRec<float>) == sizeof(Rec<float>) &&
+                  sizeof(BandRec<double>) == sizeof(Rec<double>),
+              "band records replace binned records one for one");
+
+// xy of a padding record: cy0 out of reach of every band row
+constexpr uint32_t BAND_XY_NONE = 0x40000000u;
+
+template <typename T>
+__device__ inline BandRec<T> band_record(const Rec<T> &r, int S, int Hb, int z)
+{
+    const int gx = (int)(r.pos & 0xffffu) - S, gy = (int)(r.pos >> 16) - S;
+    const int cy0 = Hb - gy;
+    BandRec<T> o;
+    o.xy = (uint32_t)((gx + S) * z) | ((uint32_t)cy0 << 16);
+    o.kb = ((int32_t)r.kidx * S * S + cy0 * S) * z;
+    o.vis = r.vis;
+    return o;
+}
+
+// Kernel taps through a buffer resource: a lane whose offset lies beyond the
+// table (the lanes a footprint misses get 0x80000000) reads zeros.
+__device__ inline float2 tap_load(__amdgpu_buffer_rsrc_t r, uint32_t off, float2 *)
+{
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ inline double2 tap_load(__amdgpu_buffer_rsrc_t r, uint32_t off, double2 *)
+{
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+constexpr uint32_t TAP_NONE = 0x80000000u;
+// word 3 of a raw (unswizzled, stride 0) buffer resource on gfx9
+constexpr int BUF_RSRC_W3 = 0x00020000;
+
 // ---- record order inside band items -------------------------------------------------------
 // A band lane's cell depends only on the footprint column gx (its row is fixed),
 // so records sorted by gx keep each lane on one cell for long runs and the
 // Romein register sum rarely flushes.  Every band work item (window records
 // src = wstart[key] + (start - prefix[key]) .. + count of the binned array) is
 // counting-sorted by gx + S (bins < n + S <= SORT_BINS) into its own slot
-// [start, start + count) of the expanded array; the order inside a bin
-// follows LDS atomics.  Workgroups loop over the items.
+// [start, start + count) of the expanded array as band records of its band
+// b = key mod nb; the order inside a bin follows LDS atomics.  Workgroups
+// loop over the items.
 constexpr int SORT_BINS = 1024;
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_sort_items(const Rec<T> *__restrict__ in,
-                                                    Rec<T> *__restrict__ out,
+                                                    BandRec<T> *__restrict__ out,
                                                     const Item *__restrict__ items,
                                                     const unsigned long long *__restrict__ stats,
                                                     const uint32_t *__restrict__ prefix,
-                                                    const uint32_t *__restrict__ wstart, int nbins)
+                                                    const uint32_t *__restrict__ wstart, int nbins,
+                                                    int S, int H, int nb)
 {
     __shared__ uint32_t bins[SORT_BINS];
     __shared__ uint32_t wsum[4];
+    constexpr int z = (int)sizeof(typename Vec2<T>::type);
     const uint32_t nitems = (uint32_t)stats[CNT_ITEMS];
     for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
         const Item item = items[it];
         const Rec<T> *src = in + (item.start - prefix[item.key] + wstart[item.key]);
-        Rec<T> *dst = out + item.start;
+        BandRec<T> *dst = out + item.start;
+        const int Hb = H * (int)(item.key % (uint32_t)nb);
         if (item.count <= 32) {
             if (threadIdx.x < item.count)
-                dst[threadIdx.x] = src[threadIdx.x];
+                dst[threadIdx.x] = band_record(src[threadIdx.x], S, Hb, z);
             continue;
         }
         __syncthreads(); // bins of the previous item are no longer read
@@ -249,7 +300,7 @@ __global__ __launch_bounds__(256) void k_sort_items(const Rec<T> *__restrict__ i
         for (uint32_t i = threadIdx.x; i < item.count; i += 256) {
             const Rec<T> r = src[i];
             const uint32_t d = atomicAdd(&bins[min(r.pos & 0xffffu, (uint32_t)nbins - 1)], 1u);
-            dst[d] = r;
+            dst[d] = band_record(r, S, Hb, z);
         }
     }
 }
@@ -273,43 +324,48 @@ __global__ __launch_bounds__(256) void k_window_len(const uint32_t *__restrict__
 }
 
 // ---- batched band pass -------------------------------------------------------------------
-// One workgroup per work item: up to K records of the window of band b of a
-// task -- records of keys b - mk + 1 .. b, sorted by footprint column --
-// accumulated into subgrid rows [H b, H b + H).  The
-// workgroup is GRP groups of S x H lanes; the groups take alternate runs of the
-// item's records.  Lane (myU, myV) owns band row myV and the columns congruent
-// to myU modulo S: a footprint covering that row has exactly one cell there.
-// The lane keeps the register sum of its current cell and adds it into the
-// LDS band when the cell changes (LDS float atomics when GRP > 1, since lanes
-// of different groups own the same cells).  The band's rows are then inverse
-// FFT'd in LDS.  A key of one item stores the band straight into the task's
-// subgrid; a key split over several items stores each part in a spill slot
-// and k_band_combine sums the parts in item order.
+// One workgroup per work item: up to K band records of the window of band b of
+// a task -- records of keys b - mk + 1 .. b, sorted by footprint column --
+// accumulated into subgrid rows [H b, H b + H).  The workgroup is GRP groups
+// of S x H lanes; the groups take alternate runs of the item's records.  Lane
+// (myU, myV) owns band row myV and the columns congruent to myU modulo S: a
+// footprint covering that row has exactly one cell there.  The lane keeps the
+// register sum of its current cell and adds it into the LDS band when the
+// cell changes (LDS float atomics when GRP > 1, since lanes of different
+// groups own the same cells).  Band rows lie BS = n + S + 8 elements apart
+// after a lead of S elements, so a footprint column outside [0, n) lands in
+// the gap between rows and is dropped without a test.  The band's rows are
+// then inverse FFT'd in LDS.  A key of one item stores the band straight into
+// the task's subgrid; a key split over several items stores each part in a
+// spill slot and k_band_combine sums the parts in item order.
 template <typename T, int S, int H, int LOGN, int GRP>
 __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
-    const Rec<T> *__restrict__ recs, const Item *__restrict__ items,
+    const BandRec<T> *__restrict__ recs, const Item *__restrict__ items,
     const uint32_t *__restrict__ item_start, uint32_t key_begin, uint32_t key_end,
     const int32_t *__restrict__ rank_of_slot, int t0,
-    const typename Vec2<T>::type *__restrict__ kern, const typename Vec2<T>::type *__restrict__ twg,
-    typename Vec2<T>::type *G, typename Vec2<T>::type *spill, uint32_t spill_cap,
-    uint32_t *spill_ctr, uint32_t *item_spill)
+    const typename Vec2<T>::type *__restrict__ kern, uint32_t kern_bytes,
+    const typename Vec2<T>::type *__restrict__ twg, typename Vec2<T>::type *G,
+    typename Vec2<T>::type *spill, uint32_t spill_cap, uint32_t *spill_ctr, uint32_t *item_spill)
 {
     using T2 = typename Vec2<T>::type;
     constexpr int n = 1 << LOGN;
     constexpr int NL = S * H;
     constexpr int NT = NL * GRP;
     constexpr int NB = n / H;
+    constexpr int z = (int)sizeof(T2);
     // records staged RS at a time (RS / GRP per group): 128 keeps the FP32
     // 8-row band at four workgroups per CU
     constexpr int RS = 128;
     constexpr int RG = RS / GRP;
     static_assert(RS % GRP == 0 && RS <= NT, "record staging");
-    // band rows padded to BS = n + 8 elements: the column-block copy-out
-    // reads 8 rows at once without all of them hitting the same LDS banks
-    constexpr int BS = n + 8;
-    __shared__ T2 band[H * BS];
+    // row l of the band at band[S + l BS .. S + l BS + n); the S - 1 columns
+    // either side of a row fall in the gaps.  The extra 8 elements spread the
+    // column-block copy-out (8 rows at once) over the LDS banks.
+    constexpr int BS = n + S + 8;
+    constexpr int BANDE = S + H * BS;
+    __shared__ T2 band[BANDE];
     __shared__ T2 tw[n];
-    __shared__ Rec<T> s_rec[RS];
+    __shared__ BandRec<T> s_rec[RS];
     __shared__ uint32_t s_slot;
     const uint32_t ib = item_start[key_begin], ie = item_start[key_end];
     const uint32_t it = ib + blockIdx.x;
@@ -320,67 +376,79 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
     const int b = (int)(item.key - tslot * (uint32_t)NB);
     const int j = rank_of_slot[tslot] - t0;
 
-    for (int e = threadIdx.x; e < H * BS; e += NT) {
+    // thread t < RS stages record t of each RS-record stage; the next
+    // stage's record is loaded while the current stage is processed
+    auto stage_rec = [&](uint32_t base) {
+        BandRec<T> r;
+        if (base + threadIdx.x < item.count) {
+            r = recs[item.start + base + threadIdx.x];
+        } else {
+            r.xy = BAND_XY_NONE;
+            r.kb = 0;
+            r.vis.x = T(0);
+            r.vis.y = T(0);
+        }
+        return r;
+    };
+    BandRec<T> next;
+    if (threadIdx.x < RS)
+        next = stage_rec(0);
+    for (int e = threadIdx.x; e < BANDE; e += NT) {
         band[e].x = T(0);
         band[e].y = T(0);
     }
     for (int e = threadIdx.x; e < n; e += NT)
         tw[e] = twg[e];
+    const __amdgpu_buffer_rsrc_t ktab =
+        __builtin_amdgcn_make_buffer_rsrc((void *)kern, 0, (int)kern_bytes, BUF_RSRC_W3);
     const int lane = threadIdx.x % NL, grp = threadIdx.x / NL;
     const int myU = lane % S, myV = lane / S;
-    const int row = H * b + myV;
-    T *band_f = reinterpret_cast<T *>(band);
+    // per lane, in bytes: column residue, tap row, cell row
+    const int uz = myU * z, tap_row = myV * S * z, cell_row = myV * BS * z;
+    char *band_b = reinterpret_cast<char *>(band);
     int cur = -1, carry = -1;
     T sr = T(0), si = T(0);
     auto flush = [&]() {
+        T *c = reinterpret_cast<T *>(band_b + cur);
         if (GRP > 1) {
-            atomicAdd(&band_f[2 * cur], sr);
-            atomicAdd(&band_f[2 * cur + 1], si);
+            atomicAdd(c, sr);
+            atomicAdd(c + 1, si);
         } else {
-            band[cur].x += sr;
-            band[cur].y += si;
+            c[0] += sr;
+            c[1] += si;
         }
     };
     // U records at a time: U tap loads in flight, then the U updates in order.
-    // Padding records lie outside every band.
     constexpr int U = 8;
     static_assert(RG % U == 0, "record batch must be a multiple of the unroll");
     for (uint32_t base = 0; base < item.count; base += RS) {
         const uint32_t nloc = min((uint32_t)RS, item.count - base);
         __syncthreads();
-        if (threadIdx.x < nloc) {
-            s_rec[threadIdx.x] = recs[item.start + base + threadIdx.x];
-        } else if (threadIdx.x < RS) {
-            Rec<T> z;
-            z.pos = 0xffffffffu;
-            z.kidx = 0;
-            z.vis.x = T(0);
-            z.vis.y = T(0);
-            s_rec[threadIdx.x] = z;
+        if (threadIdx.x < RS) {
+            s_rec[threadIdx.x] = next;
+            if (base + RS < item.count)
+                next = stage_rec(base + RS);
         }
         __syncthreads();
         const int ng = min(RG, max(0, (int)nloc - grp * RG));
-        const Rec<T> *gr = s_rec + grp * RG;
+        const BandRec<T> *gr = s_rec + grp * RG;
         using Bt = RecBatch<T, U>;
-        // A lane whose row the footprint misses (or whose column is off the
-        // subgrid) takes a zero tap and keeps its previous cell, so it adds
-        // nothing and never flushes: the update needs no second branch.
+        // A lane whose row the footprint misses takes a zero tap and keeps its
+        // previous cell, so it adds nothing and never flushes: the update
+        // needs no second branch.
         auto fetch = [&](uint32_t q0, Bt &bt) {
 #pragma unroll
             for (int q = 0; q < U; ++q) {
-                const Rec<T> r = gr[q0 + q];
-                const int gx = (int)(r.pos & 0xffffu) - S, gy = (int)(r.pos >> 16) - S;
-                const int cy = row - gy;
-                const int cx = (myU - gx) & (S - 1);
-                const int col = gx + cx;
-                const bool in = (unsigned)cy < (unsigned)S && (unsigned)col < (unsigned)n;
-                carry = in ? myV * BS + col : carry;
+                const BandRec<T> r = gr[q0 + q];
+                const int gz = (int)(r.xy & 0xffffu);
+                const int cy = ((int)r.xy >> 16) + myV;
+                const int cxz = (uz - gz) & ((S - 1) * z);
+                const bool in = (unsigned)cy < (unsigned)S;
+                carry = in ? cell_row + gz + cxz : carry;
                 bt.cell[q] = carry;
                 bt.v[q] = r.vis;
-                T2 k = kern[in ? r.kidx * (uint32_t)(S * S) + (uint32_t)(cy * S + cx) : 0u];
-                k.x = in ? k.x : T(0);
-                k.y = in ? k.y : T(0);
-                bt.k[q] = k;
+                bt.k[q] = tap_load(ktab, in ? (uint32_t)(r.kb + tap_row + cxz) : TAP_NONE,
+                                   (T2 *)nullptr);
             }
         };
         auto update = [&](const Bt &bt) {
@@ -426,19 +494,20 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
         dst = spill + (size_t)s_slot * H * n;
     }
     __syncthreads();
-    fft_lines_lm<T2, LOGN, NT, BS, true>(band, tw, H);
+    T2 *rows = band + S;
+    fft_lines_lm<T2, LOGN, NT, BS, true>(rows, tw, H);
     if (whole) {
         // rows [H b, H b + H) of column block cb are H * GCB contiguous elements
         T2 *d0 = dst + (size_t)H * b * GCB;
         for (int e = threadIdx.x; e < H * n; e += NT) {
             const int cb = e / (H * GCB), w = e - cb * (H * GCB);
             const int l = w / GCB, c = cb * GCB + (w - l * GCB);
-            d0[(size_t)cb * n * GCB + w] = band[l * BS + c];
+            d0[(size_t)cb * n * GCB + w] = rows[l * BS + c];
         }
     } else {
         for (int e = threadIdx.x; e < H * n; e += NT) {
             const int l = e / n, c = e - l * n;
-            dst[e] = band[l * BS + c];
+            dst[e] = rows[l * BS + c];
         }
     }
 }
@@ -487,10 +556,11 @@ __global__ __launch_bounds__(256) void k_band_combine(const Item *__restrict__ i
 
 // ---- batched column pass -------------------------------------------------------------------
 // Block (cb, c): column block cb of chunk c (up to 32 consecutive tasks of
-// one tower).  For every task: load the column block of its subgrid (bands
-// without records are zero and are not read), inverse FFT in LDS, multiply by
-// exp(-2 pi i wp winc base), accumulate in registers; write the chunk's sum
-// to its partial image.
+// one tower).  For every task the column block of its subgrid is inverse
+// FFT'd with the first stage reading straight from HBM (bands without records
+// are zero and are not read) and the last stage's outputs, which stay with
+// their thread, multiplied by exp(-2 pi i wp winc base) and accumulated in
+// registers; the chunk's sum is written to its partial image.
 template <typename T, int LOGN, int CB, int H>
 __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
     const typename Vec2<T>::type *__restrict__ G, const uint32_t *__restrict__ key_start,
@@ -511,55 +581,45 @@ __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
         tw[k] = twg[k];
     const int c = chunk_base + blockIdx.y;
     const int first = chunk_first[c], count = chunk_count[c];
-    const int c0 = blockIdx.x * CB;
+    const int line = threadIdx.x % CB;
+    const int col = blockIdx.x * CB + line;
 
+    // the thread's output pixels (row fft_out_index(slot), col)
     double base[PER];
     T2 acc[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-        const int e = threadIdx.x + q * NT;
-        const int row = e / CB, col = c0 + (e - (e / CB) * CB);
-        base[q] = screen_base(col, row, n, theta);
+        base[q] = screen_base(col, fft_out_index<LOGN, CB>(q), n, theta);
         acc[q].x = T(0);
         acc[q].y = T(0);
     }
     static_assert(CB == GCB || (1 << LOGN) < GCB, "column pass reads whole column blocks");
     for (int k = first; k < first + count; ++k) {
-        const T2 *g = G + (size_t)(k - t0) * n * n + (size_t)blockIdx.x * n * CB;
+        const T2 *g = G + (size_t)(k - t0) * n * n + (size_t)blockIdx.x * n * CB + line;
         // band b of the task was written iff it has work items
         const uint32_t *f = key_start + (size_t)task_slot[k] * NB;
-        __syncthreads();
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int e = threadIdx.x + q * NT;
-            const int row = e / CB, col = e - row * CB;
+        const double ph0 = -(double)task_wp[k] * winc;
+        auto load = [&](int, int row) {
             T2 val;
             val.x = T(0);
             val.y = T(0);
             if (f[row / H + 1] > f[row / H])
-                val = g[(size_t)row * CB + col];
-            lds[e] = val;
-        }
-        __syncthreads();
-        fft_block<T2, LOGN, CB, CB, true>(lds, tw);
-        const double ph0 = -(double)task_wp[k] * winc;
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int e = threadIdx.x + q * NT;
+                val = g[(size_t)row * CB];
+            return val;
+        };
+        auto emit = [&](int q, int, T2 val) {
             T cs, sn;
             Trig<T>::rot(ph0 * base[q], cs, sn);
-            const T2 val = lds[e];
             acc[q].x += val.x * cs - val.y * sn;
             acc[q].y += val.x * sn + val.y * cs;
-        }
+        };
+        __syncthreads(); // the previous task's last stage has read lds
+        fft_block_io<T2, LOGN, CB, CB, true>(lds, tw, load, emit);
     }
-    T2 *pp = P + (size_t)blockIdx.y * n * n + c0;
+    T2 *pp = P + (size_t)blockIdx.y * n * n + col;
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int e = threadIdx.x + q * NT;
-        const int row = e / CB, col = e - row * CB;
-        pp[(size_t)row * n + col] = acc[q];
-    }
+    for (int q = 0; q < PER; ++q)
+        pp[(size_t)fft_out_index<LOGN, CB>(q) * n] = acc[q];
 }
 
 // subimage[tower of run r] += sum of the partial images of the run's chunks
@@ -719,6 +779,8 @@ static int band_height_of(const rg_grid_params *p, bool fp32)
 {
     if (p->tile == 16 && fp32)
         return 16;
+    if (p->tile == 4)
+        return 4;
     return 8;
 }
 
@@ -730,6 +792,7 @@ struct BandLaunch {
     const int32_t *rank_of_slot;
     int t0;
     const void *kern, *tw;
+    uint32_t kern_bytes; // kernel table size (< 2^31)
     void *G, *spill;
     uint32_t spill_cap;
     uint32_t *spill_ctr, *item_spill;
@@ -742,9 +805,10 @@ static int launch_bands_h(const BandLaunch &a, hipStream_t s)
     using T2 = typename Vec2<T>::type;
     constexpr int GRP = S * H >= 256 ? 1 : 256 / (S * H);
     hipLaunchKernelGGL((k_tower_band_items<T, S, H, LOGN, GRP>), dim3(a.blocks), dim3(S * H * GRP),
-                       0, s, (const Rec<T> *)a.recs, a.items, a.item_start, a.key_begin,
-                       a.key_end, a.rank_of_slot, a.t0, (const T2 *)a.kern, (const T2 *)a.tw,
-                       (T2 *)a.G, (T2 *)a.spill, a.spill_cap, a.spill_ctr, a.item_spill);
+                       0, s, (const BandRec<T> *)a.recs, a.items, a.item_start, a.key_begin,
+                       a.key_end, a.rank_of_slot, a.t0, (const T2 *)a.kern, a.kern_bytes,
+                       (const T2 *)a.tw, (T2 *)a.G, (T2 *)a.spill, a.spill_cap, a.spill_ctr,
+                       a.item_spill);
     RG_HIP(hipGetLastError());
     hipLaunchKernelGGL((k_band_combine<T2, H, LOGN>), dim3(a.blocks), dim3(256), 0, s, a.items,
                        a.item_start, a.key_begin, a.key_end, a.rank_of_slot, a.t0, (T2 *)a.G,
@@ -760,6 +824,8 @@ static int launch_bands_t(int H, const BandLaunch &a, hipStream_t s)
         if (H == 16)
             return launch_bands_h<T, S, 16, LOGN>(a, s);
     }
+    if (H == 4)
+        return launch_bands_h<T, S, 4, LOGN>(a, s);
     return launch_bands_h<T, S, 8, LOGN>(a, s);
 }
 
@@ -789,6 +855,9 @@ static int launch_cols_t(int H, const void *G, const uint32_t *ks, const int32_t
     if (H == 16)
         return launch_cols_h<T, LOGN, 16>(G, ks, tslot, P, cf, cc, chunk_base, nchunks, t0, twp,
                                           tw, theta, winc, s);
+    if (H == 4)
+        return launch_cols_h<T, LOGN, 4>(G, ks, tslot, P, cf, cc, chunk_base, nchunks, t0, twp,
+                                         tw, theta, winc, s);
     return launch_cols_h<T, LOGN, 8>(G, ks, tslot, P, cf, cc, chunk_base, nchunks, t0, twp, tw,
                                      theta, winc, s);
 }
@@ -1095,8 +1164,8 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
         // every item's records sorted by footprint column into the expanded
         // array (window records laid out item after item)
         const int64_t nexp = (int64_t)kf.mk * vis->n;
-        RG_TRY(ctx->scratch[11].ensure(sizeof(Rec<T>) * (size_t)std::max<int64_t>(nexp, 1)));
-        Rec<T> *sorted = ctx->scratch[11].as<Rec<T>>();
+        RG_TRY(ctx->scratch[11].ensure(sizeof(BandRec<T>) * (size_t)std::max<int64_t>(nexp, 1)));
+        BandRec<T> *sorted = ctx->scratch[11].as<BandRec<T>>();
         uint32_t *item_start_w = br.item_start;
         Item *items_w = br.items;
         {
@@ -1111,7 +1180,7 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
                                dim3((unsigned)std::min<int64_t>(max_items_w, 8192)), dim3(256), 0, s,
                                (const Rec<T> *)br.recs, sorted, (const Item *)items_w,
                                (const unsigned long long *)ctx->counters.p, (const uint32_t *)prefix,
-                               (const uint32_t *)d_wstart, n + S);
+                               (const uint32_t *)d_wstart, n + S, S, H, NB);
             RG_HIP(hipGetLastError());
         }
         BandLaunch bl;
@@ -1120,6 +1189,15 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
         bl.item_start = item_start_w;
         bl.rank_of_slot = d_tab + o_ros;
         bl.kern = wk->data;
+        {
+            // taps are addressed by 32-bit byte offsets (buffer loads)
+            const int64_t kb = (int64_t)wk->plane_count * wk->oversampling * wk->oversampling * S * S *
+                               (int64_t)sizeof(T2);
+            if (kb >= ((int64_t)1 << 31))
+                return fail(RG_ERR_UNSUPPORTED, "batched W-towers kernel table too large (%lld bytes)",
+                            (long long)kb);
+            bl.kern_bytes = (uint32_t)kb;
+        }
         bl.tw = tw_inv;
         bl.G = G;
         bl.spill = spill;
