@@ -89,7 +89,8 @@ typedef struct rg_grid_params {
     int32_t subgrid_margin; /* W-towers margin M */
     double winc;            /* W-towers floor spacing (wavelengths) */
     int32_t tile;           /* batched W-projection: tile edge in cells (32 or 48, 0 = 32);
-                               batched W-towers: band height in rows (8 or 16, 0 = 8) */
+                               batched W-towers: band height in rows (4, 8 or 16 (16 in
+                               FP32 only), 0 = 8) */
     int32_t vis_per_item;   /* batched visibilities per work item (0 = default) */
     int32_t compensated;    /* batched FP32: 0 plain FP32 tile sums (default), 1 two-sum
                                compensated tile sums (worth it only into an RG_C128 grid) */

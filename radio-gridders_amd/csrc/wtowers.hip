@@ -514,7 +514,8 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
 
 // Sums the spilled parts of every key split over several items (in item
 // order) into the task's subgrid band.  Launched with the grid of
-// k_tower_band_items; the first item of each split key does the work.
+// k_tower_band_items; the P blocks of a key split into P parts each sum a
+// 1 / P share of the band's elements.
 template <typename T2, int H, int LOGN>
 __global__ __launch_bounds__(256) void k_band_combine(const Item *__restrict__ items,
                                                       const uint32_t *__restrict__ item_start,
@@ -532,13 +533,17 @@ __global__ __launch_bounds__(256) void k_band_combine(const Item *__restrict__ i
         return;
     const uint32_t key = items[it].key;
     const uint32_t k0 = item_start[key], k1 = item_start[key + 1];
-    if (it != k0 || k1 - k0 < 2)
+    const uint32_t parts = k1 - k0;
+    if (parts < 2)
         return;
     const uint32_t tslot = key / (uint32_t)NB;
     const int b = (int)(key - tslot * (uint32_t)NB);
     const int j = rank_of_slot[tslot] - t0;
     T2 *dst = G + (size_t)j * n * n;
-    for (int e = threadIdx.x; e < H * n; e += blockDim.x) {
+    const uint32_t share = it - k0;
+    const int e0 = (int)((uint64_t)(H * n) * share / parts);
+    const int e1 = (int)((uint64_t)(H * n) * (share + 1) / parts);
+    for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x) {
         T2 a;
         a.x = 0;
         a.y = 0;

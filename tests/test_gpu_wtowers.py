@@ -43,18 +43,18 @@ def test_plain_fp64_matches_oracle(ctx, case):
     assert st.n_tasks > 1 and st.n_towers >= 1
 
 
-@pytest.mark.parametrize("floor_batch", [1, 5, 128])
-def test_batched_fp64_matches_oracle(ctx, case, floor_batch):
+@pytest.mark.parametrize("floor_batch,band", [(1, 8), (5, 8), (128, 8), (5, 4)])
+def test_batched_fp64_matches_oracle(ctx, case, floor_batch, band):
     obs, vis, table, ref = case
     g, st = run(ctx, obs, vis, table, rgamd.RG_MODE_BATCHED, floor_batch=floor_batch,
-                vis_per_item=50)
+                vis_per_item=50, tile=band)
     assert rel_max(g, ref) < 1e-11
     assert st.n_gridded == obs.n_vis and st.n_dropped == 0
 
 
-@pytest.mark.parametrize("band,per_item", [(8, 0), (8, 37), (16, 0), (16, 37)])
+@pytest.mark.parametrize("band,per_item", [(8, 0), (8, 37), (16, 0), (16, 37), (4, 0), (4, 37)])
 def test_batched_fp32_within_tolerance(ctx, case, band, per_item):
-    """FP32 bands of 8 / 16 rows; per_item 37 splits most band keys over
+    """FP32 bands of 8 / 16 / 4 rows; per_item 37 splits most band keys over
     several work items (spilled parts summed by the combine pass)."""
     obs, vis, table, ref = case
     g, st = run(ctx, obs, vis, table, rgamd.RG_MODE_BATCHED, dt=torch.complex64, tile=band,
