@@ -166,6 +166,31 @@ int rg_fft2d(rg_context *ctx, void *data, int32_t type, int32_t n, int32_t batch
 /* In-place quadrant swap of an even-sized n x n array (fft_shift_kernel). */
 int rg_fft_shift(void *data, int32_t type, int32_t n, void *stream);
 
+/* ---- image conventions of the CPU reference pipeline (main.c:151-154, 258-281) ----
+ * Uniform weighting (weight, src/reference/grid.c:59-81): every visibility is
+ * divided by the number of visibilities in its cell (y + N/2) N + (x + N/2),
+ * x = floor(theta u + 1/2), y = floor(theta v + 1/2) (coord, grid.c:25-35).
+ * vis (device, RG_C128 or RG_C64) is modified in place; the quotient is taken
+ * in FP64.  A visibility whose cell lies outside the grid (the reference
+ * indexes out of bounds) is neither counted nor changed.  counts: optional
+ * device array of N * N uint32 that receives the per-cell counts (NULL: the
+ * context's scratch memory). */
+int rg_weight(rg_context *ctx, const double *u, const double *v, int64_t n, void *vis,
+              int32_t vis_type, int32_t grid_size, double theta, uint32_t *counts,
+              void *stream);
+
+/* Hermitian completion of an n x n uv grid in place (make_hermitian,
+ * src/reference/grid.c:639-662): cells base + k and n^2 - 1 - k (base = n + 1
+ * for even n, 0 for odd n) each add the other's conjugate, for every k up to
+ * the zero frequency (n + 1)(n / 2), which adds its own conjugate. */
+int rg_make_hermitian(void *grid, int32_t type, int32_t n, void *stream);
+
+/* Grid correction: image[r][c] /= taper[r] * taper[c] (FP64 quotient), taper a
+ * device array of n doubles, e.g. the kernel's image-plane taper A at the
+ * pixel positions (no counterpart in the reference, whose images keep the
+ * taper). */
+int rg_gcf_correct(void *image, int32_t type, int32_t n, const double *taper, void *stream);
+
 /* ---- direct DFT (array level) ----
  * out[i] (RG_C128) = sum_v V exp(-2 pi i (l u + m v + (1 - sqrt(1-l^2-m^2)) w))
  * with l = (py[i] - N/2)/lambda, m = (px[i] - N/2)/lambda
@@ -234,6 +259,16 @@ int rg_image_dft_host(const char *visfile, void *grid_host, void *grid_dev, int 
 /* Selects the gridding mode the file-level wrappers use (default RG_MODE_BATCHED
  * with FP64 data, i.e. the same results as the plain path to FP64 rounding). */
 int rg_set_file_mode(int32_t mode);
+
+/* Image convention of the file-level gridding wrappers:
+ * RG_IMAGE_GPU (default): the GPU reference's image (grid_cuda.cu:907-918),
+ * no weighting, shift -> inverse FFT -> shift;
+ * RG_IMAGE_CPU: the CPU reference's image (src/reference/main.c:151-154,
+ * 258-281): uniform weighting before gridding, hermitian completion, then
+ * shift -> forward FFT -> shift. */
+#define RG_IMAGE_GPU 0
+#define RG_IMAGE_CPU 1
+int rg_set_image_convention(int32_t convention);
 
 /* ---- HDF5 files (host; libhdf5 >= 1.10 loaded at run time) ---- */
 /* 1 when libhdf5 could be loaded (RG_HDF5_LIB overrides the search). */

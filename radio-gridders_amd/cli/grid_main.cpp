@@ -4,6 +4,8 @@
 // -winc -bl_min -bl_max -wproj -flat -device, plus
 //   -float_args  parse theta/lambda as float like the reference CLI (default double)
 //   -plain       use the reference-structured FP64 gridder instead of the batched one
+//   -cpu_image   the CPU reference's image (grid_CPU.out --image, src/reference/main.c):
+//                uniform weighting, hermitian completion, forward FFT
 // Output: N rows of N doubles, the real part of the image (grid.cpp:257-274).
 #include <hip/hip_runtime_api.h>
 
@@ -32,7 +34,9 @@ static void show_help()
     printf("\t-flat                Flat visibility layout (the only layout; accepted)\n");
     printf("\t-device              GPU Device to use (Default: 0)\n");
     printf("\t-float_args          Parse theta/lambda as float (reference CLI behaviour)\n");
-    printf("\t-plain               Reference-structured FP64 gridder\n\n");
+    printf("\t-plain               Reference-structured FP64 gridder\n");
+    printf("\t-cpu_image           CPU reference image: uniform weighting, hermitian\n");
+    printf("\t                     completion, forward FFT (grid_CPU.out --image)\n\n");
 }
 
 int main(int argc, char **argv)
@@ -89,6 +93,8 @@ int main(int argc, char **argv)
     }
     if (has_flag(argc, argv, "plain"))
         rg_set_file_mode(RG_MODE_PLAIN);
+    if (has_flag(argc, argv, "cpu_image"))
+        rg_set_image_convention(RG_IMAGE_CPU);
     const size_t total = (size_t)grid_size * grid_size;
     void *grid_dev = nullptr;
     if (hipMalloc(&grid_dev, 16 * total) != hipSuccess) {

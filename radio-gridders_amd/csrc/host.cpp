@@ -395,6 +395,43 @@ int rg_fft_shift(void *data, int32_t type, int32_t n, void *stream)
     return fft_shift(data, type, n, 1, (hipStream_t)stream);
 }
 
+int rg_weight(rg_context *ctx, const double *u, const double *v, int64_t n, void *vis,
+              int32_t vis_type, int32_t grid_size, double theta, uint32_t *counts, void *stream)
+{
+    if (!ctx)
+        return fail(RG_ERR_PARAM, "NULL context");
+    if (n < 0 || (n > 0 && (!u || !v || !vis)))
+        return fail(RG_ERR_PARAM, "invalid visibility arrays");
+    if (vis_type != RG_C128 && vis_type != RG_C64)
+        return fail(RG_ERR_PARAM, "unknown visibility type %d", vis_type);
+    if (grid_size <= 0 || grid_size > 65536 || !(theta > 0))
+        return fail(RG_ERR_PARAM, "invalid grid geometry (N %d, theta %g)", grid_size, theta);
+    RG_HIP(hipSetDevice(ctx->device));
+    return weight_vis(ctx, u, v, n, vis, vis_type, grid_size, theta, counts, (hipStream_t)stream);
+}
+
+int rg_make_hermitian(void *grid, int32_t type, int32_t n, void *stream)
+{
+    if (!grid)
+        return fail(RG_ERR_PARAM, "NULL argument");
+    if (type != RG_C128 && type != RG_C64)
+        return fail(RG_ERR_PARAM, "unknown type %d", type);
+    if (n <= 0 || n > 65536)
+        return fail(RG_ERR_PARAM, "invalid grid size %d", n);
+    return make_hermitian(grid, type, n, (hipStream_t)stream);
+}
+
+int rg_gcf_correct(void *image, int32_t type, int32_t n, const double *taper, void *stream)
+{
+    if (!image || !taper)
+        return fail(RG_ERR_PARAM, "NULL argument");
+    if (type != RG_C128 && type != RG_C64)
+        return fail(RG_ERR_PARAM, "unknown type %d", type);
+    if (n <= 0 || n > 65536)
+        return fail(RG_ERR_PARAM, "invalid grid size %d", n);
+    return gcf_correct(image, type, n, taper, (hipStream_t)stream);
+}
+
 int rg_dft(rg_context *ctx, const rg_flat_vis *vis, const int32_t *px, const int32_t *py,
            int64_t npix, int32_t grid_size, double lambda, void *out, int32_t mode, void *stream)
 {

@@ -490,6 +490,7 @@ static int upload_kern(const HostKern &k, DeviceInputs &d)
 }
 
 static int32_t g_file_mode = RG_MODE_BATCHED;
+static int32_t g_image_conv = RG_IMAGE_GPU;
 
 static int file_grid(const char *visfile, const char *wkernfile, void *grid, int grid_size,
                      double theta, double bl_min, double bl_max, int subgrid, int margin,
@@ -522,7 +523,11 @@ static int file_grid(const char *visfile, const char *wkernfile, void *grid, int
             p.subgrid_margin = margin;
             p.winc = winc;
             p.compensated = 1;
-            st = rg_grid(ctx, &p, &d.fv, &d.wk, grid, RG_C128, nullptr);
+            if (g_image_conv == RG_IMAGE_CPU) // weight before gridding (main.c:151-154)
+                st = rg_weight(ctx, d.fv.u, d.fv.v, d.fv.n, d.vis.p, RG_C128, grid_size, theta,
+                               nullptr, nullptr);
+            if (st == RG_OK)
+                st = rg_grid(ctx, &p, &d.fv, &d.wk, grid, RG_C128, nullptr);
         }
         if (st == RG_OK) {
             rg_grid_stats gs;
@@ -531,8 +536,20 @@ static int file_grid(const char *visfile, const char *wkernfile, void *grid, int
                 st = fail(RG_ERR_RANGE, "%lld visibilities fall outside the W-kernel table's w range",
                           (long long)gs.n_bad_plane);
         }
-        if (st == RG_OK)
-            st = rg_grid_to_image(ctx, grid, RG_C128, grid_size, nullptr);
+        if (st == RG_OK) {
+            if (g_image_conv == RG_IMAGE_CPU) {
+                // make_hermitian, shift, forward FFT, shift (main.c:258-281)
+                st = rg_make_hermitian(grid, RG_C128, grid_size, nullptr);
+                if (st == RG_OK)
+                    st = rg_fft_shift(grid, RG_C128, grid_size, nullptr);
+                if (st == RG_OK)
+                    st = rg_fft2d(ctx, grid, RG_C128, grid_size, 1, -1, nullptr);
+                if (st == RG_OK)
+                    st = rg_fft_shift(grid, RG_C128, grid_size, nullptr);
+            } else {
+                st = rg_grid_to_image(ctx, grid, RG_C128, grid_size, nullptr);
+            }
+        }
         if (st == RG_OK) {
             hipError_t e = hipDeviceSynchronize();
             if (e != hipSuccess)
@@ -585,6 +602,14 @@ int rg_set_file_mode(int32_t mode)
     if (mode != RG_MODE_PLAIN && mode != RG_MODE_BATCHED)
         return fail(RG_ERR_PARAM, "unknown mode %d", mode);
     g_file_mode = mode;
+    return RG_OK;
+}
+
+int rg_set_image_convention(int32_t convention)
+{
+    if (convention != RG_IMAGE_GPU && convention != RG_IMAGE_CPU)
+        return fail(RG_ERR_PARAM, "unknown image convention %d", convention);
+    g_image_conv = convention;
     return RG_OK;
 }
 

@@ -112,7 +112,7 @@ struct rg_context {
     int last_kind = 0; // 0 none, 1 wproj, 2 wtowers
     int64_t last_nvis = 0;
     // generic scratch arrays
-    rg::DevBuf scratch[16];
+    rg::DevBuf scratch[17];
     // channel offsets resident in scratch[13] (uploaded only when they change,
     // so a repeated multi-channel call issues no copy and can be captured)
     std::vector<int64_t> chan_off;
@@ -162,5 +162,9 @@ int wtowers_run(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis
 int fft2d(rg_context *ctx, void *data, int type, int n, int batch, int sign, hipStream_t s);
 int fft_shift(void *data, int type, int n, int batch, hipStream_t s);
 int zero_fill(void *p, size_t bytes, hipStream_t s);
+int weight_vis(rg_context *ctx, const double *u, const double *v, int64_t n, void *vis,
+               int vis_type, int N, double theta, uint32_t *counts, hipStream_t s);
+int make_hermitian(void *grid, int type, int n, hipStream_t s);
+int gcf_correct(void *img, int type, int n, const double *taper, hipStream_t s);
 
 } // namespace rg

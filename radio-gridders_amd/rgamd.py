@@ -30,6 +30,8 @@ RG_MODE_PLAIN = 0
 RG_MODE_BATCHED = 1
 RG_DFT_FP64 = 0
 RG_DFT_MIXED = 1
+RG_IMAGE_GPU = 0
+RG_IMAGE_CPU = 1
 
 TIMERS = {"wproj_tiles": 0, "wproj_plain": 1, "binning": 2, "tower_tiles": 3, "tower_rows": 4,
           "tower_cols": 5, "tower_finish": 6}
@@ -84,6 +86,10 @@ SIGNATURES = [
     ("rg_grid_to_image", C.c_int, [_P, _P, _I, _I, _P]),
     ("rg_fft2d", C.c_int, [_P, _P, _I, _I, _I, _I, _P]),
     ("rg_fft_shift", C.c_int, [_P, _I, _I, _P]),
+    ("rg_weight", C.c_int, [_P, _P, _P, _L, _P, _I, _I, _D, _P, _P]),
+    ("rg_make_hermitian", C.c_int, [_P, _I, _I, _P]),
+    ("rg_gcf_correct", C.c_int, [_P, _I, _I, _P, _P]),
+    ("rg_set_image_convention", C.c_int, [_I]),
     ("rg_dft", C.c_int, [_P, C.POINTER(FlatVis), _P, _P, _L, _I, _D, _P, _I, _P]),
     ("rg_dft_image", C.c_int, [_P, C.POINTER(FlatVis), _I, _D, _P, _I, _I, _I, _P]),
     ("rg_predict_points", C.c_int, [_P, _P, _P, _L, _P, _P, _P, _I, _P, _I, _P]),
@@ -232,6 +238,13 @@ class Context:
         check(lib().rg_fft2d(self.h, _ptr(data), grid_type_of(data), n, batch, int(sign),
                              _stream(stream)))
 
+    def weight(self, u, v, vis, grid_size: int, theta: float, counts=None, stream=None):
+        """Uniform weighting of the device visibilities `vis` in place
+        (weight, src/reference/grid.c:59-81); `counts` (int32/uint32 device
+        tensor of grid_size^2) receives the per-cell counts when given."""
+        check(lib().rg_weight(self.h, _ptr(u), _ptr(v), u.numel(), _ptr(vis), grid_type_of(vis),
+                              int(grid_size), float(theta), _ptr(counts), _stream(stream)))
+
     def dft(self, fv: FlatVis, px, py, grid_size: int, lam: float, out, mode=RG_DFT_FP64,
             stream=None):
         check(lib().rg_dft(self.h, C.byref(fv), _ptr(px), _ptr(py), px.numel(), int(grid_size),
@@ -245,6 +258,23 @@ class Context:
 
 def fft_shift(data, stream=None):
     check(lib().rg_fft_shift(_ptr(data), grid_type_of(data), data.shape[-1], _stream(stream)))
+
+
+def make_hermitian(grid, stream=None):
+    """Hermitian completion of an n x n device grid in place
+    (make_hermitian, src/reference/grid.c:639-662)."""
+    check(lib().rg_make_hermitian(_ptr(grid), grid_type_of(grid), grid.shape[-1], _stream(stream)))
+
+
+def gcf_correct(image, taper, stream=None):
+    """image[r][c] /= taper[r] * taper[c] in place (taper: float64 device tensor)."""
+    check(lib().rg_gcf_correct(_ptr(image), grid_type_of(image), image.shape[-1], _ptr(taper),
+                               _stream(stream)))
+
+
+def set_image_convention(convention: int):
+    """Image convention of the file-level wrappers (RG_IMAGE_GPU / RG_IMAGE_CPU)."""
+    check(lib().rg_set_image_convention(int(convention)))
 
 
 def predict_points(u, v, w, l, m, flux, out, stream=None):

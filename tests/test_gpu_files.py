@@ -119,3 +119,27 @@ def test_clis_write_the_real_image(files):
     ys, xs = np.divmod(np.arange(N * N), N)
     ref = orc.dft(xs, ys, N, LAM, files["u"], files["v"], files["w"], files["vis"]).reshape(N, N)
     assert rel_max(got, ref.real) < 1e-11
+
+
+@pytest.mark.parametrize("wproj", [True, False])
+def test_cli_cpu_image_matches_oracle_pipeline(files, wproj):
+    """grid.out -cpu_image: the CPU reference's image (uniform weighting,
+    hermitian completion, shift / forward FFT / shift; src/reference/main.c)."""
+    d = files["dir"]
+    img = str(d / f"cpu_{wproj}.img")
+    args = [os.path.join(PKG, "grid.out"), f"-vis={files['visfile']}", f"-theta={THETA}",
+            f"-lambda={LAM}", "-flat", f"-image={img}", "-cpu_image"]
+    args += ([f"-wkernel={files['wkp']}", "-wproj"] if wproj else
+             [f"-wkernel={files['wkt']}", "-subgrid=64", "-margin=16", "-winc=10"])
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = np.fromfile(img, dtype=np.float64).reshape(N, N)
+    wvis, _ = orc.weight(N, THETA, files["u"], files["v"], files["vis"])
+    if wproj:
+        ref, _, _ = orc.grid_wprojection(N, THETA, files["u"], files["v"], files["w"], wvis,
+                                         files["tab"], files["w_min"], 10.0)
+    else:
+        ref, _, _ = orc.grid_wtowers(N, THETA, files["u"], files["v"], files["w"], wvis,
+                                     files["tabt"], -5.0, 1.0, 64, 16, 10.0)
+    ref = orc.fft_shift(orc.fft2d(orc.fft_shift(orc.make_hermitian(ref)), -1))
+    assert rel_max(got, ref.real) < 1e-11
