@@ -283,6 +283,14 @@ int rg_load_flat_vis(const char *visfile, double bl_min, double bl_max, int64_t 
 int rg_load_wkern(const char *wkernfile, double theta, int32_t *planes, int32_t *oversampling,
                   int32_t *size, double *w_min, double *w_step, double **data);
 void rg_free_host(void *p);
+
+/* Separable (Sze-Tan) kernel file (load_sep_kern / load_sep_kern_T,
+ * src/common/hdf5.c:372-500): dataset "sepkern/kern" (2-D FP64) read in file
+ * order into a malloc'd host array (free with rg_free_host), and the file
+ * attributes du, dw, x0 (NaN when absent).  transposed = 0: dims are
+ * [oversampling][size]; 1: [size][oversampling] (the _T loader). */
+int rg_load_sep_kern(const char *filename, int32_t transposed, int32_t *size,
+                     int32_t *oversampling, double *du, double *dw, double *x0, double **data);
 /* Writers of the same layouts: vis/<a1>/<a2>/{frequency, time, uvw, vis} with
  * uvw [n_bl][T][3] metres and vis [n_bl][T][F] complex128; and
  * wkern/<theta %g>/<w>/kern [R][R][S][S] complex128 for w = w_min + p w_step. */

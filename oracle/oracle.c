@@ -552,3 +552,210 @@ void orc_dft_flat(orc_c128 *out, const int *px, const int *py, int64_t npix,
         out[p].im = im;
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* W-stacking predict (src/wstack/predict.cpp, wstack_common.h)         */
+/* ------------------------------------------------------------------ */
+
+/* wstack_predict (predict.cpp:410-428): grid g = floor(theta lam), oversampled
+ * grid og = round(0.5 / x0) g, w_planes = ceil(sin(theta/2) sqrt(2 g^2) / dw)
+ * + aa_w + 1. */
+int orc_wstack_geometry(double theta, double lam, double dw, int aa_w, double x0, int *og,
+                        int *w_planes)
+{
+    const int g = (int)floor(theta * lam);
+    const double x0ih = round(0.5 / x0);
+    *og = (int)(x0ih * g);
+    /* generate_sky (predict.cpp:228-231) rounds after the product */
+    const int og_sky = (int)round(0.5 / x0 * g);
+    const double max_w = sin(theta / 2) * sqrt(2.0 * ((double)g * g));
+    *w_planes = (int)ceil(max_w / dw) + aa_w + 1;
+    return (*og > g && og_sky == *og) ? 0 : -1;
+}
+
+/* Quantised sky position of a point (generate_sky, predict.cpp:236-246). */
+static void wstack_quantise(double l, double m, double theta, double lam, int g, int *lc, int *mc,
+                            double *lq, double *mq)
+{
+    *lc = (int)floor((l / theta + 0.5) * (double)g);
+    *mc = (int)floor((m / theta + 0.5) * (double)g);
+    *lq = (double)*lc / lam - theta / 2;
+    *mq = (double)*mc / lam - theta / 2;
+}
+
+/* The W-stack (predict.cpp:182-216, 219-278, 438-497): the sky of point
+ * sources (flux / correction at the quantised pixel (row mc, column lc) of
+ * the inner g x g region), fft-shifted, times F^(t0 + k) with
+ * F = exp(2 pi i dw (1 - sqrt(1 - l^2 - m^2))) on the inner region and
+ * t0 = -(w_planes / 2), forward 2-D FFT, fft-shifted: plane k of
+ * stack[w_planes][og][og] (row v, column u).  F^(t0 + k) is evaluated as
+ * exp(2 pi i (t0 + k) ph) (the reference takes an integer power of F and
+ * then multiplies by F per plane: equal to rounding).  Points whose pixel or
+ * correction index falls outside its table are skipped (the reference
+ * indexes out of bounds); returns their number. */
+int orc_wstack_build(orc_c128 *stack, double theta, double lam, const double *pl, const double *pm,
+                     const double *flux, int npts, double du, double dw, int aa_w, double x0,
+                     const double *corr_lm, int corr_lm_n, const double *corr_n, int corr_n_n)
+{
+    int og, w_planes;
+    if (orc_wstack_geometry(theta, lam, dw, aa_w, x0, &og, &w_planes) != 0)
+        return -1;
+    const int g = (int)floor(theta * lam);
+    const int gd = (og - g) / 2;
+    const size_t plane = (size_t)og * og;
+    orc_c128 *sky = calloc(plane, sizeof(orc_c128));
+    double *ph = calloc(plane, sizeof(double));
+    char *inner = calloc(plane, 1);
+    int dropped = 0;
+    for (int i = 0; i < npts; i++) {
+        int lc, mc;
+        double lq, mq;
+        wstack_quantise(pl[i], pm[i], theta, lam, g, &lc, &mc, &lq, &mq);
+        const double n = sqrt(1.0 - lq * lq - mq * mq) - 1.0;
+        const double lm_step = 1.0 / (double)corr_lm_n, n_step = 1.0 / (double)corr_n_n;
+        const int aau = (int)floor((du * lq) / lm_step) + corr_lm_n / 2;
+        const int aav = (int)floor((du * mq) / lm_step) + corr_lm_n / 2;
+        const int aaw = (int)floor((dw * n) / n_step) + corr_n_n / 2;
+        if (lc < 0 || lc >= g || mc < 0 || mc >= g || aau < 0 || aau >= corr_lm_n || aav < 0 ||
+            aav >= corr_lm_n || aaw < 0 || aaw >= corr_n_n) {
+            dropped++;
+            continue;
+        }
+        double a = 1.0;
+        a *= corr_lm[aau];
+        a *= corr_lm[aav];
+        a *= corr_n[aaw];
+        const double f = flux ? flux[i] : 1.0;
+        orc_c128 *s = &sky[(size_t)(gd + mc) * og + (gd + lc)];
+        s->re += f / a;
+        s->im += 0.0 / a;
+    }
+    for (int y = 0; y < g; y++) {
+        for (int x = 0; x < g; x++) {
+            const double l = theta * ((double)x - g / 2) / g;
+            const double m = theta * ((double)y - g / 2) / g;
+            const size_t o = (size_t)(y + gd) * og + (x + gd);
+            ph[o] = dw * (1.0 - sqrt(1.0 - l * l - m * m));
+            inner[o] = 1;
+        }
+    }
+    /* fft_shift_2Darray of the sky and of the screen (quadrant swap) */
+    orc_fft_shift(sky, og);
+    for (int y = 0; y < og / 2; y++) {
+        for (int x = 0; x < og; x++) {
+            const size_t a0 = (size_t)y * og + x;
+            const size_t a1 = (size_t)(y + og / 2) * og + (x + og / 2) % og;
+            double t = ph[a0];
+            ph[a0] = ph[a1];
+            ph[a1] = t;
+            char c = inner[a0];
+            inner[a0] = inner[a1];
+            inner[a1] = c;
+        }
+    }
+    const int t0 = -(w_planes / 2);
+    for (int k = 0; k < w_planes; k++) {
+        orc_c128 *p = stack + (size_t)k * plane;
+        const double e = (double)(t0 + k);
+        for (size_t o = 0; o < plane; o++) {
+            if (!inner[o]) {
+                p[o].re = p[o].im = 0.0;
+                continue;
+            }
+            const double c = cos(TWO_PI * e * ph[o]), s = sin(TWO_PI * e * ph[o]);
+            p[o].re = sky[o].re * c - sky[o].im * s;
+            p[o].im = sky[o].re * s + sky[o].im * c;
+        }
+        orc_fft2d(p, og, -1);
+        orc_fft_shift(p, og);
+    }
+    free(sky);
+    free(ph);
+    free(inner);
+    return dropped;
+}
+
+/* deconvolve_visibility_ (wstack_common.h:420-470): separable uv / w kernel
+ * taps over the stack,
+ *   vis = sum_{dwi, dvi, dui} stack[dws][dvs][dus] (cw cv) cu,
+ * dus = ceil(u/du) + g + dui, dvs = ceil(v/du) + g + dvi,
+ * dws = ceil(w/dw) + w_planes/2 + dwi, dui, dvi in [-aa_uv/2, aa_uv/2),
+ * dwi in [-aa_w/2, aa_w/2), tap rows aa * ov + (d + aa/2) with
+ * ov = floor(|x - ceil(x/d) d| / d * oversampling) (clamped to
+ * oversampling - 1).  A visibility whose taps leave the stack is not predicted
+ * (0) and counted. */
+int64_t orc_wstack_degrid(orc_c128 *out, const double *u, const double *v, const double *w,
+                          int64_t nvis, const orc_c128 *stack, int og, int w_planes, int g,
+                          double du, double dw, int aa_uv, int aa_w, const double *conv_uv,
+                          int ov_uv, const double *conv_w, int ov_w, int nthreads)
+{
+    int64_t dropped = 0;
+    const int aa_h = aa_uv / 2, aaw_h = aa_w / 2;
+#ifdef _OPENMP
+    if (nthreads <= 0)
+        nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(static, 1000) reduction(+ : dropped) num_threads(nthreads)
+#endif
+    for (int64_t i = 0; i < nvis; i++) {
+        const double cu = ceil(u[i] / du), cv = ceil(v[i] / du), cw = ceil(w[i] / dw);
+        const double flu = u[i] - cu * du, flv = v[i] - cv * du, flw = w[i] - cw * dw;
+        int ovu = (int)floor(fabs(flu) / du * ov_uv);
+        int ovv = (int)floor(fabs(flv) / du * ov_uv);
+        int ovw = (int)floor(fabs(flw) / dw * ov_w);
+        ovu = ovu < ov_uv ? ovu : ov_uv - 1;
+        ovv = ovv < ov_uv ? ovv : ov_uv - 1;
+        ovw = ovw < ov_w ? ovw : ov_w - 1;
+        const int us = (int)cu + g, vs = (int)cv + g, ws = (int)cw + w_planes / 2;
+        orc_c128 acc = { 0.0, 0.0 };
+        if (!(cu > -1e9 && cu < 1e9 && cv > -1e9 && cv < 1e9 && cw > -1e9 && cw < 1e9) ||
+            us - aa_h < 0 || us + aa_h > og || vs - aa_h < 0 || vs + aa_h > og ||
+            ws - aaw_h < 0 || ws + aaw_h > w_planes) {
+            dropped++;
+            out[i] = acc;
+            continue;
+        }
+        for (int dwi = -aaw_h; dwi < aaw_h; dwi++) {
+            const double gw = conv_w[aa_w * ovw + (dwi + aaw_h)];
+            for (int dvi = -aa_h; dvi < aa_h; dvi++) {
+                const double gv = gw * conv_uv[aa_uv * ovv + (dvi + aa_h)];
+                const orc_c128 *row = stack + ((size_t)(ws + dwi) * og + (vs + dvi)) * og;
+                for (int dui = -aa_h; dui < aa_h; dui++) {
+                    const double gu = gv * conv_uv[aa_uv * ovu + (dui + aa_h)];
+                    acc.re += row[us + dui].re * gu;
+                    acc.im += row[us + dui].im * gu;
+                }
+            }
+        }
+        out[i] = acc;
+    }
+    return dropped;
+}
+
+/* predict_visibility_quantized_vec (predict.cpp:88-131): the direct sum over
+ * the quantised points, sum flux exp(-2 pi i (u lq + v mq + w (n - 1))). */
+void orc_predict_quantized(orc_c128 *out, const double *pl, const double *pm, const double *flux,
+                           int npts, double theta, double lam, const double *u, const double *v,
+                           const double *w, int64_t nvis, int nthreads)
+{
+    const int g = (int)floor(theta * lam);
+#ifdef _OPENMP
+    if (nthreads <= 0)
+        nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+#endif
+    for (int64_t i = 0; i < nvis; i++) {
+        double re = 0.0, im = 0.0;
+        for (int p = 0; p < npts; p++) {
+            int lc, mc;
+            double lq, mq;
+            wstack_quantise(pl[p], pm[p], theta, lam, g, &lc, &mc, &lq, &mq);
+            const double n = sqrt(1.0 - lq * lq - mq * mq) - 1.0;
+            const double ang = -TWO_PI * (u[i] * lq + v[i] * mq + w[i] * n);
+            const double f = flux ? flux[p] : 1.0;
+            re += f * cos(ang);
+            im += f * sin(ang);
+        }
+        out[i].re = re;
+        out[i].im = im;
+    }
+}

@@ -125,6 +125,20 @@ void orc_dft_flat(orc_c128 *out, const int *px, const int *py, int64_t npix,
                   const double *u, const double *v, const double *w,
                   const orc_c128 *vis, int64_t nvis, int nthreads);
 
+/* W-stacking predict (src/wstack/predict.cpp, wstack_common.h). */
+int orc_wstack_geometry(double theta, double lam, double dw, int aa_w, double x0, int *og,
+                        int *w_planes);
+int orc_wstack_build(orc_c128 *stack, double theta, double lam, const double *pl, const double *pm,
+                     const double *flux, int npts, double du, double dw, int aa_w, double x0,
+                     const double *corr_lm, int corr_lm_n, const double *corr_n, int corr_n_n);
+int64_t orc_wstack_degrid(orc_c128 *out, const double *u, const double *v, const double *w,
+                          int64_t nvis, const orc_c128 *stack, int og, int w_planes, int g,
+                          double du, double dw, int aa_uv, int aa_w, const double *conv_uv,
+                          int ov_uv, const double *conv_w, int ov_w, int nthreads);
+void orc_predict_quantized(orc_c128 *out, const double *pl, const double *pm, const double *flux,
+                           int npts, double theta, double lam, const double *u, const double *v,
+                           const double *w, int64_t nvis, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

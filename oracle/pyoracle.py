@@ -70,6 +70,15 @@ def lib(fast: bool = False):
         L.orc_grid_to_image.restype = None
         L.orc_dft_flat.argtypes = [_P, _P, _P, _L, _I, _D, _P, _P, _P, _P, _L, _I]
         L.orc_dft_flat.restype = None
+        L.orc_wstack_geometry.argtypes = [_D, _D, _D, _I, _D, C.POINTER(_I), C.POINTER(_I)]
+        L.orc_wstack_geometry.restype = _I
+        L.orc_wstack_build.argtypes = [_P, _D, _D, _P, _P, _P, _I, _D, _D, _I, _D, _P, _I, _P, _I]
+        L.orc_wstack_build.restype = _I
+        L.orc_wstack_degrid.argtypes = [_P, _P, _P, _P, _L, _P, _I, _I, _I, _D, _D, _I, _I, _P, _I,
+                                        _P, _I, _I]
+        L.orc_wstack_degrid.restype = _L
+        L.orc_predict_quantized.argtypes = [_P, _P, _P, _P, _I, _D, _D, _P, _P, _P, _L, _I]
+        L.orc_predict_quantized.restype = None
         _libs[name] = L
     return _libs[name]
 
@@ -196,4 +205,51 @@ def dft(px, py, N, lam, u, v, w, vis, nthreads=0):
     lib().orc_dft_flat(out.ctypes.data, px.ctypes.data, py.ctypes.data, len(px), N, lam,
                        u.ctypes.data, v.ctypes.data, w.ctypes.data, vis.ctypes.data, len(u),
                        nthreads or (os.cpu_count() or 1))
+    return out
+
+
+def wstack_geometry(theta, lam, dw, aa_w, x0):
+    """(oversampled grid size, number of w-planes) of wstack_predict."""
+    og, wp = _I(), _I()
+    if lib().orc_wstack_geometry(theta, lam, dw, aa_w, x0, C.byref(og), C.byref(wp)) != 0:
+        raise ValueError("inconsistent W-stack geometry")
+    return og.value, wp.value
+
+
+def wstack_build(theta, lam, l, m, flux, du, dw, aa_w, x0, corr_lm, corr_n, fast=False):
+    """W-stack [w_planes][og][og] complex128 and the number of skipped points."""
+    og, wp = wstack_geometry(theta, lam, dw, aa_w, x0)
+    l, m = _c(l, np.float64), _c(m, np.float64)
+    fl = None if flux is None else _c(flux, np.float64)
+    clm, cn = _c(corr_lm, np.float64), _c(corr_n, np.float64)
+    stack = np.zeros((wp, og, og), np.complex128)
+    dropped = lib(fast).orc_wstack_build(stack.ctypes.data, theta, lam, l.ctypes.data, m.ctypes.data,
+                                         None if fl is None else fl.ctypes.data, len(l), du, dw,
+                                         aa_w, x0, clm.ctypes.data, len(clm), cn.ctypes.data, len(cn))
+    return stack, dropped
+
+
+def wstack_degrid(stack, g, u, v, w, du, dw, conv_uv, aa_uv, ov_uv, conv_w, aa_w, ov_w,
+                  nthreads=0, fast=False):
+    """Predicted visibilities (complex128) and the number not predicted."""
+    u, v, w = _c(u, np.float64), _c(v, np.float64), _c(w, np.float64)
+    cuv, cw = _c(conv_uv, np.float64), _c(conv_w, np.float64)
+    stack = np.ascontiguousarray(stack, dtype=np.complex128)
+    out = np.zeros(len(u), np.complex128)
+    dropped = lib(fast).orc_wstack_degrid(out.ctypes.data, u.ctypes.data, v.ctypes.data,
+                                          w.ctypes.data, len(u), stack.ctypes.data, stack.shape[1],
+                                          stack.shape[0], g, du, dw, aa_uv, aa_w, cuv.ctypes.data,
+                                          ov_uv, cw.ctypes.data, ov_w, nthreads)
+    return out, dropped
+
+
+def predict_quantized(l, m, flux, theta, lam, u, v, w, nthreads=0):
+    """Direct sum over the quantised points (predict_visibility_quantized_vec)."""
+    l, m = _c(l, np.float64), _c(m, np.float64)
+    fl = None if flux is None else _c(flux, np.float64)
+    u, v, w = _c(u, np.float64), _c(v, np.float64), _c(w, np.float64)
+    out = np.zeros(len(u), np.complex128)
+    lib().orc_predict_quantized(out.ctypes.data, l.ctypes.data, m.ctypes.data,
+                                None if fl is None else fl.ctypes.data, len(l), theta, lam,
+                                u.ctypes.data, v.ctypes.data, w.ctypes.data, len(u), nthreads)
     return out

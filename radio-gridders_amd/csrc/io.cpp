@@ -708,6 +708,70 @@ int rg_load_wkern(const char *wkernfile, double theta, int32_t *planes, int32_t 
     return RG_OK;
 }
 
+// load_sep_kern / load_sep_kern_T (src/common/hdf5.c:372-500): dataset
+// "sepkern/kern" (2-D, FP64) and the file attributes du, dw, x0.  The data are
+// read in file order; the two loaders differ only in which dimension they
+// call the oversampling.
+int rg_load_sep_kern(const char *filename, int32_t transposed, int32_t *size,
+                     int32_t *oversampling, double *du, double *dw, double *x0, double **data)
+{
+    using namespace h5;
+    if (!filename || !size || !oversampling || !du || !dw || !x0 || !data)
+        return fail(RG_ERR_PARAM, "NULL argument");
+    Api *a = api();
+    if (!a)
+        return fail(RG_ERR_UNSUPPORTED, "libhdf5 (>= 1.10) could not be loaded");
+    hid_t f = a->fopen(filename, ACC_RDONLY, P_DEFAULT);
+    if (f < 0)
+        return fail(RG_ERR_IO, "could not open separable kernel file %s", filename);
+    hid_t ds = a->lexists(f, "sepkern", P_DEFAULT) > 0 ? a->dopen2(f, "sepkern/kern", P_DEFAULT) : -1;
+    if (ds < 0) {
+        a->fclose(f);
+        return fail(RG_ERR_IO, "'sepkern/kern' dataset could not be opened from file %s", filename);
+    }
+    hid_t sp = a->dget_space(ds), ty = a->dget_type(ds);
+    hsize_t dims[2] = {0, 0};
+    const bool fmt = a->sget_simple_extent_ndims(sp) == 2 && a->tget_size(ty) == sizeof(double) &&
+                     a->sget_simple_extent_dims(sp, dims, nullptr) >= 0 && dims[0] > 0 &&
+                     dims[1] > 0;
+    a->sclose(sp);
+    a->tclose(ty);
+    int st = RG_OK;
+    std::vector<double> buf;
+    if (!fmt) {
+        st = fail(RG_ERR_SHAPE, "'sepkern/kern' dataset has wrong format in file %s", filename);
+    } else {
+        buf.resize((size_t)dims[0] * dims[1]);
+        if (a->dread(ds, *a->t_ieee_f64le, S_ALL, S_ALL, P_DEFAULT, buf.data()) < 0)
+            st = fail(RG_ERR_IO, "failed to read separable kernel data from %s", filename);
+    }
+    a->dclose(ds);
+    auto attr = [&](const char *name) {
+        double v = std::nan("");
+        if (a->aexists(f, name) > 0) {
+            hid_t at = a->aopen(f, name, P_DEFAULT);
+            if (at >= 0) {
+                if (a->aread(at, *a->t_ieee_f64le, &v) < 0)
+                    v = std::nan("");
+                a->aclose(at);
+            }
+        }
+        return v;
+    };
+    const double vdu = attr("du"), vdw = attr("dw"), vx0 = attr("x0");
+    a->fclose(f);
+    if (st != RG_OK)
+        return st;
+    *size = (int32_t)(transposed ? dims[0] : dims[1]);
+    *oversampling = (int32_t)(transposed ? dims[1] : dims[0]);
+    *du = vdu;
+    *dw = vdw;
+    *x0 = vx0;
+    *data = (double *)malloc(sizeof(double) * buf.size());
+    std::copy(buf.begin(), buf.end(), *data);
+    return RG_OK;
+}
+
 void rg_free_host(void *p) { free(p); }
 
 int rg_write_vis_hdf5(const char *filename, int32_t n_bl, const int32_t *a1, const int32_t *a2,

@@ -109,6 +109,8 @@ SIGNATURES = [
     ("rg_load_wkern", C.c_int, [C.c_char_p, _D, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I),
                                 C.POINTER(_D), C.POINTER(_D), C.POINTER(_P)]),
     ("rg_free_host", None, [_P]),
+    ("rg_load_sep_kern", C.c_int, [C.c_char_p, _I, C.POINTER(_I), C.POINTER(_I), C.POINTER(_D),
+                                   C.POINTER(_D), C.POINTER(_D), C.POINTER(_P)]),
     ("rg_write_vis_hdf5", C.c_int, [C.c_char_p, _I, _P, _P, _I, _I, _P, _P, _P, _P]),
     ("rg_write_wkern_hdf5", C.c_int, [C.c_char_p, _D, _I, _I, _I, _D, _D, _P]),
 ]
@@ -339,6 +341,30 @@ def load_wkern(path: str, theta: float) -> LoadedKern:
     lib().rg_free_host(p)
     return LoadedKern(arr.view(np.complex128).reshape(P.value, R.value, R.value, S.value, S.value),
                       wmin.value, wstep.value)
+
+
+@dataclass
+class SepKern:
+    """Separable kernel: data[oversampling][size] in file order, and the
+    spacing attributes of the file."""
+    data: np.ndarray
+    size: int
+    oversampling: int
+    du: float
+    dw: float
+    x0: float
+
+
+def load_sep_kern(path: str, transposed: bool = False) -> SepKern:
+    size, ov = C.c_int32(), C.c_int32()
+    du, dw, x0 = C.c_double(), C.c_double(), C.c_double()
+    p = C.c_void_p()
+    check(lib().rg_load_sep_kern(path.encode(), int(transposed), C.byref(size), C.byref(ov),
+                                 C.byref(du), C.byref(dw), C.byref(x0), C.byref(p)))
+    cnt = size.value * ov.value
+    arr = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_double)), shape=(cnt,)).copy()
+    lib().rg_free_host(p)
+    return SepKern(arr, size.value, ov.value, du.value, dw.value, x0.value)
 
 
 def write_vis_hdf5(path, a1, a2, time, freq, uvw, vis):
