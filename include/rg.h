@@ -205,6 +205,56 @@ int rg_dft(rg_context *ctx, const rg_flat_vis *vis, const int32_t *px, const int
 int rg_dft_image(rg_context *ctx, const rg_flat_vis *vis, int32_t grid_size, double lambda,
                  void *out, int32_t blocks, int32_t threads_block, int32_t mode, void *stream);
 
+/* ---- W-stacking predict (sky -> visibilities), separable kernels ----
+ * wstack_predict_cu_3D (src/wstack/predict.cuh:33-47; CPU semantics
+ * wstack_predict, src/wstack/predict.cpp:396-560).  Kernel tables are device
+ * arrays in the layout of the reference's loaders: convolution kernels
+ * data[oversampling][size] (row ov holds the size taps), correction
+ * functions size * oversampling samples. */
+typedef struct rg_sep_kern {
+    const double *data;   /* device */
+    int32_t size;
+    int32_t oversampling;
+} rg_sep_kern;
+
+typedef struct rg_wstack_params {
+    double theta;          /* field of view (radians) */
+    double lambda;         /* uv extent (wavelengths); g = floor(theta * lambda) */
+    double du, dw;         /* uv cell and w-plane spacing of the kernels */
+    double x0;             /* image fraction of the kernels: og = round(0.5 / x0) * g */
+    int32_t aa_support_uv; /* uv kernel support (its size) */
+    int32_t aa_support_w;  /* w kernel support (its size) */
+} rg_wstack_params;
+
+/* Builds the W-stack of a point-source sky in the context: w_planes planes
+ * of og x og RG_C128, plane k the fft-shifted forward FFT of the sky (flux
+ * (NULL: 1) / correction at each source's quantised pixel, generate_sky
+ * predict.cpp:219-278) times F^(k - w_planes/2), F the w-screen of spacing
+ * dw (generate_fresnel, predict.cpp:182-216).  l, m, flux are device arrays
+ * of npts.  Sources whose pixel or correction index falls outside its table
+ * are skipped (the reference indexes out of bounds).  og_out, planes_out,
+ * skipped_out: optional host outputs (skipped_out synchronises the stream). */
+int rg_wstack_build(rg_context *ctx, const rg_wstack_params *p, const double *l, const double *m,
+                    const double *flux, int32_t npts, const rg_sep_kern *corr_lm,
+                    const rg_sep_kern *corr_n, int32_t *og_out, int32_t *planes_out,
+                    int64_t *skipped_out, void *stream);
+
+/* Predicts nvis visibilities (device u, v, w in wavelengths; out RG_C128)
+ * from the context's W-stack with the separable uv / w kernels
+ * (deconvolve_visibility_, src/wstack/wstack_common.h:420-470).  A
+ * visibility whose taps leave the stack is predicted as 0 and counted in
+ * rg_context_stats' n_dropped. */
+int rg_wstack_degrid(rg_context *ctx, const rg_wstack_params *p, const double *u, const double *v,
+                     const double *w, int64_t nvis, const rg_sep_kern *conv_uv,
+                     const rg_sep_kern *conv_w, void *out, void *stream);
+
+/* rg_wstack_build then rg_wstack_degrid (wstack_predict_cu_3D). */
+int rg_wstack_predict(rg_context *ctx, const rg_wstack_params *p, const double *l, const double *m,
+                      const double *flux, int32_t npts, const double *u, const double *v,
+                      const double *w, int64_t nvis, const rg_sep_kern *conv_uv,
+                      const rg_sep_kern *conv_w, const rg_sep_kern *corr_lm,
+                      const rg_sep_kern *corr_n, void *out, void *stream);
+
 /* ---- input generation ---- */
 /* Point-source visibilities V = sum_s flux_s exp(-2 pi i (u l_s + v m_s + w (n_s - 1)))
  * (predict.cpp:27-45 convention) evaluated in FP64 on the device.

@@ -574,13 +574,18 @@ int orc_wstack_geometry(double theta, double lam, double dw, int aa_w, double x0
 }
 
 /* Quantised sky position of a point (generate_sky, predict.cpp:236-246). */
-static void wstack_quantise(double l, double m, double theta, double lam, int g, int *lc, int *mc,
-                            double *lq, double *mq)
+static int wstack_quantise(double l, double m, double theta, double lam, int g, int *lc, int *mc,
+                           double *lq, double *mq)
 {
-    *lc = (int)floor((l / theta + 0.5) * (double)g);
-    *mc = (int)floor((m / theta + 0.5) * (double)g);
+    const double flc = floor((l / theta + 0.5) * (double)g);
+    const double fmc = floor((m / theta + 0.5) * (double)g);
+    if (!(flc >= 0 && flc < g && fmc >= 0 && fmc < g))
+        return -1; /* outside the sky (or NaN) */
+    *lc = (int)flc;
+    *mc = (int)fmc;
     *lq = (double)*lc / lam - theta / 2;
     *mq = (double)*mc / lam - theta / 2;
+    return 0;
 }
 
 /* The W-stack (predict.cpp:182-216, 219-278, 438-497): the sky of point
@@ -610,13 +615,16 @@ int orc_wstack_build(orc_c128 *stack, double theta, double lam, const double *pl
     for (int i = 0; i < npts; i++) {
         int lc, mc;
         double lq, mq;
-        wstack_quantise(pl[i], pm[i], theta, lam, g, &lc, &mc, &lq, &mq);
+        if (wstack_quantise(pl[i], pm[i], theta, lam, g, &lc, &mc, &lq, &mq) != 0) {
+            dropped++;
+            continue;
+        }
         const double n = sqrt(1.0 - lq * lq - mq * mq) - 1.0;
         const double lm_step = 1.0 / (double)corr_lm_n, n_step = 1.0 / (double)corr_n_n;
         const int aau = (int)floor((du * lq) / lm_step) + corr_lm_n / 2;
         const int aav = (int)floor((du * mq) / lm_step) + corr_lm_n / 2;
         const int aaw = (int)floor((dw * n) / n_step) + corr_n_n / 2;
-        if (lc < 0 || lc >= g || mc < 0 || mc >= g || aau < 0 || aau >= corr_lm_n || aav < 0 ||
+        if (aau < 0 || aau >= corr_lm_n || aav < 0 ||
             aav >= corr_lm_n || aaw < 0 || aaw >= corr_n_n) {
             dropped++;
             continue;
@@ -748,7 +756,8 @@ void orc_predict_quantized(orc_c128 *out, const double *pl, const double *pm, co
         for (int p = 0; p < npts; p++) {
             int lc, mc;
             double lq, mq;
-            wstack_quantise(pl[p], pm[p], theta, lam, g, &lc, &mc, &lq, &mq);
+            if (wstack_quantise(pl[p], pm[p], theta, lam, g, &lc, &mc, &lq, &mq) != 0)
+                continue; /* outside the sky */
             const double n = sqrt(1.0 - lq * lq - mq * mq) - 1.0;
             const double ang = -TWO_PI * (u[i] * lq + v[i] * mq + w[i] * n);
             const double f = flux ? flux[p] : 1.0;

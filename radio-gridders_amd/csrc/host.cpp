@@ -432,6 +432,84 @@ int rg_gcf_correct(void *image, int32_t type, int32_t n, const double *taper, vo
     return gcf_correct(image, type, n, taper, (hipStream_t)stream);
 }
 
+static int check_sep(const rg_sep_kern *k, const char *what, int support)
+{
+    if (!k || !k->data || k->size <= 0 || k->oversampling <= 0)
+        return fail(RG_ERR_PARAM, "invalid %s kernel", what);
+    if (support > 0 && k->size != support)
+        return fail(RG_ERR_PARAM, "%s kernel size %d differs from its support %d", what, k->size,
+                    support);
+    return RG_OK;
+}
+
+static int check_wstack(const rg_wstack_params *p)
+{
+    if (!p)
+        return fail(RG_ERR_PARAM, "NULL parameters");
+    if (!(p->theta > 0) || !(p->lambda > 0) || !(p->du > 0) || !(p->dw > 0) || !(p->x0 > 0) ||
+        p->aa_support_uv < 2 || p->aa_support_w < 2 || p->aa_support_uv % 2 ||
+        p->aa_support_w % 2 || p->aa_support_uv * p->aa_support_w > 64)
+        return fail(RG_ERR_PARAM, "invalid W-stack parameters (even supports, product <= 64)");
+    return RG_OK;
+}
+
+int rg_wstack_build(rg_context *ctx, const rg_wstack_params *p, const double *l, const double *m,
+                    const double *flux, int32_t npts, const rg_sep_kern *corr_lm,
+                    const rg_sep_kern *corr_n, int32_t *og_out, int32_t *planes_out,
+                    int64_t *skipped_out, void *stream)
+{
+    if (!ctx)
+        return fail(RG_ERR_PARAM, "NULL context");
+    RG_TRY(check_wstack(p));
+    RG_TRY(check_sep(corr_lm, "l/m correction", 0));
+    RG_TRY(check_sep(corr_n, "n correction", 0));
+    if (npts < 0 || (npts > 0 && (!l || !m)))
+        return fail(RG_ERR_PARAM, "invalid sky arrays");
+    RG_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = (hipStream_t)stream;
+    ctx->last_stream = s;
+    RG_TRY(wstack_build(ctx, p, l, m, flux, npts, corr_lm, corr_n, s));
+    if (og_out)
+        *og_out = ctx->ws_og;
+    if (planes_out)
+        *planes_out = ctx->ws_planes;
+    if (skipped_out) {
+        unsigned long long c = 0;
+        RG_HIP(hipMemcpyAsync(&c, ctx->counters.as<unsigned long long>() + CNT_SKY_SKIPPED,
+                              sizeof(c), hipMemcpyDeviceToHost, s));
+        RG_HIP(hipStreamSynchronize(s));
+        *skipped_out = (int64_t)c;
+    }
+    return RG_OK;
+}
+
+int rg_wstack_degrid(rg_context *ctx, const rg_wstack_params *p, const double *u, const double *v,
+                     const double *w, int64_t nvis, const rg_sep_kern *conv_uv,
+                     const rg_sep_kern *conv_w, void *out, void *stream)
+{
+    if (!ctx)
+        return fail(RG_ERR_PARAM, "NULL context");
+    RG_TRY(check_wstack(p));
+    RG_TRY(check_sep(conv_uv, "uv", p->aa_support_uv));
+    RG_TRY(check_sep(conv_w, "w", p->aa_support_w));
+    if (nvis < 0 || (nvis > 0 && (!u || !v || !w || !out)))
+        return fail(RG_ERR_PARAM, "invalid visibility arrays");
+    RG_HIP(hipSetDevice(ctx->device));
+    ctx->last_stream = (hipStream_t)stream;
+    return wstack_degrid(ctx, p, u, v, w, nvis, conv_uv, conv_w, out, (hipStream_t)stream);
+}
+
+int rg_wstack_predict(rg_context *ctx, const rg_wstack_params *p, const double *l, const double *m,
+                      const double *flux, int32_t npts, const double *u, const double *v,
+                      const double *w, int64_t nvis, const rg_sep_kern *conv_uv,
+                      const rg_sep_kern *conv_w, const rg_sep_kern *corr_lm,
+                      const rg_sep_kern *corr_n, void *out, void *stream)
+{
+    RG_TRY(rg_wstack_build(ctx, p, l, m, flux, npts, corr_lm, corr_n, nullptr, nullptr, nullptr,
+                           stream));
+    return rg_wstack_degrid(ctx, p, u, v, w, nvis, conv_uv, conv_w, out, stream);
+}
+
 int rg_dft(rg_context *ctx, const rg_flat_vis *vis, const int32_t *px, const int32_t *py,
            int64_t npix, int32_t grid_size, double lambda, void *out, int32_t mode, void *stream)
 {

@@ -111,8 +111,10 @@ struct rg_context {
     hipStream_t last_stream = nullptr;
     int last_kind = 0; // 0 none, 1 wproj, 2 wtowers
     int64_t last_nvis = 0;
+    // geometry of the W-stack held in scratch[17] (0 = none)
+    int ws_g = 0, ws_og = 0, ws_planes = 0;
     // generic scratch arrays
-    rg::DevBuf scratch[17];
+    rg::DevBuf scratch[18];
     // channel offsets resident in scratch[13] (uploaded only when they change,
     // so a repeated multi-channel call issues no copy and can be captured)
     std::vector<int64_t> chan_off;
@@ -133,6 +135,7 @@ enum Counter {
     CNT_TOWERS = 4,
     CNT_RECORDS = 5, // records written by the sort (one per key a visibility reaches)
     CNT_MAX_ITEMS = 6,
+    CNT_SKY_SKIPPED = 7, // W-stack predict: sky points outside the image or tables
     CNT__N = 8
 };
 
@@ -166,5 +169,13 @@ int weight_vis(rg_context *ctx, const double *u, const double *v, int64_t n, voi
                int vis_type, int N, double theta, uint32_t *counts, hipStream_t s);
 int make_hermitian(void *grid, int type, int n, hipStream_t s);
 int gcf_correct(void *img, int type, int n, const double *taper, hipStream_t s);
+int wstack_geometry(double theta, double lambda, double dw, int aa_w, double x0, int &g, int &og,
+                    int &w_planes);
+int wstack_build(rg_context *ctx, const rg_wstack_params *p, const double *l, const double *m,
+                 const double *flux, int32_t npts, const rg_sep_kern *corr_lm,
+                 const rg_sep_kern *corr_n, hipStream_t s);
+int wstack_degrid(rg_context *ctx, const rg_wstack_params *p, const double *u, const double *v,
+                  const double *w, int64_t nvis, const rg_sep_kern *conv_uv,
+                  const rg_sep_kern *conv_w, void *out, hipStream_t s);
 
 } // namespace rg

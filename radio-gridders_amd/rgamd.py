@@ -61,6 +61,16 @@ class GridParams(C.Structure):
                 ("floor_batch", C.c_int32)]
 
 
+class SepKernDesc(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("size", C.c_int32), ("oversampling", C.c_int32)]
+
+
+class WStackParams(C.Structure):
+    _fields_ = [("theta", C.c_double), ("lambda_", C.c_double), ("du", C.c_double),
+                ("dw", C.c_double), ("x0", C.c_double), ("aa_support_uv", C.c_int32),
+                ("aa_support_w", C.c_int32)]
+
+
 class GridStats(C.Structure):
     _fields_ = [("n_vis", C.c_int64), ("n_gridded", C.c_int64), ("n_dropped", C.c_int64),
                 ("n_bad_plane", C.c_int64), ("n_items", C.c_int64), ("n_tasks", C.c_int64),
@@ -93,6 +103,14 @@ SIGNATURES = [
     ("rg_dft", C.c_int, [_P, C.POINTER(FlatVis), _P, _P, _L, _I, _D, _P, _I, _P]),
     ("rg_dft_image", C.c_int, [_P, C.POINTER(FlatVis), _I, _D, _P, _I, _I, _I, _P]),
     ("rg_predict_points", C.c_int, [_P, _P, _P, _L, _P, _P, _P, _I, _P, _I, _P]),
+    ("rg_wstack_build", C.c_int, [_P, C.POINTER(WStackParams), _P, _P, _P, _I,
+                                  C.POINTER(SepKernDesc), C.POINTER(SepKernDesc), C.POINTER(_I),
+                                  C.POINTER(_I), C.POINTER(C.c_int64), _P]),
+    ("rg_wstack_degrid", C.c_int, [_P, C.POINTER(WStackParams), _P, _P, _P, _L,
+                                   C.POINTER(SepKernDesc), C.POINTER(SepKernDesc), _P, _P]),
+    ("rg_wstack_predict", C.c_int, [_P, C.POINTER(WStackParams), _P, _P, _P, _I, _P, _P, _P, _L,
+                                    C.POINTER(SepKernDesc), C.POINTER(SepKernDesc),
+                                    C.POINTER(SepKernDesc), C.POINTER(SepKernDesc), _P, _P]),
     ("rg_pswf", C.c_double, [_D, _D]),
     ("rg_pswf_eval", C.c_int, [_D, _P, _L, _P]),
     ("rg_wkern_generate", C.c_int, [_P, _I, _I, _I, _I, _D, _D, _D, _I, _D, _I]),
@@ -247,6 +265,27 @@ class Context:
         check(lib().rg_weight(self.h, _ptr(u), _ptr(v), u.numel(), _ptr(vis), grid_type_of(vis),
                               int(grid_size), float(theta), _ptr(counts), _stream(stream)))
 
+    def wstack_build(self, p: WStackParams, l, m, flux, corr_lm: SepKernDesc, corr_n: SepKernDesc,
+                     stream=None):
+        """Builds the context's W-stack; returns (og, w_planes, skipped points)."""
+        og, wp, sk = C.c_int32(), C.c_int32(), C.c_int64()
+        check(lib().rg_wstack_build(self.h, C.byref(p), _ptr(l), _ptr(m), _ptr(flux), l.numel(),
+                                    C.byref(corr_lm), C.byref(corr_n), C.byref(og), C.byref(wp),
+                                    C.byref(sk), _stream(stream)))
+        return og.value, wp.value, sk.value
+
+    def wstack_degrid(self, p: WStackParams, u, v, w, conv_uv: SepKernDesc, conv_w: SepKernDesc,
+                      out, stream=None):
+        check(lib().rg_wstack_degrid(self.h, C.byref(p), _ptr(u), _ptr(v), _ptr(w), u.numel(),
+                                     C.byref(conv_uv), C.byref(conv_w), _ptr(out), _stream(stream)))
+
+    def wstack_predict(self, p: WStackParams, l, m, flux, u, v, w, conv_uv, conv_w, corr_lm,
+                       corr_n, out, stream=None):
+        check(lib().rg_wstack_predict(self.h, C.byref(p), _ptr(l), _ptr(m), _ptr(flux), l.numel(),
+                                      _ptr(u), _ptr(v), _ptr(w), u.numel(), C.byref(conv_uv),
+                                      C.byref(conv_w), C.byref(corr_lm), C.byref(corr_n),
+                                      _ptr(out), _stream(stream)))
+
     def dft(self, fv: FlatVis, px, py, grid_size: int, lam: float, out, mode=RG_DFT_FP64,
             stream=None):
         check(lib().rg_dft(self.h, C.byref(fv), _ptr(px), _ptr(py), px.numel(), int(grid_size),
@@ -353,6 +392,16 @@ class SepKern:
     du: float
     dw: float
     x0: float
+
+
+def sep_kern_desc(data, size: int, oversampling: int) -> SepKernDesc:
+    """Descriptor of a device float64 kernel table."""
+    return SepKernDesc(_ptr(data), int(size), int(oversampling))
+
+
+def wstack_params(theta, lam, du, dw, x0, aa_uv, aa_w) -> WStackParams:
+    return WStackParams(float(theta), float(lam), float(du), float(dw), float(x0), int(aa_uv),
+                        int(aa_w))
 
 
 def load_sep_kern(path: str, transposed: bool = False) -> SepKern:
