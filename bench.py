@@ -58,7 +58,10 @@ def main():
     ap.add_argument("--dumps", type=int, default=64)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true", help="skip the FP64 plain runs")
-    ap.add_argument("--only", choices=["all", "wproj", "wtowers", "channels"], default="all")
+    ap.add_argument("--only", choices=["all", "wproj", "wtowers", "channels", "wstack"],
+                    default="all")
+    ap.add_argument("--skip-wstack", action="store_true",
+                    help="skip the W-stacking predict workload")
     ap.add_argument("--skip-channels", action="store_true",
                     help="skip the 256-channel batched W-projection workload")
     ap.add_argument("--c5", action="store_true",
@@ -209,6 +212,13 @@ def main():
             log(f"[bench] channels workload failed: {e!r}")
             res["channels"] = {"error": repr(e)}
 
+    if args.only in ("all", "wstack") and not args.skip_wstack:
+        try:
+            res["wstack"] = bench_wstack(ctx, K, W, args.skip_cpu, args.cpu_seconds)
+        except Exception as e:  # reported, never hides the headline
+            log(f"[bench] W-stack predict failed: {e!r}")
+            res["wstack"] = {"error": repr(e)}
+
     if args.c5:
         try:
             res["stress"] = bench_stress(ctx, K, W)
@@ -269,6 +279,114 @@ def main():
         "results": res,
     }
     print(json.dumps(out), flush=True)
+
+
+def bench_wstack(ctx, steps, warmup, skip_cpu, cpu_seconds):
+    """W-stacking predict at the reference test's workload (test/test_wstack.cpp:
+    theta 0.1, lambda 20480, test-card sky, 40 M random visibilities with u, v
+    in [-theta lambda/2, theta lambda/2) and w in [-dw, dw), the Sze-Tan
+    kernels of test/kernels/sze): stack build (sources, 4096^2 FFTs of every
+    plane) and degridding timed apart; degridding Mvis/s against the oracle's
+    OpenMP degridding of the same stack on a sample."""
+    import torch
+
+    import rgamd
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import pyoracle as orc  # the checker: accuracy sample and CPU baseline only
+
+    sze = os.path.join(REPO, "tests", "golden", "sze")
+    ku = rgamd.load_sep_kern(os.path.join(sze, "sepkern_uv_transpose.hdf5"), True)
+    kw = rgamd.load_sep_kern(os.path.join(sze, "sepkern_w_transpose.hdf5"), True)
+    klm = rgamd.load_sep_kern(os.path.join(sze, "sepkern_lm.hdf5"), False)
+    kn = rgamd.load_sep_kern(os.path.join(sze, "sepkern_n.hdf5"), False)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    dk = {n: dev(k.data) for n, k in dict(uv=ku, w=kw, lm=klm, n=kn).items()}
+    desc = {n: rgamd.sep_kern_desc(dk[n], k.size, k.oversampling)
+            for n, k in dict(uv=ku, w=kw, lm=klm, n=kn).items()}
+    theta, lam = 0.1, 20480.0
+    p = rgamd.wstack_params(theta, lam, ku.du, kw.dw, klm.x0, ku.size, kw.size)
+    card = np.array([0.95, 0.95, -0.95, -0.95, 0.95, -0.95, -0.95, 0.95, 0.0, 0.5, 0.0, -0.5, 0.5,
+                     0.0, -0.5, 0.0]) * (theta / 2)
+    l, m = card[0::2].copy(), card[1::2].copy()
+    dl, dm = dev(l), dev(m)
+    nv = 40_000_000
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(7)
+    u = (torch.rand(nv, dtype=torch.float64, device="cuda", generator=gen) - 0.5) * (theta * lam)
+    v = (torch.rand(nv, dtype=torch.float64, device="cuda", generator=gen) - 0.5) * (theta * lam)
+    w = (torch.rand(nv, dtype=torch.float64, device="cuda", generator=gen) * 2 - 1) * kw.dw
+    out = torch.empty(nv, dtype=torch.complex128, device="cuda")
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    og, wp, skipped = ctx.wstack_build(p, dl, dm, None, desc["lm"], desc["n"])
+    for _ in range(warmup):
+        ctx.wstack_build(p, dl, dm, None, desc["lm"], desc["n"])
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        ctx.wstack_build(p, dl, dm, None, desc["lm"], desc["n"])
+    e1.record()
+    torch.cuda.synchronize()
+    build_ms = e0.elapsed_time(e1) / steps
+    for _ in range(warmup):
+        ctx.wstack_degrid(p, u, v, w, desc["uv"], desc["w"], out)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(steps):
+        ctx.wstack_degrid(p, u, v, w, desc["uv"], desc["w"], out)
+    e1.record()
+    torch.cuda.synchronize()
+    degrid_ms = e0.elapsed_time(e1) / steps
+    st = ctx.stats()
+    # accuracy against the direct sum over the quantised sky on a sample
+    sel = torch.randperm(nv, device="cuda", generator=gen)[:20000]
+    us, vs, ws = u[sel].cpu().numpy(), v[sel].cpu().numpy(), w[sel].cpu().numpy()
+    ref = orc.predict_quantized(l, m, None, theta, lam, us, vs, ws)
+    err = np.abs(out[sel].cpu().numpy() - ref)
+    taps = kw.size * ku.size * ku.size
+    per_vis = taps * 16 + 24 + 16
+    ach = nv * per_vis / (degrid_ms * 1e-3) / 1e9
+    r = {"n_vis": nv, "planes": wp, "plane": og, "sources": len(l), "skipped_sources": skipped,
+         "not_predicted": int(st.n_dropped), "build_ms": build_ms, "degrid_ms": degrid_ms,
+         "degrid_mvis_s": nv / (degrid_ms * 1e-3) / 1e6,
+         "total_mvis_s": nv / ((build_ms + degrid_ms) * 1e-3) / 1e6,
+         "error_vs_quantised_dft": {"max_abs": float(err.max()), "mean_abs": float(err.mean()),
+                                    "per_source_mean": float(err.mean() / len(l))},
+         "roofline": {"kernel": "k_ws_degrid", "bound": "hbm", "achieved": ach,
+                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                      "bytes_per_unit": {"visibility": per_vis}, "traffic": None,
+                      "note": "taps served from cache count as algorithmic bytes"}}
+    log(f"[bench] W-stack predict: {og}^2 x {wp} planes, build {build_ms:.2f} ms, degrid "
+        f"{degrid_ms:.2f} ms ({r['degrid_mvis_s']:.0f} Mvis/s), mean error per source "
+        f"{r['error_vs_quantised_dft']['per_source_mean']:.2e}")
+    if not skip_cpu:
+        # the oracle's OpenMP degridding of the same stack on a sample
+        stack = ctx.wstack_stack(og, wp)
+        threads = min(16, os.cpu_count() or 1)
+        ns = 200_000
+        su, sv, sw = u[:ns].cpu().numpy(), v[:ns].cpu().numpy(), w[:ns].cpu().numpy()
+        t = time.perf_counter()
+        orc.wstack_degrid(stack, int(np.floor(theta * lam)), su, sv, sw, ku.du, kw.dw, ku.data,
+                          ku.size, ku.oversampling, kw.data, kw.size, kw.oversampling,
+                          nthreads=threads, fast=True)
+        dt = time.perf_counter() - t
+        reps = max(1, int(cpu_seconds / max(dt, 1e-3)))
+        reps = min(reps, 40)
+        t = time.perf_counter()
+        for _ in range(reps):
+            orc.wstack_degrid(stack, int(np.floor(theta * lam)), su, sv, sw, ku.du, kw.dw, ku.data,
+                              ku.size, ku.oversampling, kw.data, kw.size, kw.oversampling,
+                              nthreads=threads, fast=True)
+        dt = (time.perf_counter() - t) / reps
+        r["cpu_baseline"] = {"value": ns / dt / 1e6, "unit": "Mvis/s", "cores": threads,
+                             "kind": "port",
+                             "sample": f"{ns} of the visibilities, degridding only, "
+                                       f"{reps} repetitions"}
+        log(f"[bench] CPU W-stack degrid: {r['cpu_baseline']['value']:.1f} Mvis/s on {threads} "
+            f"threads")
+        del stack
+    del u, v, w, out
+    torch.cuda.empty_cache()
+    return r
 
 
 def bench_channels(ctx, steps, warmup):

@@ -248,6 +248,12 @@ int rg_wstack_degrid(rg_context *ctx, const rg_wstack_params *p, const double *u
                      const double *w, int64_t nvis, const rg_sep_kern *conv_uv,
                      const rg_sep_kern *conv_w, void *out, void *stream);
 
+/* Copies the context's W-stack ([planes][og][og] RG_C128, row v, column u)
+ * into dst (host or device memory of capacity_bytes) and waits for the copy.
+ * No counterpart in the reference (its stack is internal); for inspection
+ * and tests. */
+int rg_wstack_copy(rg_context *ctx, void *dst, int64_t capacity_bytes, void *stream);
+
 /* rg_wstack_build then rg_wstack_degrid (wstack_predict_cu_3D). */
 int rg_wstack_predict(rg_context *ctx, const rg_wstack_params *p, const double *l, const double *m,
                       const double *flux, int32_t npts, const double *u, const double *v,

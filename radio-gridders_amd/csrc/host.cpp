@@ -499,6 +499,23 @@ int rg_wstack_degrid(rg_context *ctx, const rg_wstack_params *p, const double *u
     return wstack_degrid(ctx, p, u, v, w, nvis, conv_uv, conv_w, out, (hipStream_t)stream);
 }
 
+int rg_wstack_copy(rg_context *ctx, void *dst, int64_t capacity_bytes, void *stream)
+{
+    if (!ctx || !dst)
+        return fail(RG_ERR_PARAM, "NULL argument");
+    if (ctx->ws_og == 0)
+        return fail(RG_ERR_PARAM, "no W-stack has been built in the context");
+    const int64_t bytes = (int64_t)sizeof(double2) * ctx->ws_og * ctx->ws_og * ctx->ws_planes;
+    if (capacity_bytes < bytes)
+        return fail(RG_ERR_PARAM, "destination holds %lld bytes, the stack %lld",
+                    (long long)capacity_bytes, (long long)bytes);
+    RG_HIP(hipSetDevice(ctx->device));
+    hipStream_t s = (hipStream_t)stream;
+    RG_HIP(hipMemcpyAsync(dst, ctx->scratch[17].p, (size_t)bytes, hipMemcpyDefault, s));
+    RG_HIP(hipStreamSynchronize(s));
+    return RG_OK;
+}
+
 int rg_wstack_predict(rg_context *ctx, const rg_wstack_params *p, const double *l, const double *m,
                       const double *flux, int32_t npts, const double *u, const double *v,
                       const double *w, int64_t nvis, const rg_sep_kern *conv_uv,

@@ -108,6 +108,7 @@ SIGNATURES = [
                                   C.POINTER(_I), C.POINTER(C.c_int64), _P]),
     ("rg_wstack_degrid", C.c_int, [_P, C.POINTER(WStackParams), _P, _P, _P, _L,
                                    C.POINTER(SepKernDesc), C.POINTER(SepKernDesc), _P, _P]),
+    ("rg_wstack_copy", C.c_int, [_P, _P, _L, _P]),
     ("rg_wstack_predict", C.c_int, [_P, C.POINTER(WStackParams), _P, _P, _P, _I, _P, _P, _P, _L,
                                     C.POINTER(SepKernDesc), C.POINTER(SepKernDesc),
                                     C.POINTER(SepKernDesc), C.POINTER(SepKernDesc), _P, _P]),
@@ -278,6 +279,12 @@ class Context:
                       out, stream=None):
         check(lib().rg_wstack_degrid(self.h, C.byref(p), _ptr(u), _ptr(v), _ptr(w), u.numel(),
                                      C.byref(conv_uv), C.byref(conv_w), _ptr(out), _stream(stream)))
+
+    def wstack_stack(self, og: int, planes: int, stream=None) -> np.ndarray:
+        """The context's W-stack as a host array [planes][og][og] complex128."""
+        out = np.empty((planes, og, og), np.complex128)
+        check(lib().rg_wstack_copy(self.h, out.ctypes.data, out.nbytes, _stream(stream)))
+        return out
 
     def wstack_predict(self, p: WStackParams, l, m, flux, u, v, w, conv_uv, conv_w, corr_lm,
                        corr_n, out, stream=None):

@@ -72,6 +72,8 @@ def test_predict_matches_oracle(ctx, kernels, lam, sky):
     og, wp, sk = ctx.wstack_build(p, dev(l), dev(m), None if flux is None else dev(flux), d["lm"],
                                   d["n"])
     assert (og, wp, sk) == (stack.shape[1], stack.shape[0], 0)
+    gs = ctx.wstack_stack(og, wp)
+    assert np.abs(gs - stack).max() < 1e-11 * np.abs(stack).max()
     ctx.wstack_degrid(p, dev(u), dev(v), dev(w), d["uv"], d["w"], out)
     torch.cuda.synchronize()
     got = out.cpu().numpy()
