@@ -447,9 +447,11 @@ static int check_wstack(const rg_wstack_params *p)
     if (!p)
         return fail(RG_ERR_PARAM, "NULL parameters");
     if (!(p->theta > 0) || !(p->lambda > 0) || !(p->du > 0) || !(p->dw > 0) || !(p->x0 > 0) ||
-        p->aa_support_uv < 2 || p->aa_support_w < 2 || p->aa_support_uv % 2 ||
-        p->aa_support_w % 2 || p->aa_support_uv * p->aa_support_w > 64)
-        return fail(RG_ERR_PARAM, "invalid W-stack parameters (even supports, product <= 64)");
+        p->aa_support_uv < 2 || p->aa_support_uv > 64 ||
+        (p->aa_support_uv & (p->aa_support_uv - 1)) != 0 || p->aa_support_w < 2 ||
+        p->aa_support_w > 64 || p->aa_support_w % 2)
+        return fail(RG_ERR_PARAM, "invalid W-stack parameters (uv support a power of two in "
+                                  "2..64, even w support <= 64)");
     return RG_OK;
 }
 

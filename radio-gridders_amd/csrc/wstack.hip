@@ -10,9 +10,9 @@
 // their quantised pixels, so each plane's input is a zero plane plus one
 // value per source; one batched FFT then transforms every plane.
 //
-// Degridding: a group of L lanes per visibility, one lane per (w, v) kernel
-// row (aa_w * aa_uv <= L rows); each lane sums its aa_uv taps along u (one
-// contiguous run of the stack row) and the group reduces with shuffles.
+// Degridding: one wavefront per visibility, lanes spread over the kernel's
+// taps so that each load instruction reads whole stack rows; the wavefront
+// reduces with shuffles.
 #include "binning.hpp"
 
 namespace rg {
@@ -95,12 +95,15 @@ __global__ __launch_bounds__(256) void k_ws_sources(
 }
 
 // ---- degridding ------------------------------------------------------------------------------
-// deconvolve_visibility_ (wstack_common.h:420-470).  Group of L lanes per
-// visibility; lane r < aa_w * aa_uv takes kernel row (dwi, dvi) =
-// (r / aa_uv - aa_w / 2, r % aa_uv - aa_uv / 2) and sums
-// stack[dws][dvs][dus] (cw cv) cu over dui.  A visibility whose taps leave
-// the stack is predicted as 0 and counted.
-template <int L>
+// deconvolve_visibility_ (wstack_common.h:420-470).  One wavefront per
+// visibility (grid-stride over visibilities).  Lane = (pair, tap): tap
+// t = lane % aa_uv along u, pair q = lane / aa_uv + k (64 / aa_uv) over the
+// aa_w * aa_uv (plane, row) pairs, so one load instruction reads 64 / aa_uv
+// whole kernel rows (contiguous runs of the stack).  Each lane sums
+// stack[dws][dvs][dus] (cw cv) cu over its pairs; the wavefront reduces with
+// shuffles.  A visibility whose taps leave the stack is predicted as 0 and
+// counted.
+template <int LV>
 __global__ __launch_bounds__(256) void k_ws_degrid(
     const double2 *__restrict__ stack, int og, int w_planes, int g, const double *__restrict__ u,
     const double *__restrict__ v, const double *__restrict__ w, int64_t nvis, double du, double dw,
@@ -108,44 +111,54 @@ __global__ __launch_bounds__(256) void k_ws_degrid(
     const double *__restrict__ conv_w, int ov_w, double2 *out, unsigned long long *dropped)
 {
 #pragma clang fp contract(off)
-    const int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / L;
-    const int r = (int)(threadIdx.x % L);
-    const bool live = i < nvis;
-    const double ui = live ? u[i] : 0.0, vi = live ? v[i] : 0.0, wi = live ? w[i] : 0.0;
-    const double cu = ceil(ui / du), cv = ceil(vi / du), cw = ceil(wi / dw);
+    // LV lanes per visibility (64 / LV visibilities per wavefront)
+    const int lane = (int)(threadIdx.x % LV);
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / LV;
+    const int64_t waves = ((int64_t)gridDim.x * blockDim.x) / LV;
     const int aa_h = aa_uv / 2, aaw_h = aa_w / 2;
-    const bool finite = cu > -1e9 && cu < 1e9 && cv > -1e9 && cv < 1e9 && cw > -1e9 && cw < 1e9;
-    const int us = finite ? (int)cu + g : 0, vs = finite ? (int)cv + g : 0;
-    const int ws = finite ? (int)cw + w_planes / 2 : 0;
-    const bool inside = finite && us - aa_h >= 0 && us + aa_h <= og && vs - aa_h >= 0 &&
-                        vs + aa_h <= og && ws - aaw_h >= 0 && ws + aaw_h <= w_planes;
-    double re = 0.0, im = 0.0;
-    if (live && inside && r < aa_w * aa_uv) {
-        const double flu = ui - cu * du, flv = vi - cv * du, flw = wi - cw * dw;
-        const int ovu = min((int)floor(fabs(flu) / du * ov_uv), ov_uv - 1);
-        const int ovv = min((int)floor(fabs(flv) / du * ov_uv), ov_uv - 1);
-        const int ovw = min((int)floor(fabs(flw) / dw * ov_w), ov_w - 1);
-        const int dwi = r / aa_uv - aaw_h, dvi = r % aa_uv - aa_h;
-        const double gw = conv_w[aa_w * ovw + (dwi + aaw_h)];
-        const double gv = gw * conv_uv[aa_uv * ovv + (dvi + aa_h)];
-        const double2 *row = stack + ((size_t)(ws + dwi) * og + (vs + dvi)) * og + (us - aa_h);
-        const double *cuk = conv_uv + aa_uv * ovu;
-        for (int t = 0; t < aa_uv; ++t) {
-            const double gu = gv * cuk[t];
-            const double2 s = row[t];
-            re += s.x * gu;
-            im += s.y * gu;
+    const int t = lane % aa_uv, q0 = lane / aa_uv, qs = LV / aa_uv, npairs = aa_w * aa_uv;
+    // every lane of a wavefront runs the same number of iterations
+    const int64_t iters = (nvis + waves - 1) / waves;
+    for (int64_t it = 0; it < iters; ++it) {
+        const int64_t i0 = wave + it * waves;
+        const bool live = i0 < nvis;
+        const int64_t i = live ? i0 : 0;
+        const double ui = u[i], vi = v[i], wi = w[i];
+        const double cu = ceil(ui / du), cv = ceil(vi / du), cw = ceil(wi / dw);
+        const bool finite =
+            cu > -1e9 && cu < 1e9 && cv > -1e9 && cv < 1e9 && cw > -1e9 && cw < 1e9;
+        const int us = finite ? (int)cu + g : 0, vs = finite ? (int)cv + g : 0;
+        const int ws = finite ? (int)cw + w_planes / 2 : 0;
+        const bool inside = finite && us - aa_h >= 0 && us + aa_h <= og && vs - aa_h >= 0 &&
+                            vs + aa_h <= og && ws - aaw_h >= 0 && ws + aaw_h <= w_planes;
+        double re = 0.0, im = 0.0;
+        if (live && inside && t < aa_uv && q0 < qs) {
+            const double flu = ui - cu * du, flv = vi - cv * du, flw = wi - cw * dw;
+            const int ovu = min((int)floor(fabs(flu) / du * ov_uv), ov_uv - 1);
+            const int ovv = min((int)floor(fabs(flv) / du * ov_uv), ov_uv - 1);
+            const int ovw = min((int)floor(fabs(flw) / dw * ov_w), ov_w - 1);
+            const double gu = conv_uv[aa_uv * ovu + t];
+            const double2 *col = stack + (us - aa_h + t);
+            for (int q = q0; q < npairs; q += qs) {
+                const int pw = q / aa_uv, pv = q - pw * aa_uv; // dwi + aaw_h, dvi + aa_h
+                const double gw = conv_w[aa_w * ovw + pw];
+                const double gv = gw * conv_uv[aa_uv * ovv + pv];
+                const double2 s = col[((size_t)(ws - aaw_h + pw) * og + (vs - aa_h + pv)) * og];
+                const double k = gv * gu;
+                re += s.x * k;
+                im += s.y * k;
+            }
         }
-    }
 #pragma unroll
-    for (int o = L / 2; o > 0; o >>= 1) {
-        re += __shfl_xor(re, o, L);
-        im += __shfl_xor(im, o, L);
-    }
-    if (live && r == 0) {
-        out[i] = make_double2(inside ? re : 0.0, inside ? im : 0.0);
-        if (!inside)
-            atomicAdd(dropped, 1ull);
+        for (int o = LV / 2; o > 0; o >>= 1) {
+            re += __shfl_xor(re, o, LV);
+            im += __shfl_xor(im, o, LV);
+        }
+        if (live && lane == 0) {
+            out[i] = make_double2(inside ? re : 0.0, inside ? im : 0.0);
+            if (!inside)
+                atomicAdd(dropped, 1ull);
+        }
     }
 }
 
@@ -188,7 +201,6 @@ int wstack_degrid(rg_context *ctx, const rg_wstack_params *p, const double *u, c
     RG_TRY(wstack_geometry(p->theta, p->lambda, p->dw, p->aa_support_w, p->x0, g, og, wp));
     if (ctx->ws_og != og || ctx->ws_planes != wp || ctx->ws_g != g)
         return fail(RG_ERR_PARAM, "no W-stack of this geometry has been built in the context");
-    const int rows = p->aa_support_w * p->aa_support_uv;
     auto *cnt = ctx->counters.as<unsigned long long>();
     // per call: visibilities not predicted (the sky counter is kept)
     RG_TRY(zero_fill(cnt + CNT_DROPPED, sizeof(unsigned long long) * 2, s));
@@ -196,17 +208,21 @@ int wstack_degrid(rg_context *ctx, const rg_wstack_params *p, const double *u, c
     if (nvis == 0)
         return RG_OK;
     const double2 *stack = ctx->scratch[17].as<double2>();
+    // LV lanes per visibility: at least one whole kernel row per group
+    constexpr int LV_DEFAULT = 64;
+    const int lv = std::max(LV_DEFAULT, (int)p->aa_support_uv);
     auto launch = [&](auto lanes) {
-        constexpr int L = decltype(lanes)::value;
-        const int64_t threads = nvis * L;
-        hipLaunchKernelGGL(k_ws_degrid<L>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
-                           s, stack, og, wp, g, u, v, w, nvis, p->du, p->dw, p->aa_support_uv,
-                           p->aa_support_w, conv_uv->data, conv_uv->oversampling, conv_w->data,
+        constexpr int LV = decltype(lanes)::value;
+        const int per_block = 256 / LV;
+        const int64_t blocks = std::min<int64_t>((nvis + per_block - 1) / per_block, (int64_t)1 << 20);
+        hipLaunchKernelGGL(k_ws_degrid<LV>, dim3((unsigned)blocks), dim3(256), 0, s, stack, og, wp,
+                           g, u, v, w, nvis, p->du, p->dw, p->aa_support_uv, p->aa_support_w,
+                           conv_uv->data, conv_uv->oversampling, conv_w->data,
                            conv_w->oversampling, (double2 *)out, cnt + CNT_DROPPED);
     };
-    if (rows <= 16)
+    if (lv <= 16)
         launch(std::integral_constant<int, 16>());
-    else if (rows <= 32)
+    else if (lv <= 32)
         launch(std::integral_constant<int, 32>());
     else
         launch(std::integral_constant<int, 64>());
