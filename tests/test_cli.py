@@ -1,4 +1,4 @@
-"""grid.out / dft.out argument handling (no GPU needed for these paths)."""
+"""grid.out / dft.out / wstack.out argument handling (no GPU needed for these paths)."""
 import os
 import subprocess
 
@@ -22,3 +22,10 @@ def test_help_and_missing_arguments():
     assert r.returncode == 1 and "blocks" in r.stdout
     r = run("dft.out", "-HELP")
     assert r.returncode == 0 and "-threadblock" in r.stdout
+    r = run("wstack.out", "-help")
+    assert r.returncode == 0 and "-sepkern_uv" in r.stdout
+    r = run("wstack.out", "-theta=0.1", "-lambda=20480")
+    assert r.returncode == 1 and "sepkern_uv" in r.stdout
+    r = run("wstack.out", "-theta=0.1", "-lambda=20480", "-sepkern_uv=a", "-sepkern_w=b",
+            "-sepkern_lm=c", "-sepkern_n=d", "-mode=0")
+    assert r.returncode == 2

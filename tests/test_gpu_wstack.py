@@ -3,6 +3,8 @@ grid, FP64 to 1e-10) and, at the reference test's full geometry (theta 0.1,
 lambda 20480: 4096^2 x 8 planes), against the direct sum over the quantised
 sky within the method's error."""
 import os
+import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -136,3 +138,22 @@ def test_invalid_parameters_are_rejected(ctx, kernels):
     p = params(kernels, 0.1, 600.0)  # og = 120 is no power of two
     with pytest.raises(rgamd.RGError):
         ctx.wstack_build(p, e, e, None, d["lm"], d["n"])
+
+
+def test_cli_reports_the_aggregate_error():
+    """wstack.out (src/wstack/wstack.cpp) in predict mode with the reference's
+    kernel files: random visibilities (seeded) and a fixed (pu, pv, pw)."""
+    exe = os.path.join(os.path.dirname(SZE), "..", "..", "radio-gridders_amd", "wstack.out")
+    base = [exe, "-theta=0.1", "-lambda=20480", "-mode=0", "-cuda=1",
+            f"-sepkern_uv={SZE}/sepkern_uv_transpose.hdf5", f"-sepkern_w={SZE}/sepkern_w_transpose.hdf5",
+            f"-sepkern_lm={SZE}/sepkern_lm.hdf5", f"-sepkern_n={SZE}/sepkern_n.hdf5"]
+    r = subprocess.run(base + ["-npts=200000", "-random_points=1", "-seed=4"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    per = float(re.search(r"Aggregate Error / Npts: (\S+)", r.stdout).group(1))
+    assert 0 < per < 8 * 4e-4, r.stdout
+    r = subprocess.run(base + ["-npts=16", "-pu=100.3", "-pv=-250.7", "-pw=12.5"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    per = float(re.search(r"Aggregate Error / Npts: (\S+)", r.stdout).group(1))
+    assert per < 8 * 1e-3, r.stdout
