@@ -58,8 +58,9 @@ def main():
     ap.add_argument("--dumps", type=int, default=64)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true", help="skip the FP64 plain runs")
-    ap.add_argument("--only", choices=["all", "wproj", "wtowers", "channels", "wstack"],
+    ap.add_argument("--only", choices=["all", "wproj", "wtowers", "channels", "wstack", "epic"],
                     default="all")
+    ap.add_argument("--skip-epic", action="store_true", help="skip the EPIC gridding workload")
     ap.add_argument("--skip-wstack", action="store_true",
                     help="skip the W-stacking predict workload")
     ap.add_argument("--skip-channels", action="store_true",
@@ -218,6 +219,13 @@ def main():
         except Exception as e:  # reported, never hides the headline
             log(f"[bench] W-stack predict failed: {e!r}")
             res["wstack"] = {"error": repr(e)}
+
+    if args.only in ("all", "epic") and not args.skip_epic:
+        try:
+            res["epic"] = bench_epic(ctx, K, W, args.skip_cpu)
+        except Exception as e:  # reported, never hides the headline
+            log(f"[bench] EPIC gridding failed: {e!r}")
+            res["epic"] = {"error": repr(e)}
 
     if args.c5:
         try:
@@ -386,6 +394,56 @@ def bench_wstack(ctx, steps, warmup, skip_cpu, cpu_seconds):
         del stack
     del u, v, w, out
     torch.cuda.empty_cache()
+    return r
+
+
+def bench_epic(ctx, steps, warmup, skip_cpu):
+    """EPIC E-field gridding at the reference's workload (src/epic/epic_grid.cu:
+    1000 timestamps x 256 antennas, 64^2 grids, 2x2 top-hat illumination,
+    positions in [2, 62]); one launch grids every timestamp."""
+    import torch
+
+    import rgamd
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import pyoracle as orc  # the checker: accuracy and CPU baseline only
+
+    g, S, nb, na = 64, 2, 1000, 256
+    rng = np.random.default_rng(9)
+    f = (rng.normal(0, 5, (nb, na)) + 1j * rng.normal(0, 5, (nb, na))).astype(np.complex64)
+    x = rng.integers(S, g - S + 1, nb * na).astype(np.int32)
+    y = rng.integers(S, g - S + 1, nb * na).astype(np.int32)
+    il = np.ones((S, S), np.complex64)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    df, dx, dy, dil = dev(f), dev(x), dev(y), dev(il)
+    grids = torch.empty((nb, g, g), dtype=torch.complex64, device="cuda")
+    for _ in range(max(warmup, 1)):
+        ctx.epic_grid(df, dx, dy, dil, grids)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = max(steps, 20)
+    e0.record()
+    for _ in range(reps):
+        ctx.epic_grid(df, dx, dy, dil, grids)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    t = time.perf_counter()
+    ref = orc.epic_grid(f.astype(np.complex128), x, y, il.astype(np.complex128), g)
+    cpu_s = time.perf_counter() - t
+    got = grids.cpu().numpy()
+    per_ts = g * g * 8 + na * (8 + 4 + 4)
+    ach = nb * per_ts / (ms * 1e-3) / 1e9
+    r = {"timestamps": nb, "antennas": na, "grid": g, "support": S, "ms": ms,
+         "timestamps_per_s": nb / (ms * 1e-3), "samples_per_s": nb * na / (ms * 1e-3),
+         "max_rel_err_vs_oracle": float(np.abs(got - ref).max() / np.abs(ref).max()),
+         "roofline": {"kernel": "k_epic_grid", "bound": "hbm", "achieved": ach,
+                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                      "bytes_per_unit": {"timestamp": per_ts}}}
+    if not skip_cpu:
+        r["cpu_baseline"] = {"value": nb / cpu_s, "unit": "timestamps/s", "cores": 1,
+                             "kind": "port", "sample": "the whole workload, FP64 naive scatter"}
+    log(f"[bench] EPIC: {nb} timestamps x {na} antennas on {g}^2: {ms * 1e3:.1f} us "
+        f"({r['timestamps_per_s'] / 1e6:.2f} M timestamps/s, {ach:.0f} GB/s)")
     return r
 
 

@@ -261,6 +261,18 @@ int rg_wstack_predict(rg_context *ctx, const rg_wstack_params *p, const double *
                       const rg_sep_kern *conv_w, const rg_sep_kern *corr_lm,
                       const rg_sep_kern *corr_n, void *out, void *stream);
 
+/* ---- EPIC E-field gridding ----
+ * epic_romein / scatter_grid_kernel (src/epic/epic_grid.cu:54-218): batch b
+ * holds n_ant antenna E-fields f[b][a] (RG_C64) at integer grid positions
+ * (x, y)[b][a]; grids[b] (g x g RG_C64, row y, column x) =
+ * sum_a conj(illum[ty][tx]) f[b][a] at (y + ty, x + tx) over the
+ * support x support illumination (RG_C64); taps outside the grid are dropped.
+ * z is accepted for the reference's signature and not read (nor is it by the
+ * reference's kernel).  All arrays are device arrays. */
+int rg_epic_grid(rg_context *ctx, const void *fdata, const int32_t *x, const int32_t *y,
+                 const int32_t *z, int32_t n_ant, int32_t nbatch, const void *illum,
+                 int32_t support, int32_t grid_size, void *grids, void *stream);
+
 /* ---- input generation ---- */
 /* Point-source visibilities V = sum_s flux_s exp(-2 pi i (u l_s + v m_s + w (n_s - 1)))
  * (predict.cpp:27-45 convention) evaluated in FP64 on the device.

@@ -768,3 +768,36 @@ void orc_predict_quantized(orc_c128 *out, const double *pl, const double *pm, co
         out[i].im = im;
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* EPIC E-field gridding (src/epic/epic_grid.cu)                        */
+/* ------------------------------------------------------------------ */
+
+/* scatter_grid_point / scatter_grid_kernel (epic_grid.cu:54-179) and
+ * epic_naive (epic_grid.cu:337-359): for every batch b and antenna a,
+ * grid[b][y_a + ty][x_a + tx] += conj(illum[ty][tx]) f[b][a] over the
+ * support x support illumination; taps outside [0, g)^2 are dropped
+ * (scatter_grid_add's bounds check).  The naive loop multiplies by illum
+ * itself, which is the same for the real top-hat the reference uses. */
+void orc_epic_grid(orc_c128 *grids, const orc_c128 *f, const int *x, const int *y, int n_ant,
+                   int nbatch, const orc_c128 *illum, int supp, int g)
+{
+    memset(grids, 0, sizeof(orc_c128) * (size_t)nbatch * g * g);
+    for (int b = 0; b < nbatch; b++) {
+        orc_c128 *grid = grids + (size_t)b * g * g;
+        for (int a = 0; a < n_ant; a++) {
+            const size_t i = (size_t)b * n_ant + a;
+            for (int ty = 0; ty < supp; ty++) {
+                for (int tx = 0; tx < supp; tx++) {
+                    const int X = x[i] + tx, Y = y[i] + ty;
+                    if (X < 0 || X >= g || Y < 0 || Y >= g)
+                        continue;
+                    const orc_c128 k = illum[ty * supp + tx];
+                    orc_c128 *c = &grid[(size_t)Y * g + X];
+                    c->re += k.re * f[i].re + k.im * f[i].im;
+                    c->im += k.re * f[i].im - k.im * f[i].re;
+                }
+            }
+        }
+    }
+}

@@ -139,6 +139,10 @@ void orc_predict_quantized(orc_c128 *out, const double *pl, const double *pm, co
                            int npts, double theta, double lam, const double *u, const double *v,
                            const double *w, int64_t nvis, int nthreads);
 
+/* EPIC E-field gridding (src/epic/epic_grid.cu:54-179, 337-359). */
+void orc_epic_grid(orc_c128 *grids, const orc_c128 *f, const int *x, const int *y, int n_ant,
+                   int nbatch, const orc_c128 *illum, int supp, int g);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,6 +79,8 @@ def lib(fast: bool = False):
         L.orc_wstack_degrid.restype = _L
         L.orc_predict_quantized.argtypes = [_P, _P, _P, _P, _I, _D, _D, _P, _P, _P, _L, _I]
         L.orc_predict_quantized.restype = None
+        L.orc_epic_grid.argtypes = [_P, _P, _P, _P, _I, _I, _P, _I, _I]
+        L.orc_epic_grid.restype = None
         _libs[name] = L
     return _libs[name]
 
@@ -252,4 +254,16 @@ def predict_quantized(l, m, flux, theta, lam, u, v, w, nthreads=0):
     lib().orc_predict_quantized(out.ctypes.data, l.ctypes.data, m.ctypes.data,
                                 None if fl is None else fl.ctypes.data, len(l), theta, lam,
                                 u.ctypes.data, v.ctypes.data, w.ctypes.data, len(u), nthreads)
+    return out
+
+
+def epic_grid(f, x, y, illum, g):
+    """E-field grids [nbatch][g][g] complex128; f [nbatch][n_ant], x, y int, illum [S][S]."""
+    f = np.ascontiguousarray(f, dtype=np.complex128)
+    nbatch, n_ant = f.shape
+    x, y = _c(x, np.int32), _c(y, np.int32)
+    il = np.ascontiguousarray(illum, dtype=np.complex128)
+    out = np.empty((nbatch, g, g), np.complex128)
+    lib().orc_epic_grid(out.ctypes.data, f.ctypes.data, x.ctypes.data, y.ctypes.data, n_ant, nbatch,
+                        il.ctypes.data, il.shape[0], g)
     return out

@@ -529,6 +529,26 @@ int rg_wstack_predict(rg_context *ctx, const rg_wstack_params *p, const double *
     return rg_wstack_degrid(ctx, p, u, v, w, nvis, conv_uv, conv_w, out, stream);
 }
 
+int rg_epic_grid(rg_context *ctx, const void *fdata, const int32_t *x, const int32_t *y,
+                 const int32_t *z, int32_t n_ant, int32_t nbatch, const void *illum,
+                 int32_t support, int32_t grid_size, void *grids, void *stream)
+{
+    (void)z;
+    if (!ctx || !grids || !illum)
+        return fail(RG_ERR_PARAM, "NULL argument");
+    if (n_ant < 0 || nbatch < 0 || (n_ant > 0 && nbatch > 0 && (!fdata || !x || !y)))
+        return fail(RG_ERR_PARAM, "invalid E-field arrays");
+    if (support < 1 || support > 64 || grid_size < 1 || grid_size > 4096 ||
+        (int64_t)n_ant * support * support > ((int64_t)1 << 30))
+        return fail(RG_ERR_PARAM, "invalid EPIC geometry (support %d, grid %d)", support,
+                    grid_size);
+    RG_HIP(hipSetDevice(ctx->device));
+    if (nbatch == 0)
+        return RG_OK;
+    return epic_grid(fdata, x, y, n_ant, nbatch, illum, support, grid_size, grids,
+                     (hipStream_t)stream);
+}
+
 int rg_dft(rg_context *ctx, const rg_flat_vis *vis, const int32_t *px, const int32_t *py,
            int64_t npix, int32_t grid_size, double lambda, void *out, int32_t mode, void *stream)
 {

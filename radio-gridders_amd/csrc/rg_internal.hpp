@@ -171,6 +171,8 @@ int make_hermitian(void *grid, int type, int n, hipStream_t s);
 int gcf_correct(void *img, int type, int n, const double *taper, hipStream_t s);
 int wstack_geometry(double theta, double lambda, double dw, int aa_w, double x0, int &g, int &og,
                     int &w_planes);
+int epic_grid(const void *f, const int32_t *x, const int32_t *y, int n_ant, int nbatch,
+              const void *illum, int supp, int g, void *grids, hipStream_t s);
 int wstack_build(rg_context *ctx, const rg_wstack_params *p, const double *l, const double *m,
                  const double *flux, int32_t npts, const rg_sep_kern *corr_lm,
                  const rg_sep_kern *corr_n, hipStream_t s);

@@ -109,6 +109,7 @@ SIGNATURES = [
     ("rg_wstack_degrid", C.c_int, [_P, C.POINTER(WStackParams), _P, _P, _P, _L,
                                    C.POINTER(SepKernDesc), C.POINTER(SepKernDesc), _P, _P]),
     ("rg_wstack_copy", C.c_int, [_P, _P, _L, _P]),
+    ("rg_epic_grid", C.c_int, [_P, _P, _P, _P, _P, _I, _I, _P, _I, _I, _P, _P]),
     ("rg_wstack_predict", C.c_int, [_P, C.POINTER(WStackParams), _P, _P, _P, _I, _P, _P, _P, _L,
                                     C.POINTER(SepKernDesc), C.POINTER(SepKernDesc),
                                     C.POINTER(SepKernDesc), C.POINTER(SepKernDesc), _P, _P]),
@@ -292,6 +293,14 @@ class Context:
                                       _ptr(u), _ptr(v), _ptr(w), u.numel(), C.byref(conv_uv),
                                       C.byref(conv_w), C.byref(corr_lm), C.byref(corr_n),
                                       _ptr(out), _stream(stream)))
+
+    def epic_grid(self, f, x, y, illum, grids, stream=None):
+        """E-field grids: f [nbatch][n_ant] complex64, x, y int32 device tensors,
+        illum [S][S] complex64, grids [nbatch][g][g] complex64 (all device)."""
+        nbatch, n_ant = f.shape
+        check(lib().rg_epic_grid(self.h, _ptr(f), _ptr(x), _ptr(y), None, n_ant, nbatch,
+                                 _ptr(illum), illum.shape[0], grids.shape[-1], _ptr(grids),
+                                 _stream(stream)))
 
     def dft(self, fv: FlatVis, px, py, grid_size: int, lam: float, out, mode=RG_DFT_FP64,
             stream=None):
