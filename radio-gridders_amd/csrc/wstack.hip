@@ -10,12 +10,82 @@
 // their quantised pixels, so each plane's input is a zero plane plus one
 // value per source; one batched FFT then transforms every plane.
 //
-// Degridding: one wavefront per visibility, lanes spread over the kernel's
-// taps so that each load instruction reads whole stack rows; the wavefront
-// reduces with shuffles.
-#include "binning.hpp"
+// Degridding: visibilities counting-sorted by uv tile, one wavefront per
+// visibility with lanes spread over the kernel's taps so that each load
+// instruction reads whole stack rows, each wavefront walking a contiguous
+// chunk of the sorted order (chunks grouped per XCD); shuffle reduction.
+#include "binning_impl.hpp"
 
 namespace rg {
+
+// ---- visibility order ---------------------------------------------------------------------
+// Degridding reads 128-byte stack rows at random addresses when visibilities
+// come in random order (the L2 misses nearly every row).  Visibilities are
+// therefore counting-sorted by 16 x 16-cell uv tile of their first tap.  The
+// sort's scatter pass also evaluates everything a visibility's taps depend on
+// (ceil / floor of u / du, v / du, w / dw; deconvolve_visibility_,
+// wstack_common.h:430-460) and stores it packed in the record:
+//   pos = visibility index, kidx = us | vs << 16,
+//   vis bits = ws | ovu << 16 | ovv << 32 | ovw << 48
+// (us, vs, ws the stack cell of the centre tap, ov* the oversampling rows,
+// all < 2^16).  Visibilities whose taps leave the stack are dropped here
+// (counted; their output stays 0).
+struct WsCell {
+    int us, vs, ws, ovu, ovv, ovw;
+};
+
+struct WStackKeyer {
+    const double *u, *v, *w;
+    double du, dw;
+    int g, og, wp, aa_h, aaw_h, ntx, ov_uv, ov_w;
+    static constexpr int TILE_SHIFT = 4;
+
+    __device__ bool cell(int64_t i, WsCell &c) const
+    {
+#pragma clang fp contract(off)
+        const double ui = u[i], vi = v[i], wi = w[i];
+        const double cu = ceil(ui / du), cv = ceil(vi / du), cw = ceil(wi / dw);
+        if (!(cu > -1e9 && cu < 1e9 && cv > -1e9 && cv < 1e9 && cw > -1e9 && cw < 1e9))
+            return false;
+        c.us = (int)cu + g;
+        c.vs = (int)cv + g;
+        c.ws = (int)cw + wp / 2;
+        if (c.us - aa_h < 0 || c.us + aa_h > og || c.vs - aa_h < 0 || c.vs + aa_h > og ||
+            c.ws - aaw_h < 0 || c.ws + aaw_h > wp)
+            return false;
+        const double flu = ui - cu * du, flv = vi - cv * du, flw = wi - cw * dw;
+        c.ovu = min((int)floor(fabs(flu) / du * ov_uv), ov_uv - 1);
+        c.ovv = min((int)floor(fabs(flv) / du * ov_uv), ov_uv - 1);
+        c.ovw = min((int)floor(fabs(flw) / dw * ov_w), ov_w - 1);
+        return true;
+    }
+
+    __device__ int classify(int64_t i, uint32_t &key, uint32_t &pos, uint32_t &kidx) const
+    {
+        WsCell c;
+        if (!cell(i, c))
+            return VIS_DROPPED;
+        key = (uint32_t)((c.vs >> TILE_SHIFT) * ntx + (c.us >> TILE_SHIFT));
+        pos = (uint32_t)i;
+        kidx = (uint32_t)c.us | ((uint32_t)c.vs << 16);
+        return VIS_OK;
+    }
+
+    // called for visibilities classify accepted
+    template <typename T> __device__ typename Vec2<T>::type vis_value(int64_t i) const
+    {
+        WsCell c;
+        cell(i, c);
+        const unsigned long long b = (unsigned long long)c.ws |
+                                     ((unsigned long long)c.ovu << 16) |
+                                     ((unsigned long long)c.ovv << 32) |
+                                     ((unsigned long long)c.ovw << 48);
+        typename Vec2<T>::type r;
+        r.x = __longlong_as_double((long long)b);
+        r.y = 0.0;
+        return r;
+    }
+};
 
 // ---- stack geometry -------------------------------------------------------------------------
 // wstack_predict (predict.cpp:410-428) and generate_sky (predict.cpp:228-231)
@@ -95,58 +165,123 @@ __global__ __launch_bounds__(256) void k_ws_sources(
 }
 
 // ---- degridding ------------------------------------------------------------------------------
-// deconvolve_visibility_ (wstack_common.h:420-470).  One wavefront per
-// visibility (grid-stride over visibilities).  Lane = (pair, tap): tap
-// t = lane % aa_uv along u, pair q = lane / aa_uv + k (64 / aa_uv) over the
-// aa_w * aa_uv (plane, row) pairs, so one load instruction reads 64 / aa_uv
-// whole kernel rows (contiguous runs of the stack).  Each lane sums
-// stack[dws][dvs][dus] (cw cv) cu over its pairs; the wavefront reduces with
-// shuffles.  A visibility whose taps leave the stack is predicted as 0 and
-// counted.
+// deconvolve_visibility_ (wstack_common.h:420-470) over the tile-sorted
+// records.  LV lanes per visibility: tap t = lane % aa_uv along u, pair
+// q = lane / aa_uv + k (LV / aa_uv) over the aa_w * aa_uv (plane, row) pairs,
+// so one load instruction reads whole kernel rows (contiguous runs of the
+// stack).  Each lane sums stack[dws][dvs][dus] (cw cv) cu over its pairs; the
+// group reduces with shuffles.  Every group walks one contiguous chunk of the
+// sorted records; workgroup b runs on XCD b mod 8, so chunks are numbered to
+// give each XCD one contiguous eighth of the sorted order and its L2 the
+// neighbouring visibilities.
 template <int LV>
 __global__ __launch_bounds__(256) void k_ws_degrid(
-    const double2 *__restrict__ stack, int og, int w_planes, int g, const double *__restrict__ u,
-    const double *__restrict__ v, const double *__restrict__ w, int64_t nvis, double du, double dw,
-    int aa_uv, int aa_w, const double *__restrict__ conv_uv, int ov_uv,
-    const double *__restrict__ conv_w, int ov_w, double2 *out, unsigned long long *dropped)
+    const double2 *__restrict__ stack, int og, const Rec<double> *__restrict__ recs,
+    const uint32_t *__restrict__ nrec_p, int aa_uv, int aa_w,
+    const double *__restrict__ conv_uv, const double *__restrict__ conv_w, double2 *out)
 {
 #pragma clang fp contract(off)
-    // LV lanes per visibility (64 / LV visibilities per wavefront)
+    constexpr int XCDS = 8;
     const int lane = (int)(threadIdx.x % LV);
-    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / LV;
-    const int64_t waves = ((int64_t)gridDim.x * blockDim.x) / LV;
+    const int per_block = 256 / LV;
+    const int nb = gridDim.x; // a multiple of XCDS
+    const int vb = (blockIdx.x % XCDS) * (nb / XCDS) + blockIdx.x / XCDS;
+    const int64_t group = (int64_t)vb * per_block + threadIdx.x / LV;
+    const int64_t groups = (int64_t)nb * per_block;
+    const int64_t nrec = *nrec_p;
+    const int64_t chunk = (nrec + groups - 1) / groups;
+    const int64_t r0 = min(nrec, group * chunk), r1 = min(nrec, r0 + chunk);
     const int aa_h = aa_uv / 2, aaw_h = aa_w / 2;
     const int t = lane % aa_uv, q0 = lane / aa_uv, qs = LV / aa_uv, npairs = aa_w * aa_uv;
-    // every lane of a wavefront runs the same number of iterations
-    const int64_t iters = (nvis + waves - 1) / waves;
-    for (int64_t it = 0; it < iters; ++it) {
-        const int64_t i0 = wave + it * waves;
-        const bool live = i0 < nvis;
-        const int64_t i = live ? i0 : 0;
-        const double ui = u[i], vi = v[i], wi = w[i];
-        const double cu = ceil(ui / du), cv = ceil(vi / du), cw = ceil(wi / dw);
-        const bool finite =
-            cu > -1e9 && cu < 1e9 && cv > -1e9 && cv < 1e9 && cw > -1e9 && cw < 1e9;
-        const int us = finite ? (int)cu + g : 0, vs = finite ? (int)cv + g : 0;
-        const int ws = finite ? (int)cw + w_planes / 2 : 0;
-        const bool inside = finite && us - aa_h >= 0 && us + aa_h <= og && vs - aa_h >= 0 &&
-                            vs + aa_h <= og && ws - aaw_h >= 0 && ws + aaw_h <= w_planes;
+    Rec<double> next{};
+    if (r0 < r1)
+        next = recs[r0];
+    for (int64_t k = r0; k < r1; ++k) { // uniform across the group
+        const Rec<double> rec = next;
+        if (k + 1 < r1)
+            next = recs[k + 1];
+        const int us = (int)(rec.kidx & 0xffffu), vs = (int)(rec.kidx >> 16);
+        const unsigned long long b = (unsigned long long)__double_as_longlong(rec.vis.x);
+        const int ws = (int)(b & 0xffffu), ovu = (int)((b >> 16) & 0xffffu);
+        const int ovv = (int)((b >> 32) & 0xffffu), ovw = (int)(b >> 48);
+        const double gu = conv_uv[aa_uv * ovu + t];
+        const double2 *col = stack + (us - aa_h + t);
         double re = 0.0, im = 0.0;
-        if (live && inside && t < aa_uv && q0 < qs) {
-            const double flu = ui - cu * du, flv = vi - cv * du, flw = wi - cw * dw;
-            const int ovu = min((int)floor(fabs(flu) / du * ov_uv), ov_uv - 1);
-            const int ovv = min((int)floor(fabs(flv) / du * ov_uv), ov_uv - 1);
-            const int ovw = min((int)floor(fabs(flw) / dw * ov_w), ov_w - 1);
-            const double gu = conv_uv[aa_uv * ovu + t];
-            const double2 *col = stack + (us - aa_h + t);
-            for (int q = q0; q < npairs; q += qs) {
-                const int pw = q / aa_uv, pv = q - pw * aa_uv; // dwi + aaw_h, dvi + aa_h
-                const double gw = conv_w[aa_w * ovw + pw];
-                const double gv = gw * conv_uv[aa_uv * ovv + pv];
-                const double2 s = col[((size_t)(ws - aaw_h + pw) * og + (vs - aa_h + pv)) * og];
-                const double k = gv * gu;
-                re += s.x * k;
-                im += s.y * k;
+        for (int q = q0; q < npairs; q += qs) {
+            const int pw = q / aa_uv, pv = q - pw * aa_uv; // dwi + aaw_h, dvi + aa_h
+            const double gw = conv_w[aa_w * ovw + pw];
+            const double gv = gw * conv_uv[aa_uv * ovv + pv];
+            const double2 sv = col[((size_t)(ws - aaw_h + pw) * og + (vs - aa_h + pv)) * og];
+            const double kk = gv * gu;
+            re += sv.x * kk;
+            im += sv.y * kk;
+        }
+#pragma unroll
+        for (int o = LV / 2; o > 0; o >>= 1) {
+            re += __shfl_xor(re, o, LV);
+            im += __shfl_xor(im, o, LV);
+        }
+        if (lane == 0)
+            out[rec.pos] = make_double2(re, im);
+    }
+}
+
+// The same with the supports known at compile time (the reference's Sze-Tan
+// kernels are 8 (uv) x 4 (w)): LV lanes per visibility, lane = (tap t, pair
+// group h), each lane summing PAIRS / (LV / AUV) (plane, row) pairs fully
+// unrolled, so the divisions by the support vanish and the 64 / LV
+// visibilities of a wavefront share every instruction, the reduction
+// included.
+template <int AUV, int AW, int LV>
+__global__ __launch_bounds__(256) void k_ws_degrid_fixed(
+    const double2 *__restrict__ stack, int og, const Rec<double> *__restrict__ recs,
+    const uint32_t *__restrict__ nrec_p, const double *__restrict__ conv_uv,
+    const double *__restrict__ conv_w, double2 *out)
+{
+#pragma clang fp contract(off)
+    constexpr int XCDS = 8, GROUPS = LV / AUV, PAIRS = AUV * AW, PER_LANE = PAIRS / GROUPS;
+    static_assert(LV % AUV == 0 && PAIRS % GROUPS == 0 && 64 % LV == 0, "lane layout");
+    const int lane = (int)(threadIdx.x % LV);
+    const int t = lane % AUV, h = lane / AUV;
+    constexpr int per_block = 256 / LV;
+    const int nb = gridDim.x; // a multiple of XCDS
+    const int vb = (blockIdx.x % XCDS) * (nb / XCDS) + blockIdx.x / XCDS;
+    const int64_t group = (int64_t)vb * per_block + threadIdx.x / LV;
+    const int64_t groups = (int64_t)nb * per_block;
+    const int64_t nrec = *nrec_p;
+    const int64_t chunk = (nrec + groups - 1) / groups;
+    const int64_t r0 = min(nrec, group * chunk), r1 = min(nrec, r0 + chunk);
+    const size_t plane = (size_t)og * og;
+    // every group of a wavefront runs the same number of iterations
+    const int64_t iters = chunk;
+    Rec<double> next{};
+    if (r0 < r1)
+        next = recs[r0];
+    for (int64_t it = 0; it < iters; ++it) {
+        const int64_t k = r0 + it;
+        const bool live = k < r1;
+        const Rec<double> rec = next;
+        if (k + 1 < r1)
+            next = recs[k + 1];
+        double re = 0.0, im = 0.0;
+        if (live) {
+            const int us = (int)(rec.kidx & 0xffffu), vs = (int)(rec.kidx >> 16);
+            const unsigned long long b = (unsigned long long)__double_as_longlong(rec.vis.x);
+            const int ws = (int)(b & 0xffffu), ovu = (int)((b >> 16) & 0xffffu);
+            const int ovv = (int)((b >> 32) & 0xffffu), ovw = (int)(b >> 48);
+            const double gu = conv_uv[AUV * ovu + t];
+            const double *cw = conv_w + AW * ovw, *cv = conv_uv + AUV * ovv;
+            const double2 *base =
+                stack + ((size_t)(ws - AW / 2) * og + (vs - AUV / 2)) * og + (us - AUV / 2 + t);
+#pragma unroll
+            for (int j = 0; j < PER_LANE; ++j) {
+                const int q = h + j * GROUPS;
+                const int pw = q / AUV, pv = q % AUV; // dwi + AW/2, dvi + AUV/2
+                const double gv = cw[pw] * cv[pv];
+                const double2 sv = base[(size_t)pw * plane + (size_t)pv * og];
+                const double kk = gv * gu;
+                re += sv.x * kk;
+                im += sv.y * kk;
             }
         }
 #pragma unroll
@@ -154,11 +289,8 @@ __global__ __launch_bounds__(256) void k_ws_degrid(
             re += __shfl_xor(re, o, LV);
             im += __shfl_xor(im, o, LV);
         }
-        if (live && lane == 0) {
-            out[i] = make_double2(inside ? re : 0.0, inside ? im : 0.0);
-            if (!inside)
-                atomicAdd(dropped, 1ull);
-        }
+        if (live && lane == 0)
+            out[rec.pos] = make_double2(re, im);
     }
 }
 
@@ -208,24 +340,47 @@ int wstack_degrid(rg_context *ctx, const rg_wstack_params *p, const double *u, c
     if (nvis == 0)
         return RG_OK;
     const double2 *stack = ctx->scratch[17].as<double2>();
-    // LV lanes per visibility: at least one whole kernel row per group
-    constexpr int LV_DEFAULT = 64;
-    const int lv = std::max(LV_DEFAULT, (int)p->aa_support_uv);
-    auto launch = [&](auto lanes) {
-        constexpr int LV = decltype(lanes)::value;
-        const int per_block = 256 / LV;
-        const int64_t blocks = std::min<int64_t>((nvis + per_block - 1) / per_block, (int64_t)1 << 20);
-        hipLaunchKernelGGL(k_ws_degrid<LV>, dim3((unsigned)blocks), dim3(256), 0, s, stack, og, wp,
-                           g, u, v, w, nvis, p->du, p->dw, p->aa_support_uv, p->aa_support_w,
-                           conv_uv->data, conv_uv->oversampling, conv_w->data,
-                           conv_w->oversampling, (double2 *)out, cnt + CNT_DROPPED);
+    // sort by uv tile (dropped visibilities keep the zero written here)
+    RG_TRY(zero_fill(out, sizeof(double2) * (size_t)nvis, s));
+    WStackKeyer kf;
+    kf.u = u;
+    kf.v = v;
+    kf.w = w;
+    kf.du = p->du;
+    kf.dw = p->dw;
+    kf.g = g;
+    kf.og = og;
+    kf.wp = wp;
+    kf.aa_h = p->aa_support_uv / 2;
+    kf.aaw_h = p->aa_support_w / 2;
+    kf.ntx = (og + (1 << WStackKeyer::TILE_SHIFT) - 1) >> WStackKeyer::TILE_SHIFT;
+    kf.ov_uv = conv_uv->oversampling;
+    kf.ov_w = conv_w->oversampling;
+    if (og > 65535 || wp > 65535 || kf.ov_uv > 65535 || kf.ov_w > 65535)
+        return fail(RG_ERR_UNSUPPORTED, "W-stack indices beyond 16 bits");
+    BinResult br;
+    RG_TRY((bin_visibilities<WStackKeyer, double>(ctx, kf, nvis, (int64_t)kf.ntx * kf.ntx,
+                                                   0x40000000u, s, br)));
+    const Rec<double> *recs = reinterpret_cast<const Rec<double> *>(br.recs);
+    const uint32_t *nrec = br.vis_start + (int64_t)kf.ntx * kf.ntx;
+    // one wavefront per visibility (the uv support, a power of two <= 64,
+    // divides it); a multiple of 8 workgroups (one share per XCD), at least
+    // 16 records per wavefront
+    auto nblocks = [&](int per_block) {
+        int64_t blocks = std::min<int64_t>((nvis + 16 * per_block - 1) / (16 * per_block), 8192);
+        return (blocks + 7) / 8 * 8;
     };
-    if (lv <= 16)
-        launch(std::integral_constant<int, 16>());
-    else if (lv <= 32)
-        launch(std::integral_constant<int, 32>());
-    else
-        launch(std::integral_constant<int, 64>());
+    if (p->aa_support_uv == 8 && p->aa_support_w == 4) {
+        constexpr int LV = 16;
+        hipLaunchKernelGGL((k_ws_degrid_fixed<8, 4, LV>), dim3((unsigned)nblocks(256 / LV)),
+                           dim3(256), 0, s, stack, og, recs, nrec, conv_uv->data, conv_w->data,
+                           (double2 *)out);
+    } else {
+        constexpr int LV = 64;
+        hipLaunchKernelGGL(k_ws_degrid<LV>, dim3((unsigned)nblocks(256 / LV)), dim3(256), 0, s,
+                           stack, og, recs, nrec, p->aa_support_uv, p->aa_support_w, conv_uv->data,
+                           conv_w->data, (double2 *)out);
+    }
     RG_HIP(hipGetLastError());
     return RG_OK;
 }
