@@ -158,7 +158,7 @@ struct BinResult {
 
 template <typename Keyer, typename T>
 int bin_visibilities(rg_context *ctx, const Keyer &kf, int64_t n, int64_t nkeys, uint32_t per_item,
-                     hipStream_t s, BinResult &out);
+                     hipStream_t s, BinResult &out, int64_t max_counters = (int64_t)1 << 22);
 
 // Work items over per-key lengths len[0..nkeys): prefix[k] = exclusive sum of
 // len, item_start[k] = first item of key k (both nkeys + 1 entries), items of

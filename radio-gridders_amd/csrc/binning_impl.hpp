@@ -8,7 +8,7 @@ namespace rg {
 
 template <typename Keyer, typename T>
 int bin_visibilities(rg_context *ctx, const Keyer &kf, int64_t n, int64_t nkeys, uint32_t per_item,
-                     hipStream_t s, BinResult &out)
+                     hipStream_t s, BinResult &out, int64_t max_counters)
 {
     // record indices are 32-bit
     const int64_t nrec = n;
@@ -20,11 +20,13 @@ int bin_visibilities(rg_context *ctx, const Keyer &kf, int64_t n, int64_t nkeys,
     if (nblocks_scan > 1024 * 64)
         return fail(RG_ERR_PARAM, "key space too large (%lld)", (long long)nkeys);
     const int64_t max_items = nkeys + nrec / per_item + 1;
-    // counter replicas: up to 16, at most 4M counters in all
+    // counter replicas: up to 16, at most max_counters counters in all
+    // (replicas spread the atomics of hot keys; a stream without hot keys
+    // is better served by fewer counters that stay in cache)
     Replicas rp;
     rp.stride = (nkeys + 1 + 63) / 64 * 64;
     uint32_t nrep = 16;
-    while (nrep > 1 && (int64_t)nrep * rp.stride > ((int64_t)1 << 22))
+    while (nrep > 1 && (int64_t)nrep * rp.stride > max_counters)
         nrep >>= 1;
     rp.mask = nrep - 1;
 

@@ -21,7 +21,8 @@ namespace rg {
 // ---- visibility order ---------------------------------------------------------------------
 // Degridding reads 128-byte stack rows at random addresses when visibilities
 // come in random order (the L2 misses nearly every row).  Visibilities are
-// therefore counting-sorted by 16 x 16-cell uv tile of their first tap.  The
+// therefore counting-sorted by (16 x 16-cell uv tile, w plane) of their
+// centre tap (key = tile * w_planes + ws).  The
 // sort's scatter pass also evaluates everything a visibility's taps depend on
 // (ceil / floor of u / du, v / du, w / dw; deconvolve_visibility_,
 // wstack_common.h:430-460) and stores it packed in the record:
@@ -40,7 +41,7 @@ struct WStackKeyer {
     int g, og, wp, aa_h, aaw_h, ntx, ov_uv, ov_w;
     static constexpr int TILE_SHIFT = 4;
 
-    __device__ bool cell(int64_t i, WsCell &c) const
+    __device__ bool cell(int64_t i, WsCell &c, bool with_ov = true) const
     {
 #pragma clang fp contract(off)
         const double ui = u[i], vi = v[i], wi = w[i];
@@ -53,6 +54,8 @@ struct WStackKeyer {
         if (c.us - aa_h < 0 || c.us + aa_h > og || c.vs - aa_h < 0 || c.vs + aa_h > og ||
             c.ws - aaw_h < 0 || c.ws + aaw_h > wp)
             return false;
+        if (!with_ov)
+            return true;
         const double flu = ui - cu * du, flv = vi - cv * du, flw = wi - cw * dw;
         c.ovu = min((int)floor(fabs(flu) / du * ov_uv), ov_uv - 1);
         c.ovv = min((int)floor(fabs(flv) / du * ov_uv), ov_uv - 1);
@@ -63,9 +66,9 @@ struct WStackKeyer {
     __device__ int classify(int64_t i, uint32_t &key, uint32_t &pos, uint32_t &kidx) const
     {
         WsCell c;
-        if (!cell(i, c))
+        if (!cell(i, c, false))
             return VIS_DROPPED;
-        key = (uint32_t)((c.vs >> TILE_SHIFT) * ntx + (c.us >> TILE_SHIFT));
+        key = (uint32_t)(((c.vs >> TILE_SHIFT) * ntx + (c.us >> TILE_SHIFT)) * wp + c.ws);
         pos = (uint32_t)i;
         kidx = (uint32_t)c.us | ((uint32_t)c.vs << 16);
         return VIS_OK;
@@ -294,6 +297,111 @@ __global__ __launch_bounds__(256) void k_ws_degrid_fixed(
     }
 }
 
+// Fixed 8 x 4 support with the stack staged in LDS: one work item (<= K
+// records of one (uv tile, w plane) key) at a time per workgroup; the
+// (16 + 8)^2 cells x 4 planes its taps can reach (36 KiB) are loaded once,
+// then every record reads its 32 rows from LDS.  Lanes as k_ws_degrid_fixed.
+template <int AUV, int AW, int LV>
+__global__ __launch_bounds__(256) void k_ws_degrid_tile(
+    const double2 *__restrict__ stack, int og, int wp, int ntx,
+    const Rec<double> *__restrict__ recs, const Item *__restrict__ items,
+    const uint32_t *__restrict__ nitems_p, const double *__restrict__ conv_uv,
+    const double *__restrict__ conv_w, double2 *out)
+{
+#pragma clang fp contract(off)
+    constexpr int T = 1 << WStackKeyer::TILE_SHIFT, H = AUV / 2, R = T + AUV, P = AW;
+    constexpr int GROUPS = LV / AUV, PAIRS = AUV * AW, PER_LANE = PAIRS / GROUPS;
+    constexpr int GPB = 256 / LV; // groups per workgroup
+    static_assert(LV % AUV == 0 && PAIRS % GROUPS == 0 && 64 % LV == 0, "lane layout");
+    __shared__ double2 reg[P * R * R];
+    const int lane = (int)(threadIdx.x % LV);
+    const int t = lane % AUV, h = lane / AUV, grp = (int)(threadIdx.x / LV);
+    const uint32_t nitems = *nitems_p;
+    for (uint32_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const Item item = items[it];
+        const int ws = (int)(item.key % (uint32_t)wp);
+        const int tile = (int)(item.key / (uint32_t)wp);
+        const int col0 = (tile % ntx) * T - H, row0 = (tile / ntx) * T - H, pl0 = ws - AW / 2;
+        __syncthreads(); // the previous item's records have been read
+        for (int e = threadIdx.x; e < P * R * R; e += blockDim.x) {
+            const int pp = e / (R * R), rr = (e / R) % R, cc = e % R;
+            const int row = row0 + rr, col = col0 + cc;
+            double2 val = make_double2(0.0, 0.0);
+            if (row >= 0 && row < og && col >= 0 && col < og)
+                val = stack[((size_t)(pl0 + pp) * og + row) * og + col];
+            reg[e] = val;
+        }
+        __syncthreads();
+        const int rounds = (int)((item.count + GPB - 1) / GPB);
+        // two-deep pipeline: round rd sums record rd with the weights loaded
+        // in round rd - 1, loads the weights of record rd + 1 (read in round
+        // rd - 1) and reads record rd + 2
+        struct Taps {
+            uint32_t pos;
+            int lr, lc; // local row / column of tap (-H, -H) + (0, t)
+            double gu, gw[AW], gv[PER_LANE < AUV ? PER_LANE : AUV];
+        };
+        auto record = [&](int rd) {
+            Rec<double> r{};
+            const uint32_t k = (uint32_t)(rd * GPB + grp);
+            if (rd < rounds && k < item.count)
+                r = recs[item.start + k];
+            return r;
+        };
+        auto weights = [&](const Rec<double> &rec) {
+            Taps w;
+            w.pos = rec.pos;
+            const int us = (int)(rec.kidx & 0xffffu), vs = (int)(rec.kidx >> 16);
+            const unsigned long long b = (unsigned long long)__double_as_longlong(rec.vis.x);
+            const int ovu = (int)((b >> 16) & 0xffffu);
+            const int ovv = (int)((b >> 32) & 0xffffu), ovw = (int)(b >> 48);
+            w.lr = vs - H - row0;
+            w.lc = us - H - col0 + t;
+            w.gu = conv_uv[AUV * ovu + t];
+#pragma unroll
+            for (int pw = 0; pw < AW; ++pw)
+                w.gw[pw] = conv_w[AW * ovw + pw];
+            // the rows a lane visits: pv = (h + j GROUPS) mod AUV
+#pragma unroll
+            for (int j = 0; j < (PER_LANE < AUV ? PER_LANE : AUV); ++j)
+                w.gv[j] = conv_uv[AUV * ovv + (h + j * GROUPS) % AUV];
+            return w;
+        };
+        Rec<double> r1 = record(1);
+        Taps cur = weights(record(0));
+        for (int rd = 0; rd < rounds; ++rd) { // uniform across the workgroup
+            const uint32_t k = (uint32_t)(rd * GPB + grp);
+            const bool live = k < item.count;
+            const Rec<double> r2 = record(rd + 2);
+            const Taps nxt = weights(r1);
+            double re = 0.0, im = 0.0;
+            if (live) {
+                const double2 *base = reg + cur.lr * R + cur.lc;
+#pragma unroll
+                for (int j = 0; j < PER_LANE; ++j) {
+                    const int q = h + j * GROUPS;
+                    const int pw = q / AUV, pv = q % AUV; // dwi + AW/2, dvi + AUV/2
+                    constexpr int NV = PER_LANE < AUV ? PER_LANE : AUV;
+                    const double gv = cur.gw[pw] * cur.gv[j % NV];
+                    const double2 sv = base[(pw * R + pv) * R];
+                    const double kk = gv * cur.gu;
+                    re += sv.x * kk;
+                    im += sv.y * kk;
+                }
+            }
+#pragma unroll
+            for (int o = LV / 2; o > 0; o >>= 1) {
+                re += __shfl_xor(re, o, LV);
+                im += __shfl_xor(im, o, LV);
+            }
+            if (live && lane == 0)
+                out[cur.pos] = make_double2(re, im);
+            cur = nxt;
+            r1 = r2;
+        }
+    }
+}
+
 // ---- driver --------------------------------------------------------------------------------------
 
 int wstack_build(rg_context *ctx, const rg_wstack_params *p, const double *l, const double *m,
@@ -359,10 +467,14 @@ int wstack_degrid(rg_context *ctx, const rg_wstack_params *p, const double *u, c
     if (og > 65535 || wp > 65535 || kf.ov_uv > 65535 || kf.ov_w > 65535)
         return fail(RG_ERR_UNSUPPORTED, "W-stack indices beyond 16 bits");
     BinResult br;
-    RG_TRY((bin_visibilities<WStackKeyer, double>(ctx, kf, nvis, (int64_t)kf.ntx * kf.ntx,
-                                                   0x40000000u, s, br)));
+    const int64_t nkeys = (int64_t)kf.ntx * kf.ntx * wp;
+    constexpr uint32_t K = 1024; // records per work item
+    // arbitrary visibility order has no hot keys: one counter array (2 MiB
+    // at 524 288 keys) instead of replicas
+    RG_TRY((bin_visibilities<WStackKeyer, double>(ctx, kf, nvis, nkeys, K, s, br,
+                                                   (int64_t)1 << 20)));
     const Rec<double> *recs = reinterpret_cast<const Rec<double> *>(br.recs);
-    const uint32_t *nrec = br.vis_start + (int64_t)kf.ntx * kf.ntx;
+    const uint32_t *nrec = br.vis_start + nkeys;
     // one wavefront per visibility (the uv support, a power of two <= 64,
     // divides it); a multiple of 8 workgroups (one share per XCD), at least
     // 16 records per wavefront
@@ -372,8 +484,11 @@ int wstack_degrid(rg_context *ctx, const rg_wstack_params *p, const double *u, c
     };
     if (p->aa_support_uv == 8 && p->aa_support_w == 4) {
         constexpr int LV = 16;
-        hipLaunchKernelGGL((k_ws_degrid_fixed<8, 4, LV>), dim3((unsigned)nblocks(256 / LV)),
-                           dim3(256), 0, s, stack, og, recs, nrec, conv_uv->data, conv_w->data,
+        // persistent workgroups over the work items (nitems read on the device)
+        const int64_t blocks = std::min<int64_t>(br.max_items, 4096);
+        hipLaunchKernelGGL((k_ws_degrid_tile<8, 4, LV>), dim3((unsigned)blocks), dim3(256), 0, s,
+                           stack, og, wp, kf.ntx, recs, (const Item *)br.items,
+                           (const uint32_t *)(br.item_start + nkeys), conv_uv->data, conv_w->data,
                            (double2 *)out);
     } else {
         constexpr int LV = 64;
