@@ -229,78 +229,13 @@ __global__ __launch_bounds__(256) void k_ws_degrid(
     }
 }
 
-// The same with the supports known at compile time (the reference's Sze-Tan
-// kernels are 8 (uv) x 4 (w)): LV lanes per visibility, lane = (tap t, pair
-// group h), each lane summing PAIRS / (LV / AUV) (plane, row) pairs fully
-// unrolled, so the divisions by the support vanish and the 64 / LV
-// visibilities of a wavefront share every instruction, the reduction
-// included.
-template <int AUV, int AW, int LV>
-__global__ __launch_bounds__(256) void k_ws_degrid_fixed(
-    const double2 *__restrict__ stack, int og, const Rec<double> *__restrict__ recs,
-    const uint32_t *__restrict__ nrec_p, const double *__restrict__ conv_uv,
-    const double *__restrict__ conv_w, double2 *out)
-{
-#pragma clang fp contract(off)
-    constexpr int XCDS = 8, GROUPS = LV / AUV, PAIRS = AUV * AW, PER_LANE = PAIRS / GROUPS;
-    static_assert(LV % AUV == 0 && PAIRS % GROUPS == 0 && 64 % LV == 0, "lane layout");
-    const int lane = (int)(threadIdx.x % LV);
-    const int t = lane % AUV, h = lane / AUV;
-    constexpr int per_block = 256 / LV;
-    const int nb = gridDim.x; // a multiple of XCDS
-    const int vb = (blockIdx.x % XCDS) * (nb / XCDS) + blockIdx.x / XCDS;
-    const int64_t group = (int64_t)vb * per_block + threadIdx.x / LV;
-    const int64_t groups = (int64_t)nb * per_block;
-    const int64_t nrec = *nrec_p;
-    const int64_t chunk = (nrec + groups - 1) / groups;
-    const int64_t r0 = min(nrec, group * chunk), r1 = min(nrec, r0 + chunk);
-    const size_t plane = (size_t)og * og;
-    // every group of a wavefront runs the same number of iterations
-    const int64_t iters = chunk;
-    Rec<double> next{};
-    if (r0 < r1)
-        next = recs[r0];
-    for (int64_t it = 0; it < iters; ++it) {
-        const int64_t k = r0 + it;
-        const bool live = k < r1;
-        const Rec<double> rec = next;
-        if (k + 1 < r1)
-            next = recs[k + 1];
-        double re = 0.0, im = 0.0;
-        if (live) {
-            const int us = (int)(rec.kidx & 0xffffu), vs = (int)(rec.kidx >> 16);
-            const unsigned long long b = (unsigned long long)__double_as_longlong(rec.vis.x);
-            const int ws = (int)(b & 0xffffu), ovu = (int)((b >> 16) & 0xffffu);
-            const int ovv = (int)((b >> 32) & 0xffffu), ovw = (int)(b >> 48);
-            const double gu = conv_uv[AUV * ovu + t];
-            const double *cw = conv_w + AW * ovw, *cv = conv_uv + AUV * ovv;
-            const double2 *base =
-                stack + ((size_t)(ws - AW / 2) * og + (vs - AUV / 2)) * og + (us - AUV / 2 + t);
-#pragma unroll
-            for (int j = 0; j < PER_LANE; ++j) {
-                const int q = h + j * GROUPS;
-                const int pw = q / AUV, pv = q % AUV; // dwi + AW/2, dvi + AUV/2
-                const double gv = cw[pw] * cv[pv];
-                const double2 sv = base[(size_t)pw * plane + (size_t)pv * og];
-                const double kk = gv * gu;
-                re += sv.x * kk;
-                im += sv.y * kk;
-            }
-        }
-#pragma unroll
-        for (int o = LV / 2; o > 0; o >>= 1) {
-            re += __shfl_xor(re, o, LV);
-            im += __shfl_xor(im, o, LV);
-        }
-        if (live && lane == 0)
-            out[rec.pos] = make_double2(re, im);
-    }
-}
-
 // Fixed 8 x 4 support with the stack staged in LDS: one work item (<= K
 // records of one (uv tile, w plane) key) at a time per workgroup; the
 // (16 + 8)^2 cells x 4 planes its taps can reach (36 KiB) are loaded once,
-// then every record reads its 32 rows from LDS.  Lanes as k_ws_degrid_fixed.
+// then every record reads its 32 rows from LDS.  LV lanes per visibility,
+// lane = (tap t, pair group h), each lane summing PAIRS / (LV / AUV)
+// (plane, row) pairs fully unrolled; the 64 / LV visibilities of a wavefront
+// share every instruction, the reduction included.
 template <int AUV, int AW, int LV>
 __global__ __launch_bounds__(256) void k_ws_degrid_tile(
     const double2 *__restrict__ stack, int og, int wp, int ntx,

@@ -157,3 +157,32 @@ def test_cli_reports_the_aggregate_error():
     assert r.returncode == 0, r.stderr
     per = float(re.search(r"Aggregate Error / Npts: (\S+)", r.stdout).group(1))
     assert per < 8 * 1e-3, r.stdout
+
+
+def test_other_supports_match_oracle(ctx, kernels):
+    """Supports other than the reference's 8 x 4 (here 4 x 2 and 16 x 4, with
+    synthetic positive kernel tables) run the generic degridding kernel."""
+    theta, lam = 0.1, 1280.0
+    h, d = kernels["host"], kernels["desc"]
+    rng = np.random.default_rng(31)
+    l, m = card(theta)
+    for auv, aw in ((4, 2), (16, 4)):
+        ov = 64
+        cuv = rng.uniform(0.1, 1.0, auv * ov)
+        cw = rng.uniform(0.1, 1.0, aw * ov)
+        u, v, w = uvw(rng, theta, lam, h["w"].dw, 5000)
+        stack, _ = orc.wstack_build(theta, lam, l, m, None, h["uv"].du, h["w"].dw, aw, h["lm"].x0,
+                                    h["lm"].data, h["n"].data)
+        g = int(np.floor(theta * lam))
+        ref, nd = orc.wstack_degrid(stack, g, u, v, w, h["uv"].du, h["w"].dw, cuv, auv, ov, cw,
+                                    aw, ov)
+        p = rgamd.wstack_params(theta, lam, h["uv"].du, h["w"].dw, h["lm"].x0, auv, aw)
+        ctx.wstack_build(p, dev(l), dev(m), None, d["lm"], d["n"])
+        duv, dw_ = dev(cuv), dev(cw)
+        out = torch.empty(len(u), dtype=torch.complex128, device="cuda")
+        ctx.wstack_degrid(p, dev(u), dev(v), dev(w), rgamd.sep_kern_desc(duv, auv, ov),
+                          rgamd.sep_kern_desc(dw_, aw, ov), out)
+        torch.cuda.synchronize()
+        got = out.cpu().numpy()
+        assert np.abs(got - ref).max() < 1e-10 * np.abs(ref).max(), (auv, aw)
+        assert ctx.stats().n_dropped == nd
