@@ -360,10 +360,11 @@ def bench_wstack(ctx, steps, warmup, skip_cpu, cpu_seconds):
          "total_mvis_s": nv / ((build_ms + degrid_ms) * 1e-3) / 1e6,
          "error_vs_quantised_dft": {"max_abs": float(err.max()), "mean_abs": float(err.mean()),
                                     "per_source_mean": float(err.mean() / len(l))},
-         "roofline": {"kernel": "k_ws_degrid", "bound": "hbm", "achieved": ach,
-                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                      "bytes_per_unit": {"visibility": per_vis}, "traffic": None,
-                      "note": "taps served from cache count as algorithmic bytes"}}
+         "roofline": {"kernel": "k_ws_degrid_tile with the visibility sort (one call)",
+                      "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": ach / HBM_PEAK_GBS, "bytes_per_unit": {"visibility": per_vis},
+                      "traffic": None,
+                      "note": "taps served from LDS count as algorithmic bytes"}}
     log(f"[bench] W-stack predict: {og}^2 x {wp} planes, build {build_ms:.2f} ms, degrid "
         f"{degrid_ms:.2f} ms ({r['degrid_mvis_s']:.0f} Mvis/s), mean error per source "
         f"{r['error_vs_quantised_dft']['per_source_mean']:.2e}")
