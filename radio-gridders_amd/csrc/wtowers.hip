@@ -235,8 +235,9 @@ constexpr int BUF_RSRC_W3 = 0x00020000;
 // src = wstart[key] + (start - prefix[key]) .. + count of the binned array) is
 // counting-sorted by gx + S (bins < n + S <= SORT_BINS) into its own slot
 // [start, start + count) of the expanded array as band records of its band
-// b = key mod nb; the order inside a bin follows LDS atomics.  Workgroups
-// loop over the items.
+// b = key mod nb (items of <= 32 records by rank in one wavefront); the
+// order inside a bin is arbitrary.  The band pass relies on every item being
+// sorted.  Workgroups loop over the items.
 constexpr int SORT_BINS = 1024;
 
 template <typename T>
@@ -258,8 +259,25 @@ __global__ __launch_bounds__(256) void k_sort_items(const Rec<T> *__restrict__ i
         BandRec<T> *dst = out + item.start;
         const int Hb = H * (int)(item.key % (uint32_t)nb);
         if (item.count <= 32) {
-            if (threadIdx.x < item.count)
-                dst[threadIdx.x] = band_record(src[threadIdx.x], S, Hb, z);
+            // rank sort in the first wavefront: lane i counts the records
+            // ordered before its own (smaller column, or equal and earlier)
+            if (threadIdx.x < 64) {
+                const bool have = threadIdx.x < item.count;
+                Rec<T> r{};
+                uint32_t key = 0xffffffffu; // after every record
+                if (have) {
+                    r = src[threadIdx.x];
+                    key = min(r.pos & 0xffffu, (uint32_t)nbins - 1);
+                }
+                uint32_t rank = 0;
+#pragma unroll
+                for (int j = 0; j < 32; ++j) {
+                    const uint32_t kj = (uint32_t)__shfl((int)key, j);
+                    rank += (kj < key || (kj == key && j < (int)threadIdx.x)) ? 1u : 0u;
+                }
+                if (have)
+                    dst[rank] = band_record(r, S, Hb, z);
+            }
             continue;
         }
         __syncthreads(); // bins of the previous item are no longer read
@@ -451,7 +469,21 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
                                    (T2 *)nullptr);
             }
         };
+        // A lane's cell never decreases along the records (they are sorted by
+        // footprint column, and the cell is the first column >= gx of the
+        // lane's residue), so no lane changes cell in a batch iff every lane's
+        // last cell is its current one: then the batch is plain FMAs.
         auto update = [&](const Bt &bt) {
+            if (__all(bt.cell[U - 1] == cur)) {
+#pragma unroll
+                for (int q = 0; q < U; ++q) {
+                    sr = fma(bt.k[q].x, bt.v[q].x, sr);
+                    sr = fma(bt.k[q].y, bt.v[q].y, sr);
+                    si = fma(bt.k[q].x, bt.v[q].y, si);
+                    si = fma(-bt.k[q].y, bt.v[q].x, si);
+                }
+                return;
+            }
 #pragma unroll
             for (int q = 0; q < U; ++q) {
                 if (bt.cell[q] != cur) {
