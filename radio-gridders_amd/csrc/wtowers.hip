@@ -596,8 +596,14 @@ __global__ __launch_bounds__(256) void k_band_combine(const Item *__restrict__ i
 // one tower).  For every task the column block of its subgrid is inverse
 // FFT'd with the first stage reading straight from HBM (bands without records
 // are zero and are not read) and the last stage's outputs, which stay with
-// their thread, multiplied by exp(-2 pi i wp winc base) and accumulated in
-// registers; the chunk's sum is written to its partial image.
+// their thread, multiplied by E^wp = exp(-2 pi i wp winc base) and
+// accumulated in registers; the chunk's sum is written to its partial image.
+// Runs of up to SCREEN_RUN consecutive floors are summed by Horner's rule,
+// h = h conj(E) + val, and the run's sum h E^wp_last is added with E^wp_last
+// evaluated exactly: one screen evaluation per run instead of per floor, and
+// the rounding of E compounds over at most SCREEN_RUN - 1 floors.
+constexpr int SCREEN_RUN = 8;
+
 template <typename T, int LOGN, int CB, int H>
 __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
     const typename Vec2<T>::type *__restrict__ G, const uint32_t *__restrict__ key_start,
@@ -621,21 +627,40 @@ __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
     const int line = threadIdx.x % CB;
     const int col = blockIdx.x * CB + line;
 
-    // the thread's output pixels (row fft_out_index(slot), col)
+    // the thread's output pixels (row fft_out_index(slot), col): screen
+    // exponent, one-floor step conj(E), total and the current run's sum
     double base[PER];
-    T2 acc[PER];
+    T2 step[PER], acc[PER], h[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
         base[q] = screen_base(col, fft_out_index<LOGN, CB>(q), n, theta);
+        Trig<T>::rot(winc * base[q], step[q].x, step[q].y);
         acc[q].x = T(0);
         acc[q].y = T(0);
     }
+    // acc += h E^wl
+    auto close_run = [&](int wl) {
+        const double ph = -(double)wl * winc;
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            T cs, sn;
+            Trig<T>::rot(ph * base[q], cs, sn);
+            acc[q].x += h[q].x * cs - h[q].y * sn;
+            acc[q].y += h[q].x * sn + h[q].y * cs;
+        }
+    };
     static_assert(CB == GCB || (1 << LOGN) < GCB, "column pass reads whole column blocks");
+    int run = 0, wp_prev = 0;
     for (int k = first; k < first + count; ++k) {
         const T2 *g = G + (size_t)(k - t0) * n * n + (size_t)blockIdx.x * n * CB + line;
         // band b of the task was written iff it has work items
         const uint32_t *f = key_start + (size_t)task_slot[k] * NB;
-        const double ph0 = -(double)task_wp[k] * winc;
+        const int wp = task_wp[k];
+        if (run > 0 && (run == SCREEN_RUN || wp != wp_prev + 1)) {
+            close_run(wp_prev);
+            run = 0;
+        }
+        const bool extend = run > 0;
         auto load = [&](int, int row) {
             T2 val;
             val.x = T(0);
@@ -645,14 +670,21 @@ __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
             return val;
         };
         auto emit = [&](int q, int, T2 val) {
-            T cs, sn;
-            Trig<T>::rot(ph0 * base[q], cs, sn);
-            acc[q].x += val.x * cs - val.y * sn;
-            acc[q].y += val.x * sn + val.y * cs;
+            if (extend) { // h conj(E) + val
+                const T hx = h[q].x, hy = h[q].y;
+                h[q].x = fma(hx, step[q].x, fma(-hy, step[q].y, val.x));
+                h[q].y = fma(hx, step[q].y, fma(hy, step[q].x, val.y));
+            } else {
+                h[q] = val;
+            }
         };
         __syncthreads(); // the previous task's last stage has read lds
         fft_block_io<T2, LOGN, CB, CB, true>(lds, tw, load, emit);
+        ++run;
+        wp_prev = wp;
     }
+    if (run > 0)
+        close_run(wp_prev);
     T2 *pp = P + (size_t)blockIdx.y * n * n + col;
 #pragma unroll
     for (int q = 0; q < PER; ++q)
