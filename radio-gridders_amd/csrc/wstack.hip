@@ -74,17 +74,23 @@ struct WStackKeyer {
         return VIS_OK;
     }
 
+    // the record's vis.x bits
+    __device__ static double packed(const WsCell &c)
+    {
+        const unsigned long long b = (unsigned long long)c.ws |
+                                     ((unsigned long long)c.ovu << 16) |
+                                     ((unsigned long long)c.ovv << 32) |
+                                     ((unsigned long long)c.ovw << 48);
+        return __longlong_as_double((long long)b);
+    }
+
     // called for visibilities classify accepted
     template <typename T> __device__ typename Vec2<T>::type vis_value(int64_t i) const
     {
         WsCell c;
         cell(i, c);
-        const unsigned long long b = (unsigned long long)c.ws |
-                                     ((unsigned long long)c.ovu << 16) |
-                                     ((unsigned long long)c.ovv << 32) |
-                                     ((unsigned long long)c.ovw << 48);
         typename Vec2<T>::type r;
-        r.x = __longlong_as_double((long long)b);
+        r.x = packed(c);
         r.y = 0.0;
         return r;
     }
@@ -368,6 +374,271 @@ int wstack_build(rg_context *ctx, const rg_wstack_params *p, const double *l, co
     return RG_OK;
 }
 
+// ---- two-level sort of the visibilities --------------------------------------------------
+// The device-wide counting sort spends one device-scope atomic per
+// visibility when neighbouring visibilities do not share keys (random order),
+// so degridding sorts in two levels with LDS counters only:
+//   coarse count   : workgroup b counts its chunk of the stream per coarse
+//                    bucket c = key >> fb (at most WS_NC buckets) in LDS; each
+//                    visibility keeps (key, rank in its workgroup's share of c);
+//                    hist[c][b] = the share
+//   bucket scan    : hist[c][.] to exclusive prefixes over the workgroups,
+//                    bucket totals to bucket starts
+//   coarse scatter : record of visibility i to bstart[c] + hist[c][b] + rank,
+//                    its key kept in the record's spare word
+//   fine rank      : chunks of the bucketed records count their keys in LDS,
+//                    reserve each key's share with one device atomic per
+//                    (chunk, key) and give every record its rank in its key
+//   items          : key starts and work items of at most K records from the
+//                    per-key counts
+//   fine scatter   : record to key start + rank
+// Records of one key end up contiguous in key order, in no particular order
+// inside the key (degridding a visibility does not depend on the others).
+constexpr int WS_NC = 4096;       // coarse buckets at most
+constexpr int WS_SORT_BLOCKS = 2048; // workgroups of the coarse passes at most
+constexpr int WS_FB_MAX = 13;     // keys per bucket at most 2^13 (one fine window)
+
+// exclusive prefix of v over the 256 threads of the workgroup; total to all
+__device__ inline uint32_t block_prefix256(uint32_t v, uint32_t &total, uint32_t *tmp)
+{
+    tmp[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t t = threadIdx.x >= (unsigned)off ? tmp[threadIdx.x - off] : 0u;
+        __syncthreads();
+        tmp[threadIdx.x] += t;
+        __syncthreads();
+    }
+    total = tmp[255];
+    const uint32_t incl = tmp[threadIdx.x];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ __launch_bounds__(256) void k_ws_coarse_count(WStackKeyer kf, int64_t n, int64_t chunk,
+                                                         int nb, int fb, int nc, uint2 *kr,
+                                                         uint32_t *hist,
+                                                         unsigned long long *stats)
+{
+    __shared__ uint32_t h[WS_NC];
+    __shared__ uint32_t dropped;
+    for (int c = threadIdx.x; c < nc; c += 256)
+        h[c] = 0;
+    if (threadIdx.x == 0)
+        dropped = 0;
+    __syncthreads();
+    const int64_t i0 = (int64_t)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+    uint32_t nd = 0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
+        uint32_t key = 0, pos, kidx;
+        uint2 e = make_uint2(KEY_NONE, 0u);
+        if (kf.classify(i, key, pos, kidx) == VIS_OK) {
+            e.x = key;
+            e.y = atomicAdd(&h[key >> fb], 1u);
+        } else {
+            ++nd;
+        }
+        kr[i] = e;
+    }
+    if (nd)
+        atomicAdd(&dropped, nd);
+    __syncthreads();
+    for (int c = threadIdx.x; c < nc; c += 256)
+        hist[(size_t)c * nb + blockIdx.x] = h[c];
+    if (threadIdx.x == 0 && dropped)
+        atomicAdd(&stats[CNT_DROPPED], (unsigned long long)dropped);
+}
+
+// workgroup c: hist[c][0..nb) to its exclusive prefix, bucket total to tot[c]
+__global__ __launch_bounds__(256) void k_ws_bucket_scan(uint32_t *hist, int nb, uint32_t *tot)
+{
+    __shared__ uint32_t tmp[256];
+    uint32_t *hc = hist + (size_t)blockIdx.x * nb;
+    const int per = (nb + 255) / 256, b0 = (int)threadIdx.x * per;
+    uint32_t sum = 0;
+    for (int j = 0; j < per; ++j)
+        if (b0 + j < nb)
+            sum += hc[b0 + j];
+    uint32_t total;
+    uint32_t run = block_prefix256(sum, total, tmp);
+    for (int j = 0; j < per; ++j) {
+        if (b0 + j < nb) {
+            const uint32_t v = hc[b0 + j];
+            hc[b0 + j] = run;
+            run += v;
+        }
+    }
+    if (threadIdx.x == 0)
+        tot[blockIdx.x] = total;
+}
+
+// one workgroup: tot[0..nc) to bucket starts bstart[0..nc] (in place, nc + 1 entries)
+__global__ __launch_bounds__(256) void k_ws_bucket_starts(uint32_t *bstart, int nc)
+{
+    __shared__ uint32_t tmp[256];
+    const int per = (nc + 255) / 256, c0 = (int)threadIdx.x * per;
+    uint32_t sum = 0;
+    for (int j = 0; j < per; ++j)
+        if (c0 + j < nc)
+            sum += bstart[c0 + j];
+    uint32_t total;
+    uint32_t run = block_prefix256(sum, total, tmp);
+    for (int j = 0; j < per; ++j) {
+        if (c0 + j < nc) {
+            const uint32_t v = bstart[c0 + j];
+            bstart[c0 + j] = run;
+            run += v;
+        }
+    }
+    if (threadIdx.x == 0)
+        bstart[nc] = total;
+}
+
+__global__ __launch_bounds__(256) void k_ws_coarse_scatter(WStackKeyer kf, int64_t n,
+                                                           int64_t chunk, int nb, int fb,
+                                                           const uint2 *__restrict__ kr,
+                                                           const uint32_t *__restrict__ hist,
+                                                           const uint32_t *__restrict__ bstart,
+                                                           Rec<double> *tmp)
+{
+    const int64_t i0 = (int64_t)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
+        const uint2 e = kr[i];
+        if (e.x == KEY_NONE)
+            continue;
+        const uint32_t c = e.x >> fb;
+        Rec<double> r;
+        WsCell cl;
+        kf.cell(i, cl);
+        r.pos = (uint32_t)i;
+        r.kidx = (uint32_t)cl.us | ((uint32_t)cl.vs << 16);
+        r.vis.x = WStackKeyer::packed(cl);
+        r.vis.y = __longlong_as_double((long long)e.x);
+        tmp[bstart[c] + hist[(size_t)c * nb + blockIdx.x] + e.y] = r;
+    }
+}
+
+// Fine level over the bucketed records in chunks of WS_FINE_CHUNK (a chunk
+// may span buckets; buckets are key ranges in key order).  Chunk j counts
+// its records per key in LDS (window of WS_FINE_WIN keys from its first
+// bucket's base; a wider chunk, only possible when its buckets are sparse,
+// counts with device atomics instead), reserves each key's share with one
+// device atomic on cnt_key, and writes every record's rank inside its key.
+// The number of records, total = bstart[nc], is read on the device.
+constexpr int WS_FINE_CHUNK = 4096;
+constexpr int WS_FINE_WIN = 8192;
+
+__global__ __launch_bounds__(256) void k_ws_fine_rank(const Rec<double> *__restrict__ tmp,
+                                                      const uint32_t *__restrict__ total_p,
+                                                      int fb, uint32_t *cnt_key, uint32_t *rank)
+{
+    __shared__ uint32_t cnt[WS_FINE_WIN];
+    const uint32_t total = *total_p;
+    const uint32_t j0 = blockIdx.x * (uint32_t)WS_FINE_CHUNK;
+    if (j0 >= total)
+        return;
+    const uint32_t j1 = min(total, j0 + (uint32_t)WS_FINE_CHUNK);
+    auto key_of = [&](uint32_t j) { return (uint32_t)__double_as_longlong(tmp[j].vis.y); };
+    const uint32_t kbase = (key_of(j0) >> fb) << fb;
+    const uint64_t span = (uint64_t)((key_of(j1 - 1) >> fb) + 1) * (1u << fb) - kbase;
+    if (span > (uint64_t)WS_FINE_WIN) {
+        for (uint32_t j = j0 + threadIdx.x; j < j1; j += 256)
+            rank[j] = atomicAdd(&cnt_key[key_of(j)], 1u);
+        return;
+    }
+    const uint32_t S = (uint32_t)span;
+    for (uint32_t f = threadIdx.x; f < S; f += 256)
+        cnt[f] = 0;
+    __syncthreads();
+    for (uint32_t j = j0 + threadIdx.x; j < j1; j += 256)
+        rank[j] = atomicAdd(&cnt[key_of(j) - kbase], 1u); // rank inside the chunk's share
+    __syncthreads();
+    for (uint32_t f = threadIdx.x; f < S; f += 256) {
+        const uint32_t c = cnt[f];
+        if (c)
+            cnt[f] = atomicAdd(&cnt_key[kbase + f], c); // the share's first rank
+    }
+    __syncthreads();
+    for (uint32_t j = j0 + threadIdx.x; j < j1; j += 256)
+        rank[j] += cnt[key_of(j) - kbase];
+}
+
+// record j to key_start[key] + rank[j] (the key's records in no particular order)
+__global__ __launch_bounds__(256) void k_ws_fine_scatter(const Rec<double> *__restrict__ tmp,
+                                                         const uint32_t *__restrict__ total_p,
+                                                         const uint32_t *__restrict__ key_start,
+                                                         const uint32_t *__restrict__ rank,
+                                                         Rec<double> *recs)
+{
+    const uint32_t total = *total_p;
+    for (uint32_t j = blockIdx.x * 256u + threadIdx.x; j < total; j += gridDim.x * 256u) {
+        Rec<double> r = tmp[j];
+        const uint32_t key = (uint32_t)__double_as_longlong(r.vis.y);
+        r.vis.y = 0.0;
+        recs[key_start[key] + rank[j]] = r;
+    }
+}
+
+static int ws_sort(rg_context *ctx, const WStackKeyer &kf, int64_t n, int64_t nkeys, uint32_t K,
+                   hipStream_t s, BinResult &out)
+{
+    if (n >= (int64_t)0x7fffffff)
+        return fail(RG_ERR_PARAM, "visibility stream too long for one call (%lld)", (long long)n);
+    int fb = 0;
+    while (((nkeys + (1ll << fb) - 1) >> fb) > WS_NC)
+        ++fb;
+    if (fb > WS_FB_MAX)
+        return fail(RG_ERR_UNSUPPORTED, "W-stack key space too large (%lld)", (long long)nkeys);
+    const int nc = (int)((nkeys + (1ll << fb) - 1) >> fb);
+    // chunks of whole 256-visibility rounds, at most WS_SORT_BLOCKS of them
+    int64_t chunk = (n + WS_SORT_BLOCKS - 1) / WS_SORT_BLOCKS;
+    chunk = std::max<int64_t>(1024, (chunk + 255) / 256 * 256);
+    const int nb = (int)((n + chunk - 1) / chunk);
+    const int64_t max_items = nkeys + n / K + 1;
+
+    RG_TRY(ensure_counters(ctx));
+    RG_TRY(ctx->scratch[0].ensure(sizeof(uint32_t) * (size_t)nc * nb));      // hist
+    RG_TRY(ctx->scratch[1].ensure(sizeof(uint32_t) * (nkeys + 1)));          // vis_start
+    RG_TRY(ctx->scratch[2].ensure(sizeof(uint32_t) * (nkeys + 1)));          // item_start
+    RG_TRY(ctx->scratch[3].ensure(sizeof(uint32_t) * (nc + 1)));             // bucket starts
+    RG_TRY(ctx->scratch[4].ensure(sizeof(Item) * max_items));                // items
+    RG_TRY(ctx->scratch[5].ensure(sizeof(Rec<double>) * n));                 // records
+    RG_TRY(ctx->scratch[7].ensure(sizeof(Rec<double>) * n));                 // bucketed records
+    RG_TRY(ctx->scratch[8].ensure(sizeof(uint32_t) * nkeys));                // per-key counts
+    RG_TRY(ctx->scratch[15].ensure(sizeof(uint2) * n)); // (key, bucket rank), then key ranks
+    KernelTimer kt(ctx, RG_TIMER_BINNING, s);
+    uint32_t *hist = ctx->scratch[0].as<uint32_t>(), *bst = ctx->scratch[3].as<uint32_t>();
+    auto *stats = ctx->counters.as<unsigned long long>();
+    hipLaunchKernelGGL(k_ws_coarse_count, dim3((unsigned)nb), dim3(256), 0, s, kf, n, chunk, nb,
+                       fb, nc, ctx->scratch[15].as<uint2>(), hist, stats);
+    hipLaunchKernelGGL(k_ws_bucket_scan, dim3((unsigned)nc), dim3(256), 0, s, hist, nb, bst);
+    hipLaunchKernelGGL(k_ws_bucket_starts, dim3(1), dim3(256), 0, s, bst, nc);
+    hipLaunchKernelGGL(k_ws_coarse_scatter, dim3((unsigned)nb), dim3(256), 0, s, kf, n, chunk, nb,
+                       fb, (const uint2 *)ctx->scratch[15].as<uint2>(), (const uint32_t *)hist,
+                       (const uint32_t *)bst, ctx->scratch[7].as<Rec<double>>());
+    uint32_t *cnt_key = ctx->scratch[8].as<uint32_t>();
+    uint32_t *rank = ctx->scratch[15].as<uint32_t>(); // the (key, rank) pairs are no longer read
+    const Rec<double> *tmp = ctx->scratch[7].as<Rec<double>>();
+    RG_TRY(zero_fill(cnt_key, sizeof(uint32_t) * nkeys, s));
+    hipLaunchKernelGGL(k_ws_fine_rank, dim3((unsigned)((n + WS_FINE_CHUNK - 1) / WS_FINE_CHUNK)),
+                       dim3(256), 0, s, tmp, (const uint32_t *)(bst + nc), fb, cnt_key, rank);
+    RG_HIP(hipGetLastError());
+    RG_TRY(items_from_lengths(ctx, cnt_key, nkeys, K, ctx->scratch[1].as<uint32_t>(),
+                              ctx->scratch[2].as<uint32_t>(), ctx->scratch[4].as<Item>(), s));
+    hipLaunchKernelGGL(k_ws_fine_scatter,
+                       dim3((unsigned)std::min<int64_t>((n + 255) / 256, 65536)), dim3(256), 0, s,
+                       tmp, (const uint32_t *)(bst + nc),
+                       (const uint32_t *)ctx->scratch[1].as<uint32_t>(), (const uint32_t *)rank,
+                       ctx->scratch[5].as<Rec<double>>());
+    RG_HIP(hipGetLastError());
+    out.vis_start = ctx->scratch[1].as<uint32_t>();
+    out.item_start = ctx->scratch[2].as<uint32_t>();
+    out.items = ctx->scratch[4].as<Item>();
+    out.recs = ctx->scratch[5].p;
+    out.max_items = max_items;
+    return RG_OK;
+}
+
 int wstack_degrid(rg_context *ctx, const rg_wstack_params *p, const double *u, const double *v,
                   const double *w, int64_t nvis, const rg_sep_kern *conv_uv,
                   const rg_sep_kern *conv_w, void *out, hipStream_t s)
@@ -404,10 +675,7 @@ int wstack_degrid(rg_context *ctx, const rg_wstack_params *p, const double *u, c
     BinResult br;
     const int64_t nkeys = (int64_t)kf.ntx * kf.ntx * wp;
     constexpr uint32_t K = 1024; // records per work item
-    // arbitrary visibility order has no hot keys: one counter array (2 MiB
-    // at 524 288 keys) instead of replicas
-    RG_TRY((bin_visibilities<WStackKeyer, double>(ctx, kf, nvis, nkeys, K, s, br,
-                                                   (int64_t)1 << 20)));
+    RG_TRY(ws_sort(ctx, kf, nvis, nkeys, K, s, br));
     const Rec<double> *recs = reinterpret_cast<const Rec<double> *>(br.recs);
     const uint32_t *nrec = br.vis_start + nkeys;
     // one wavefront per visibility (the uv support, a power of two <= 64,

@@ -186,3 +186,44 @@ def test_other_supports_match_oracle(ctx, kernels):
         got = out.cpu().numpy()
         assert np.abs(got - ref).max() < 1e-10 * np.abs(ref).max(), (auv, aw)
         assert ctx.stats().n_dropped == nd
+
+
+@pytest.mark.parametrize("layout", ["sparse", "concentrated"])
+def test_sort_handles_sparse_and_concentrated_streams(ctx, kernels, layout):
+    """The degridding sort at a 1024^2 stack (24 576 (tile, plane) keys):
+    a few visibilities spread over the whole stack (chunks of the bucketed
+    records span more keys than the LDS window and rank with device atomics),
+    and one (u, v, w) repeated 50 000 times (every chunk reserves its share of
+    the same key)."""
+    theta, lam = 0.1, 5120.0
+    h, d = kernels["host"], kernels["desc"]
+    rng = np.random.default_rng(41)
+    l, m = card(theta)
+    g = int(np.floor(theta * lam))
+    reach = (g - h["uv"].size // 2 - 1) * h["uv"].du
+    if layout == "sparse":
+        nv = 6000
+        u = rng.uniform(-reach, reach, nv)
+        v = rng.uniform(-reach, reach, nv)
+        w = rng.uniform(-h["w"].dw, h["w"].dw, nv)
+    else:
+        nv = 50000
+        u, v, w = np.full(nv, 123.4), np.full(nv, -77.7), np.full(nv, 12.5)
+    stack, skipped = orc.wstack_build(theta, lam, l, m, None, h["uv"].du, h["w"].dw, h["w"].size,
+                                      h["lm"].x0, h["lm"].data, h["n"].data)
+    sel = slice(None) if layout == "sparse" else slice(0, 1)
+    ref, nd = orc.wstack_degrid(stack, g, u[sel], v[sel], w[sel], h["uv"].du, h["w"].dw,
+                                h["uv"].data, h["uv"].size, h["uv"].oversampling, h["w"].data,
+                                h["w"].size, h["w"].oversampling)
+    assert skipped == 0 and nd == 0
+    p = params(kernels, theta, lam)
+    ctx.wstack_build(p, dev(l), dev(m), None, d["lm"], d["n"])
+    out = torch.empty(nv, dtype=torch.complex128, device="cuda")
+    ctx.wstack_degrid(p, dev(u), dev(v), dev(w), d["uv"], d["w"], out)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    if layout == "concentrated":
+        ref = np.full(nv, ref[0])
+    assert np.abs(got - ref).max() < 1e-10 * np.abs(ref).max()
+    st = ctx.stats()
+    assert st.n_vis == nv and st.n_dropped == 0 and st.n_gridded == nv
