@@ -25,14 +25,19 @@ namespace rg {
 // centre tap (key = tile * w_planes + ws).  The
 // sort's scatter pass also evaluates everything a visibility's taps depend on
 // (ceil / floor of u / du, v / du, w / dw; deconvolve_visibility_,
-// wstack_common.h:430-460) and stores it packed in the record:
-//   pos = visibility index, kidx = us | vs << 16,
-//   vis bits = ws | ovu << 16 | ovv << 32 | ovw << 48
+// wstack_common.h:430-460) and stores it packed in a 16-byte record:
+//   pos = visibility index, cell = us | vs << 16,
+//   bits = ws | ovu << 16 | ovv << 32 | ovw << 48
 // (us, vs, ws the stack cell of the centre tap, ov* the oversampling rows,
 // all < 2^16).  Visibilities whose taps leave the stack are dropped here
 // (counted; their output stays 0).
 struct WsCell {
     int us, vs, ws, ovu, ovv, ovw;
+};
+
+struct alignas(16) WsRec {
+    uint32_t pos, cell;
+    unsigned long long bits;
 };
 
 struct WStackKeyer {
@@ -63,35 +68,38 @@ struct WStackKeyer {
         return true;
     }
 
+    __device__ static uint32_t key_of(int us, int vs, int ws, int ntx, int wp)
+    {
+        return (uint32_t)(((vs >> TILE_SHIFT) * ntx + (us >> TILE_SHIFT)) * wp + ws);
+    }
+
+    __device__ static uint32_t key_of(const WsRec &r, int ntx, int wp)
+    {
+        return key_of((int)(r.cell & 0xffffu), (int)(r.cell >> 16), (int)(r.bits & 0xffffu), ntx,
+                      wp);
+    }
+
     __device__ int classify(int64_t i, uint32_t &key, uint32_t &pos, uint32_t &kidx) const
     {
         WsCell c;
         if (!cell(i, c, false))
             return VIS_DROPPED;
-        key = (uint32_t)(((c.vs >> TILE_SHIFT) * ntx + (c.us >> TILE_SHIFT)) * wp + c.ws);
+        key = key_of(c.us, c.vs, c.ws, ntx, wp);
         pos = (uint32_t)i;
         kidx = (uint32_t)c.us | ((uint32_t)c.vs << 16);
         return VIS_OK;
     }
 
-    // the record's vis.x bits
-    __device__ static double packed(const WsCell &c)
-    {
-        const unsigned long long b = (unsigned long long)c.ws |
-                                     ((unsigned long long)c.ovu << 16) |
-                                     ((unsigned long long)c.ovv << 32) |
-                                     ((unsigned long long)c.ovw << 48);
-        return __longlong_as_double((long long)b);
-    }
-
-    // called for visibilities classify accepted
-    template <typename T> __device__ typename Vec2<T>::type vis_value(int64_t i) const
+    // the record of visibility i (one classify accepted)
+    __device__ WsRec record(int64_t i) const
     {
         WsCell c;
         cell(i, c);
-        typename Vec2<T>::type r;
-        r.x = packed(c);
-        r.y = 0.0;
+        WsRec r;
+        r.pos = (uint32_t)i;
+        r.cell = (uint32_t)c.us | ((uint32_t)c.vs << 16);
+        r.bits = (unsigned long long)c.ws | ((unsigned long long)c.ovu << 16) |
+                 ((unsigned long long)c.ovv << 32) | ((unsigned long long)c.ovw << 48);
         return r;
     }
 };
@@ -185,7 +193,7 @@ __global__ __launch_bounds__(256) void k_ws_sources(
 // neighbouring visibilities.
 template <int LV>
 __global__ __launch_bounds__(256) void k_ws_degrid(
-    const double2 *__restrict__ stack, int og, const Rec<double> *__restrict__ recs,
+    const double2 *__restrict__ stack, int og, const WsRec *__restrict__ recs,
     const uint32_t *__restrict__ nrec_p, int aa_uv, int aa_w,
     const double *__restrict__ conv_uv, const double *__restrict__ conv_w, double2 *out)
 {
@@ -202,15 +210,15 @@ __global__ __launch_bounds__(256) void k_ws_degrid(
     const int64_t r0 = min(nrec, group * chunk), r1 = min(nrec, r0 + chunk);
     const int aa_h = aa_uv / 2, aaw_h = aa_w / 2;
     const int t = lane % aa_uv, q0 = lane / aa_uv, qs = LV / aa_uv, npairs = aa_w * aa_uv;
-    Rec<double> next{};
+    WsRec next{};
     if (r0 < r1)
         next = recs[r0];
     for (int64_t k = r0; k < r1; ++k) { // uniform across the group
-        const Rec<double> rec = next;
+        const WsRec rec = next;
         if (k + 1 < r1)
             next = recs[k + 1];
-        const int us = (int)(rec.kidx & 0xffffu), vs = (int)(rec.kidx >> 16);
-        const unsigned long long b = (unsigned long long)__double_as_longlong(rec.vis.x);
+        const int us = (int)(rec.cell & 0xffffu), vs = (int)(rec.cell >> 16);
+        const unsigned long long b = rec.bits;
         const int ws = (int)(b & 0xffffu), ovu = (int)((b >> 16) & 0xffffu);
         const int ovv = (int)((b >> 32) & 0xffffu), ovw = (int)(b >> 48);
         const double gu = conv_uv[aa_uv * ovu + t];
@@ -245,7 +253,7 @@ __global__ __launch_bounds__(256) void k_ws_degrid(
 template <int AUV, int AW, int LV>
 __global__ __launch_bounds__(256) void k_ws_degrid_tile(
     const double2 *__restrict__ stack, int og, int wp, int ntx,
-    const Rec<double> *__restrict__ recs, const Item *__restrict__ items,
+    const WsRec *__restrict__ recs, const Item *__restrict__ items,
     const uint32_t *__restrict__ nitems_p, const double *__restrict__ conv_uv,
     const double *__restrict__ conv_w, double2 *out)
 {
@@ -283,17 +291,17 @@ __global__ __launch_bounds__(256) void k_ws_degrid_tile(
             double gu, gw[AW], gv[PER_LANE < AUV ? PER_LANE : AUV];
         };
         auto record = [&](int rd) {
-            Rec<double> r{};
+            WsRec r{};
             const uint32_t k = (uint32_t)(rd * GPB + grp);
             if (rd < rounds && k < item.count)
                 r = recs[item.start + k];
             return r;
         };
-        auto weights = [&](const Rec<double> &rec) {
+        auto weights = [&](const WsRec &rec) {
             Taps w;
             w.pos = rec.pos;
-            const int us = (int)(rec.kidx & 0xffffu), vs = (int)(rec.kidx >> 16);
-            const unsigned long long b = (unsigned long long)__double_as_longlong(rec.vis.x);
+            const int us = (int)(rec.cell & 0xffffu), vs = (int)(rec.cell >> 16);
+            const unsigned long long b = rec.bits;
             const int ovu = (int)((b >> 16) & 0xffffu);
             const int ovv = (int)((b >> 32) & 0xffffu), ovw = (int)(b >> 48);
             w.lr = vs - H - row0;
@@ -308,12 +316,12 @@ __global__ __launch_bounds__(256) void k_ws_degrid_tile(
                 w.gv[j] = conv_uv[AUV * ovv + (h + j * GROUPS) % AUV];
             return w;
         };
-        Rec<double> r1 = record(1);
+        WsRec r1 = record(1);
         Taps cur = weights(record(0));
         for (int rd = 0; rd < rounds; ++rd) { // uniform across the workgroup
             const uint32_t k = (uint32_t)(rd * GPB + grp);
             const bool live = k < item.count;
-            const Rec<double> r2 = record(rd + 2);
+            const WsRec r2 = record(rd + 2);
             const Taps nxt = weights(r1);
             double re = 0.0, im = 0.0;
             if (live) {
@@ -384,8 +392,7 @@ int wstack_build(rg_context *ctx, const rg_wstack_params *p, const double *l, co
 //                    hist[c][b] = the share
 //   bucket scan    : hist[c][.] to exclusive prefixes over the workgroups,
 //                    bucket totals to bucket starts
-//   coarse scatter : record of visibility i to bstart[c] + hist[c][b] + rank,
-//                    its key kept in the record's spare word
+//   coarse scatter : record of visibility i to bstart[c] + hist[c][b] + rank
 //   fine rank      : chunks of the bucketed records count their keys in LDS,
 //                    reserve each key's share with one device atomic per
 //                    (chunk, key) and give every record its rank in its key
@@ -499,7 +506,7 @@ __global__ __launch_bounds__(256) void k_ws_coarse_scatter(WStackKeyer kf, int64
                                                            const uint2 *__restrict__ kr,
                                                            const uint32_t *__restrict__ hist,
                                                            const uint32_t *__restrict__ bstart,
-                                                           Rec<double> *tmp)
+                                                           WsRec *tmp)
 {
     const int64_t i0 = (int64_t)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
     for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
@@ -507,14 +514,7 @@ __global__ __launch_bounds__(256) void k_ws_coarse_scatter(WStackKeyer kf, int64
         if (e.x == KEY_NONE)
             continue;
         const uint32_t c = e.x >> fb;
-        Rec<double> r;
-        WsCell cl;
-        kf.cell(i, cl);
-        r.pos = (uint32_t)i;
-        r.kidx = (uint32_t)cl.us | ((uint32_t)cl.vs << 16);
-        r.vis.x = WStackKeyer::packed(cl);
-        r.vis.y = __longlong_as_double((long long)e.x);
-        tmp[bstart[c] + hist[(size_t)c * nb + blockIdx.x] + e.y] = r;
+        tmp[bstart[c] + hist[(size_t)c * nb + blockIdx.x] + e.y] = kf.record(i);
     }
 }
 
@@ -528,9 +528,10 @@ __global__ __launch_bounds__(256) void k_ws_coarse_scatter(WStackKeyer kf, int64
 constexpr int WS_FINE_CHUNK = 4096;
 constexpr int WS_FINE_WIN = 8192;
 
-__global__ __launch_bounds__(256) void k_ws_fine_rank(const Rec<double> *__restrict__ tmp,
+__global__ __launch_bounds__(256) void k_ws_fine_rank(const WsRec *__restrict__ tmp,
                                                       const uint32_t *__restrict__ total_p,
-                                                      int fb, uint32_t *cnt_key, uint32_t *rank)
+                                                      int ntx, int wp, int fb, uint32_t *cnt_key,
+                                                      uint32_t *rank)
 {
     __shared__ uint32_t cnt[WS_FINE_WIN];
     const uint32_t total = *total_p;
@@ -538,7 +539,7 @@ __global__ __launch_bounds__(256) void k_ws_fine_rank(const Rec<double> *__restr
     if (j0 >= total)
         return;
     const uint32_t j1 = min(total, j0 + (uint32_t)WS_FINE_CHUNK);
-    auto key_of = [&](uint32_t j) { return (uint32_t)__double_as_longlong(tmp[j].vis.y); };
+    auto key_of = [&](uint32_t j) { return WStackKeyer::key_of(tmp[j], ntx, wp); };
     const uint32_t kbase = (key_of(j0) >> fb) << fb;
     const uint64_t span = (uint64_t)((key_of(j1 - 1) >> fb) + 1) * (1u << fb) - kbase;
     if (span > (uint64_t)WS_FINE_WIN) {
@@ -564,18 +565,17 @@ __global__ __launch_bounds__(256) void k_ws_fine_rank(const Rec<double> *__restr
 }
 
 // record j to key_start[key] + rank[j] (the key's records in no particular order)
-__global__ __launch_bounds__(256) void k_ws_fine_scatter(const Rec<double> *__restrict__ tmp,
+__global__ __launch_bounds__(256) void k_ws_fine_scatter(const WsRec *__restrict__ tmp,
                                                          const uint32_t *__restrict__ total_p,
+                                                         int ntx, int wp,
                                                          const uint32_t *__restrict__ key_start,
                                                          const uint32_t *__restrict__ rank,
-                                                         Rec<double> *recs)
+                                                         WsRec *recs)
 {
     const uint32_t total = *total_p;
     for (uint32_t j = blockIdx.x * 256u + threadIdx.x; j < total; j += gridDim.x * 256u) {
-        Rec<double> r = tmp[j];
-        const uint32_t key = (uint32_t)__double_as_longlong(r.vis.y);
-        r.vis.y = 0.0;
-        recs[key_start[key] + rank[j]] = r;
+        const WsRec r = tmp[j];
+        recs[key_start[WStackKeyer::key_of(r, ntx, wp)] + rank[j]] = r;
     }
 }
 
@@ -602,8 +602,8 @@ static int ws_sort(rg_context *ctx, const WStackKeyer &kf, int64_t n, int64_t nk
     RG_TRY(ctx->scratch[2].ensure(sizeof(uint32_t) * (nkeys + 1)));          // item_start
     RG_TRY(ctx->scratch[3].ensure(sizeof(uint32_t) * (nc + 1)));             // bucket starts
     RG_TRY(ctx->scratch[4].ensure(sizeof(Item) * max_items));                // items
-    RG_TRY(ctx->scratch[5].ensure(sizeof(Rec<double>) * n));                 // records
-    RG_TRY(ctx->scratch[7].ensure(sizeof(Rec<double>) * n));                 // bucketed records
+    RG_TRY(ctx->scratch[5].ensure(sizeof(WsRec) * n));                       // records
+    RG_TRY(ctx->scratch[7].ensure(sizeof(WsRec) * n));                       // bucketed records
     RG_TRY(ctx->scratch[8].ensure(sizeof(uint32_t) * nkeys));                // per-key counts
     RG_TRY(ctx->scratch[15].ensure(sizeof(uint2) * n)); // (key, bucket rank), then key ranks
     KernelTimer kt(ctx, RG_TIMER_BINNING, s);
@@ -615,21 +615,22 @@ static int ws_sort(rg_context *ctx, const WStackKeyer &kf, int64_t n, int64_t nk
     hipLaunchKernelGGL(k_ws_bucket_starts, dim3(1), dim3(256), 0, s, bst, nc);
     hipLaunchKernelGGL(k_ws_coarse_scatter, dim3((unsigned)nb), dim3(256), 0, s, kf, n, chunk, nb,
                        fb, (const uint2 *)ctx->scratch[15].as<uint2>(), (const uint32_t *)hist,
-                       (const uint32_t *)bst, ctx->scratch[7].as<Rec<double>>());
+                       (const uint32_t *)bst, ctx->scratch[7].as<WsRec>());
     uint32_t *cnt_key = ctx->scratch[8].as<uint32_t>();
     uint32_t *rank = ctx->scratch[15].as<uint32_t>(); // the (key, rank) pairs are no longer read
-    const Rec<double> *tmp = ctx->scratch[7].as<Rec<double>>();
+    const WsRec *tmp = ctx->scratch[7].as<WsRec>();
     RG_TRY(zero_fill(cnt_key, sizeof(uint32_t) * nkeys, s));
     hipLaunchKernelGGL(k_ws_fine_rank, dim3((unsigned)((n + WS_FINE_CHUNK - 1) / WS_FINE_CHUNK)),
-                       dim3(256), 0, s, tmp, (const uint32_t *)(bst + nc), fb, cnt_key, rank);
+                       dim3(256), 0, s, tmp, (const uint32_t *)(bst + nc), kf.ntx, kf.wp, fb,
+                       cnt_key, rank);
     RG_HIP(hipGetLastError());
     RG_TRY(items_from_lengths(ctx, cnt_key, nkeys, K, ctx->scratch[1].as<uint32_t>(),
                               ctx->scratch[2].as<uint32_t>(), ctx->scratch[4].as<Item>(), s));
     hipLaunchKernelGGL(k_ws_fine_scatter,
                        dim3((unsigned)std::min<int64_t>((n + 255) / 256, 65536)), dim3(256), 0, s,
-                       tmp, (const uint32_t *)(bst + nc),
+                       tmp, (const uint32_t *)(bst + nc), kf.ntx, kf.wp,
                        (const uint32_t *)ctx->scratch[1].as<uint32_t>(), (const uint32_t *)rank,
-                       ctx->scratch[5].as<Rec<double>>());
+                       ctx->scratch[5].as<WsRec>());
     RG_HIP(hipGetLastError());
     out.vis_start = ctx->scratch[1].as<uint32_t>();
     out.item_start = ctx->scratch[2].as<uint32_t>();
@@ -676,7 +677,7 @@ int wstack_degrid(rg_context *ctx, const rg_wstack_params *p, const double *u, c
     const int64_t nkeys = (int64_t)kf.ntx * kf.ntx * wp;
     constexpr uint32_t K = 1024; // records per work item
     RG_TRY(ws_sort(ctx, kf, nvis, nkeys, K, s, br));
-    const Rec<double> *recs = reinterpret_cast<const Rec<double> *>(br.recs);
+    const WsRec *recs = reinterpret_cast<const WsRec *>(br.recs);
     const uint32_t *nrec = br.vis_start + nkeys;
     // one wavefront per visibility (the uv support, a power of two <= 64,
     // divides it); a multiple of 8 workgroups (one share per XCD), at least
