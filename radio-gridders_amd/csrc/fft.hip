@@ -248,7 +248,8 @@ template int fft_rows_dispatch<double2, false>(double2 *, int, int64_t, const do
                                                const uint32_t *, hipStream_t);
 
 template <typename T2>
-static int fft2d_t(rg_context *ctx, T2 *data, int type, int n, int batch, int sign, hipStream_t s)
+static int fft2d_t(rg_context *ctx, T2 *data, int type, int n, int batch, int sign,
+                   const uint32_t *row_flags, hipStream_t s)
 {
     const void *twv = nullptr;
     RG_TRY(get_twiddles(ctx, n, type, sign, &twv));
@@ -257,10 +258,10 @@ static int fft2d_t(rg_context *ctx, T2 *data, int type, int n, int batch, int si
     const int64_t nrows = (int64_t)n * batch;
     if (n <= 512) {
         if (inv) {
-            RG_TRY((fft_rows_dispatch<T2, true>(data, n, nrows, tw, nullptr, s)));
+            RG_TRY((fft_rows_dispatch<T2, true>(data, n, nrows, tw, row_flags, s)));
             return fft_cols_dispatch<T2, true>(data, n, batch, tw, s);
         }
-        RG_TRY((fft_rows_dispatch<T2, false>(data, n, nrows, tw, nullptr, s)));
+        RG_TRY((fft_rows_dispatch<T2, false>(data, n, nrows, tw, row_flags, s)));
         return fft_cols_dispatch<T2, false>(data, n, batch, tw, s);
     }
     // rows, transpose, rows, transpose back
@@ -268,10 +269,11 @@ static int fft2d_t(rg_context *ctx, T2 *data, int type, int n, int batch, int si
     T2 *tmp = ctx->scratch[11].as<T2>();
     dim3 tg(n / 32, n / 32, batch);
     for (int pass = 0; pass < 2; ++pass) {
+        const uint32_t *flags = pass == 0 ? row_flags : nullptr;
         if (inv)
-            RG_TRY((fft_rows_dispatch<T2, true>(data, n, nrows, tw, nullptr, s)));
+            RG_TRY((fft_rows_dispatch<T2, true>(data, n, nrows, tw, flags, s)));
         else
-            RG_TRY((fft_rows_dispatch<T2, false>(data, n, nrows, tw, nullptr, s)));
+            RG_TRY((fft_rows_dispatch<T2, false>(data, n, nrows, tw, flags, s)));
         hipLaunchKernelGGL(k_transpose<T2>, tg, dim3(256), 0, s, (const T2 *)data, tmp, n);
         RG_HIP(hipGetLastError());
         std::swap(data, tmp);
@@ -280,15 +282,16 @@ static int fft2d_t(rg_context *ctx, T2 *data, int type, int n, int batch, int si
     return RG_OK;
 }
 
-int fft2d(rg_context *ctx, void *data, int type, int n, int batch, int sign, hipStream_t s)
+int fft2d(rg_context *ctx, void *data, int type, int n, int batch, int sign, hipStream_t s,
+          const uint32_t *row_flags)
 {
     if (n < 8 || n > 4096 || (n & (n - 1)))
         return fail(RG_ERR_PARAM, "FFT size %d unsupported (power of two 8..4096)", n);
     if (batch < 1 || (sign != 1 && sign != -1))
         return fail(RG_ERR_PARAM, "invalid FFT batch/sign");
     if (type == RG_C128)
-        return fft2d_t<double2>(ctx, (double2 *)data, type, n, batch, sign, s);
-    return fft2d_t<float2>(ctx, (float2 *)data, type, n, batch, sign, s);
+        return fft2d_t<double2>(ctx, (double2 *)data, type, n, batch, sign, row_flags, s);
+    return fft2d_t<float2>(ctx, (float2 *)data, type, n, batch, sign, row_flags, s);
 }
 
 } // namespace rg

@@ -162,7 +162,10 @@ int wproj_batched(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *v
                   int grid_type, hipStream_t s);
 int wtowers_run(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis,
                 const rg_wkern *wk, void *grid, int grid_type, hipStream_t s);
-int fft2d(rg_context *ctx, void *data, int type, int n, int batch, int sign, hipStream_t s);
+// row_flags (optional): bit r set for every row r (of the n * batch rows) that
+// may be nonzero on input; the first row pass skips the others, which must be 0
+int fft2d(rg_context *ctx, void *data, int type, int n, int batch, int sign, hipStream_t s,
+          const uint32_t *row_flags = nullptr);
 int fft_shift(void *data, int type, int n, int batch, hipStream_t s);
 int zero_fill(void *p, size_t bytes, hipStream_t s);
 int weight_vis(rg_context *ctx, const double *u, const double *v, int64_t n, void *vis,

@@ -134,7 +134,7 @@ __global__ __launch_bounds__(256) void k_ws_sources(
     double2 *stack, int og, int w_planes, int g, double theta, double lambda, double du, double dw,
     const double *__restrict__ pl, const double *__restrict__ pm, const double *__restrict__ flux,
     int npts, const double *__restrict__ corr_lm, int n_lm, const double *__restrict__ corr_n,
-    int n_n, unsigned long long *skipped)
+    int n_n, unsigned long long *skipped, uint32_t *row_flags)
 {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -164,13 +164,17 @@ __global__ __launch_bounds__(256) void k_ws_sources(
     a *= corr_lm[aau];
     a *= corr_lm[aav];
     a *= corr_n[aaw];
-    const double val = (flux ? flux[i] : 1.0) / a;
+    double val = (flux ? flux[i] : 1.0) / a;
     // the screen at the source's pixel (x = lc, y = mc of the inner region)
     const double l = theta * ((double)lc - g / 2) / g;
     const double m = theta * ((double)mc - g / 2) / g;
     const double ph = dw * (1.0 - sqrt(1.0 - l * l - m * m));
     const int gd = (og - g) / 2;
     const int xs = (gd + lc + og / 2) % og, ys = (gd + mc + og / 2) % og;
+    // the transform's output is fft-shifted by modulating its input with
+    // (-1)^(x + y) (og is even)
+    if ((xs + ys) & 1)
+        val = -val;
     const int t0 = -(w_planes / 2);
     for (int k = 0; k < w_planes; ++k) {
         double s, c;
@@ -178,6 +182,8 @@ __global__ __launch_bounds__(256) void k_ws_sources(
         double *dst = reinterpret_cast<double *>(stack + ((size_t)k * og + ys) * og + xs);
         atomicAdd(dst, val * c);
         atomicAdd(dst + 1, val * s);
+        const uint32_t row = (uint32_t)k * (uint32_t)og + (uint32_t)ys;
+        atomicOr(&row_flags[row >> 5], 1u << (row & 31));
     }
 }
 
@@ -366,16 +372,21 @@ int wstack_build(rg_context *ctx, const rg_wstack_params *p, const double *l, co
     RG_TRY(zero_fill(ctx->counters.p, sizeof(unsigned long long) * CNT__N, s));
     auto *cnt = ctx->counters.as<unsigned long long>();
     RG_TRY(zero_fill(stack, sizeof(double2) * plane * wp, s));
+    // rows holding a source (the first FFT pass skips the others)
+    const size_t flag_words = ((size_t)og * wp + 31) / 32;
+    RG_TRY(ctx->scratch[9].ensure(sizeof(uint32_t) * flag_words));
+    uint32_t *row_flags = ctx->scratch[9].as<uint32_t>();
+    RG_TRY(zero_fill(row_flags, sizeof(uint32_t) * flag_words, s));
     if (npts > 0) {
         hipLaunchKernelGGL(k_ws_sources, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s,
                            stack, og, wp, g, p->theta, p->lambda, p->du, p->dw, l, m, flux, npts,
                            corr_lm->data, corr_lm->size * corr_lm->oversampling, corr_n->data,
-                           corr_n->size * corr_n->oversampling, cnt + CNT_SKY_SKIPPED);
+                           corr_n->size * corr_n->oversampling, cnt + CNT_SKY_SKIPPED, row_flags);
         RG_HIP(hipGetLastError());
     }
-    // the sources sit at their fft-shifted positions already
-    RG_TRY(fft2d(ctx, stack, RG_C128, og, wp, -1, s));
-    RG_TRY(fft_shift(stack, RG_C128, og, wp, s));
+    // the sources sit at their fft-shifted positions with the output shift's
+    // modulation applied
+    RG_TRY(fft2d(ctx, stack, RG_C128, og, wp, -1, s, row_flags));
     ctx->ws_g = g;
     ctx->ws_og = og;
     ctx->ws_planes = wp;
