@@ -282,6 +282,25 @@ static int fft2d_t(rg_context *ctx, T2 *data, int type, int n, int batch, int si
     return RG_OK;
 }
 
+int fft_rows(rg_context *ctx, void *data, int type, int n, int64_t nrows, int sign, hipStream_t s)
+{
+    if (n < 8 || n > 4096 || (n & (n - 1)) || (sign != 1 && sign != -1))
+        return fail(RG_ERR_PARAM, "FFT size %d / sign %d unsupported", n, sign);
+    if (nrows <= 0)
+        return RG_OK;
+    const void *tw = nullptr;
+    RG_TRY(get_twiddles(ctx, n, type, sign, &tw));
+    if (type == RG_C128)
+        return sign > 0 ? fft_rows_dispatch<double2, true>((double2 *)data, n, nrows,
+                                                           (const double2 *)tw, nullptr, s)
+                        : fft_rows_dispatch<double2, false>((double2 *)data, n, nrows,
+                                                            (const double2 *)tw, nullptr, s);
+    return sign > 0 ? fft_rows_dispatch<float2, true>((float2 *)data, n, nrows, (const float2 *)tw,
+                                                      nullptr, s)
+                    : fft_rows_dispatch<float2, false>((float2 *)data, n, nrows,
+                                                       (const float2 *)tw, nullptr, s);
+}
+
 int fft2d(rg_context *ctx, void *data, int type, int n, int batch, int sign, hipStream_t s,
           const uint32_t *row_flags)
 {

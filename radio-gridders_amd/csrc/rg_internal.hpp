@@ -162,6 +162,8 @@ int wproj_batched(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *v
                   int grid_type, hipStream_t s);
 int wtowers_run(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis,
                 const rg_wkern *wk, void *grid, int grid_type, hipStream_t s);
+// in-place FFT of nrows contiguous rows of length n (power of two 8..4096)
+int fft_rows(rg_context *ctx, void *data, int type, int n, int64_t nrows, int sign, hipStream_t s);
 // row_flags (optional): bit r set for every row r (of the n * batch rows) that
 // may be nonzero on input; the first row pass skips the others, which must be 0
 int fft2d(rg_context *ctx, void *data, int type, int n, int batch, int sign, hipStream_t s,

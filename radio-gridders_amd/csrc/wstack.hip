@@ -104,6 +104,23 @@ struct WStackKeyer {
     }
 };
 
+// exclusive prefix of v over the 256 threads of the workgroup; total to all
+__device__ inline uint32_t block_prefix256(uint32_t v, uint32_t &total, uint32_t *tmp)
+{
+    tmp[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const uint32_t t = threadIdx.x >= (unsigned)off ? tmp[threadIdx.x - off] : 0u;
+        __syncthreads();
+        tmp[threadIdx.x] += t;
+        __syncthreads();
+    }
+    total = tmp[255];
+    const uint32_t incl = tmp[threadIdx.x];
+    __syncthreads();
+    return incl - v;
+}
+
 // ---- stack geometry -------------------------------------------------------------------------
 // wstack_predict (predict.cpp:410-428) and generate_sky (predict.cpp:228-231)
 int wstack_geometry(double theta, double lambda, double dw, int aa_w, double x0, int &g, int &og,
@@ -130,61 +147,181 @@ int wstack_geometry(double theta, double lambda, double dw, int aa_w, double x0,
 // predict.cpp:182-216).  A source whose pixel or correction index falls
 // outside its table is skipped and counted (the reference indexes out of
 // bounds).
-__global__ __launch_bounds__(256) void k_ws_sources(
-    double2 *stack, int og, int w_planes, int g, double theta, double lambda, double du, double dw,
-    const double *__restrict__ pl, const double *__restrict__ pm, const double *__restrict__ flux,
-    int npts, const double *__restrict__ corr_lm, int n_lm, const double *__restrict__ corr_n,
-    int n_n, unsigned long long *skipped, uint32_t *row_flags)
+struct WsSky {
+    int og, w_planes, g;
+    double theta, lambda, du, dw;
+    const double *pl, *pm, *flux;
+    const double *corr_lm, *corr_n;
+    int n_lm, n_n;
+};
+
+// Source i's position (xs, ys) in the planes, its value (with the output
+// shift's (-1)^(xs + ys) modulation, og being even) and its screen phase per
+// plane step; false when the source is skipped.
+__device__ inline bool ws_source(const WsSky &k, int i, int &xs, int &ys, double &val, double &ph)
 {
 #pragma clang fp contract(off)
+    const int g = k.g, og = k.og;
+    const double flc = floor((k.pl[i] / k.theta + 0.5) * (double)g);
+    const double fmc = floor((k.pm[i] / k.theta + 0.5) * (double)g);
+    if (!(flc >= 0 && flc < g && fmc >= 0 && fmc < g)) // also NaN
+        return false;
+    const int lc = (int)flc, mc = (int)fmc;
+    const double lq = (double)lc / k.lambda - k.theta / 2;
+    const double mq = (double)mc / k.lambda - k.theta / 2;
+    const double n = sqrt(1.0 - lq * lq - mq * mq) - 1.0;
+    const double lm_step = 1.0 / (double)k.n_lm, n_step = 1.0 / (double)k.n_n;
+    const double fu = floor((k.du * lq) / lm_step), fv = floor((k.du * mq) / lm_step);
+    const double fw = floor((k.dw * n) / n_step);
+    const bool ok = fabs(fu) < 1e9 && fabs(fv) < 1e9 && fabs(fw) < 1e9;
+    const int aau = ok ? (int)fu + k.n_lm / 2 : -1, aav = ok ? (int)fv + k.n_lm / 2 : -1;
+    const int aaw = ok ? (int)fw + k.n_n / 2 : -1;
+    if (!ok || aau < 0 || aau >= k.n_lm || aav < 0 || aav >= k.n_lm || aaw < 0 || aaw >= k.n_n)
+        return false;
+    double a = 1.0;
+    a *= k.corr_lm[aau];
+    a *= k.corr_lm[aav];
+    a *= k.corr_n[aaw];
+    val = (k.flux ? k.flux[i] : 1.0) / a;
+    // the screen at the source's pixel (x = lc, y = mc of the inner region)
+    const double l = k.theta * ((double)lc - g / 2) / g;
+    const double m = k.theta * ((double)mc - g / 2) / g;
+    ph = k.dw * (1.0 - sqrt(1.0 - l * l - m * m));
+    const int gd = (og - g) / 2;
+    xs = (gd + lc + og / 2) % og;
+    ys = (gd + mc + og / 2) % og;
+    if ((xs + ys) & 1)
+        val = -val;
+    return true;
+}
+
+// value of source (val, ph) in plane k: val exp(2 pi i (t0 + k) ph)
+__device__ inline double2 ws_plane_value(double val, double ph, int k, int w_planes)
+{
+    double s, c;
+    sincospi(2.0 * (double)(k - w_planes / 2) * ph, &s, &c);
+    return make_double2(val * c, val * s);
+}
+
+// Dense form: sources added into the zeroed stack; rows holding a source
+// flagged (row k og + ys), so that the first FFT pass can skip the others.
+__global__ __launch_bounds__(256) void k_ws_sources(WsSky sky, int npts, double2 *stack,
+                                                    unsigned long long *skipped,
+                                                    uint32_t *row_flags)
+{
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npts)
         return;
-    const double flc = floor((pl[i] / theta + 0.5) * (double)g);
-    const double fmc = floor((pm[i] / theta + 0.5) * (double)g);
-    if (!(flc >= 0 && flc < g && fmc >= 0 && fmc < g)) { // also NaN
+    int xs, ys;
+    double val, ph;
+    if (!ws_source(sky, i, xs, ys, val, ph)) {
         atomicAdd(skipped, 1ull);
         return;
     }
-    const int lc = (int)flc, mc = (int)fmc;
-    const double lq = (double)lc / lambda - theta / 2;
-    const double mq = (double)mc / lambda - theta / 2;
-    const double n = sqrt(1.0 - lq * lq - mq * mq) - 1.0;
-    const double lm_step = 1.0 / (double)n_lm, n_step = 1.0 / (double)n_n;
-    const double fu = floor((du * lq) / lm_step), fv = floor((du * mq) / lm_step);
-    const double fw = floor((dw * n) / n_step);
-    const bool ok = fabs(fu) < 1e9 && fabs(fv) < 1e9 && fabs(fw) < 1e9;
-    const int aau = ok ? (int)fu + n_lm / 2 : -1, aav = ok ? (int)fv + n_lm / 2 : -1;
-    const int aaw = ok ? (int)fw + n_n / 2 : -1;
-    if (!ok || aau < 0 || aau >= n_lm || aav < 0 || aav >= n_lm || aaw < 0 || aaw >= n_n) {
-        atomicAdd(skipped, 1ull);
-        return;
-    }
-    double a = 1.0;
-    a *= corr_lm[aau];
-    a *= corr_lm[aav];
-    a *= corr_n[aaw];
-    double val = (flux ? flux[i] : 1.0) / a;
-    // the screen at the source's pixel (x = lc, y = mc of the inner region)
-    const double l = theta * ((double)lc - g / 2) / g;
-    const double m = theta * ((double)mc - g / 2) / g;
-    const double ph = dw * (1.0 - sqrt(1.0 - l * l - m * m));
-    const int gd = (og - g) / 2;
-    const int xs = (gd + lc + og / 2) % og, ys = (gd + mc + og / 2) % og;
-    // the transform's output is fft-shifted by modulating its input with
-    // (-1)^(x + y) (og is even)
-    if ((xs + ys) & 1)
-        val = -val;
-    const int t0 = -(w_planes / 2);
-    for (int k = 0; k < w_planes; ++k) {
-        double s, c;
-        sincospi(2.0 * (double)(t0 + k) * ph, &s, &c);
+    const int og = sky.og;
+    for (int k = 0; k < sky.w_planes; ++k) {
+        const double2 v = ws_plane_value(val, ph, k, sky.w_planes);
         double *dst = reinterpret_cast<double *>(stack + ((size_t)k * og + ys) * og + xs);
-        atomicAdd(dst, val * c);
-        atomicAdd(dst + 1, val * s);
+        atomicAdd(dst, v.x);
+        atomicAdd(dst + 1, v.y);
         const uint32_t row = (uint32_t)k * (uint32_t)og + (uint32_t)ys;
         atomicOr(&row_flags[row >> 5], 1u << (row & 31));
     }
+}
+
+// ---- few sources: rows transformed in a side buffer, columns summed directly --------------
+// A plane's input is nonzero only on the rows ys holding a source (the same
+// rows in every plane).  With at most WS_SPARSE_ROWS sources those rows are
+// numbered (slot[ys] = j, rows[j] = ys), filled into buf[k][j] and row
+// transformed there; the column transform is then the direct sum
+//   plane[k][y][x] = sum_j buf[k][j][x] w^(rows[j] y mod og),  w = exp(-2 pi i / og),
+// written once over the whole stack (no zero fill, no transposes).
+constexpr int WS_SPARSE_ROWS = 64;
+
+__global__ __launch_bounds__(256) void k_ws_mark_rows(WsSky sky, int npts, uint32_t *row_used)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int xs, ys;
+    double val, ph;
+    if (i < npts && ws_source(sky, i, xs, ys, val, ph))
+        row_used[ys] = 1u;
+}
+
+// one workgroup of 256: slot of every used row in order, the row list and its length
+__global__ __launch_bounds__(256) void k_ws_row_slots(const uint32_t *row_used, int og,
+                                                      int32_t *slot, int32_t *rows,
+                                                      uint32_t *nrows)
+{
+    __shared__ uint32_t tmp[256];
+    const int per = (og + 255) / 256, y0 = (int)threadIdx.x * per;
+    uint32_t sum = 0;
+    for (int j = 0; j < per; ++j)
+        if (y0 + j < og)
+            sum += row_used[y0 + j] ? 1u : 0u;
+    uint32_t total;
+    uint32_t run = block_prefix256(sum, total, tmp);
+    for (int j = 0; j < per; ++j) {
+        const int y = y0 + j;
+        if (y < og) {
+            if (row_used[y]) {
+                slot[y] = (int32_t)run;
+                rows[run] = y;
+                ++run;
+            } else {
+                slot[y] = -1;
+            }
+        }
+    }
+    if (threadIdx.x == 0)
+        *nrows = total;
+}
+
+__global__ __launch_bounds__(256) void k_ws_sources_rows(WsSky sky, int npts,
+                                                         const int32_t *__restrict__ slot,
+                                                         int maxr, double2 *buf,
+                                                         unsigned long long *skipped)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts)
+        return;
+    int xs, ys;
+    double val, ph;
+    if (!ws_source(sky, i, xs, ys, val, ph)) {
+        atomicAdd(skipped, 1ull);
+        return;
+    }
+    const int og = sky.og, j = slot[ys];
+    for (int k = 0; k < sky.w_planes; ++k) {
+        const double2 v = ws_plane_value(val, ph, k, sky.w_planes);
+        double *dst = reinterpret_cast<double *>(buf + ((size_t)k * maxr + j) * og + xs);
+        atomicAdd(dst, v.x);
+        atomicAdd(dst + 1, v.y);
+    }
+}
+
+// grid (og * og / 256, w_planes): thread -> (y, x) of plane blockIdx.y
+__global__ __launch_bounds__(256) void k_ws_sparse_cols(const double2 *__restrict__ buf,
+                                                        const int32_t *__restrict__ rows,
+                                                        const uint32_t *__restrict__ nrows_p,
+                                                        int og, int maxr,
+                                                        const double2 *__restrict__ tw,
+                                                        double2 *stack)
+{
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)og * og)
+        return;
+    const int k = (int)blockIdx.y;
+    const int y = (int)(e / og), x = (int)(e - (int64_t)y * og);
+    const int nr = (int)*nrows_p;
+    const double2 *b = buf + (size_t)k * maxr * og + x;
+    double re = 0.0, im = 0.0;
+    for (int j = 0; j < nr; ++j) {
+        const double2 v = b[(size_t)j * og];
+        const double2 t = tw[(rows[j] * y) & (og - 1)];
+        re += v.x * t.x - v.y * t.y;
+        im += v.x * t.y + v.y * t.x;
+    }
+    stack[((size_t)k * og + y) * og + x] = make_double2(re, im);
 }
 
 // ---- degridding ------------------------------------------------------------------------------
@@ -371,22 +508,64 @@ int wstack_build(rg_context *ctx, const rg_wstack_params *p, const double *l, co
     RG_TRY(ensure_counters(ctx));
     RG_TRY(zero_fill(ctx->counters.p, sizeof(unsigned long long) * CNT__N, s));
     auto *cnt = ctx->counters.as<unsigned long long>();
-    RG_TRY(zero_fill(stack, sizeof(double2) * plane * wp, s));
-    // rows holding a source (the first FFT pass skips the others)
-    const size_t flag_words = ((size_t)og * wp + 31) / 32;
-    RG_TRY(ctx->scratch[9].ensure(sizeof(uint32_t) * flag_words));
-    uint32_t *row_flags = ctx->scratch[9].as<uint32_t>();
-    RG_TRY(zero_fill(row_flags, sizeof(uint32_t) * flag_words, s));
-    if (npts > 0) {
-        hipLaunchKernelGGL(k_ws_sources, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s,
-                           stack, og, wp, g, p->theta, p->lambda, p->du, p->dw, l, m, flux, npts,
-                           corr_lm->data, corr_lm->size * corr_lm->oversampling, corr_n->data,
-                           corr_n->size * corr_n->oversampling, cnt + CNT_SKY_SKIPPED, row_flags);
+    WsSky sky;
+    sky.og = og;
+    sky.w_planes = wp;
+    sky.g = g;
+    sky.theta = p->theta;
+    sky.lambda = p->lambda;
+    sky.du = p->du;
+    sky.dw = p->dw;
+    sky.pl = l;
+    sky.pm = m;
+    sky.flux = flux;
+    sky.corr_lm = corr_lm->data;
+    sky.corr_n = corr_n->data;
+    sky.n_lm = corr_lm->size * corr_lm->oversampling;
+    sky.n_n = corr_n->size * corr_n->oversampling;
+    const unsigned sblocks = (unsigned)((npts + 255) / 256);
+    if (npts <= WS_SPARSE_ROWS) {
+        // rows used, their slots, the row buffer (at most npts rows per plane)
+        const int maxr = npts > 0 ? npts : 1;
+        RG_TRY(ctx->scratch[9].ensure(sizeof(uint32_t) * (3 * (size_t)og + 1)));
+        uint32_t *used = ctx->scratch[9].as<uint32_t>();
+        int32_t *slot = reinterpret_cast<int32_t *>(used + og), *rows = slot + og;
+        uint32_t *nrows = reinterpret_cast<uint32_t *>(rows + og);
+        const size_t bufn = (size_t)wp * maxr * og;
+        RG_TRY(ctx->scratch[10].ensure(sizeof(double2) * bufn));
+        double2 *buf = ctx->scratch[10].as<double2>();
+        RG_TRY(zero_fill(used, sizeof(uint32_t) * og, s));
+        RG_TRY(zero_fill(buf, sizeof(double2) * bufn, s));
+        if (npts > 0)
+            hipLaunchKernelGGL(k_ws_mark_rows, dim3(sblocks), dim3(256), 0, s, sky, npts, used);
+        hipLaunchKernelGGL(k_ws_row_slots, dim3(1), dim3(256), 0, s, (const uint32_t *)used, og,
+                           slot, rows, nrows);
+        if (npts > 0)
+            hipLaunchKernelGGL(k_ws_sources_rows, dim3(sblocks), dim3(256), 0, s, sky, npts,
+                               (const int32_t *)slot, maxr, buf, cnt + CNT_SKY_SKIPPED);
         RG_HIP(hipGetLastError());
+        RG_TRY(fft_rows(ctx, buf, RG_C128, og, (int64_t)wp * maxr, -1, s));
+        const void *tw = nullptr;
+        RG_TRY(get_twiddles(ctx, og, RG_C128, -1, &tw));
+        hipLaunchKernelGGL(k_ws_sparse_cols, dim3((unsigned)((plane + 255) / 256), (unsigned)wp),
+                           dim3(256),
+                           0, s, (const double2 *)buf, (const int32_t *)rows,
+                           (const uint32_t *)nrows, og, maxr, (const double2 *)tw, stack);
+        RG_HIP(hipGetLastError());
+    } else {
+        RG_TRY(zero_fill(stack, sizeof(double2) * plane * wp, s));
+        // rows holding a source (the first FFT pass skips the others)
+        const size_t flag_words = ((size_t)og * wp + 31) / 32;
+        RG_TRY(ctx->scratch[9].ensure(sizeof(uint32_t) * flag_words));
+        uint32_t *row_flags = ctx->scratch[9].as<uint32_t>();
+        RG_TRY(zero_fill(row_flags, sizeof(uint32_t) * flag_words, s));
+        hipLaunchKernelGGL(k_ws_sources, dim3(sblocks), dim3(256), 0, s, sky, npts, stack,
+                           cnt + CNT_SKY_SKIPPED, row_flags);
+        RG_HIP(hipGetLastError());
+        // the sources sit at their fft-shifted positions with the output
+        // shift's modulation applied
+        RG_TRY(fft2d(ctx, stack, RG_C128, og, wp, -1, s, row_flags));
     }
-    // the sources sit at their fft-shifted positions with the output shift's
-    // modulation applied
-    RG_TRY(fft2d(ctx, stack, RG_C128, og, wp, -1, s, row_flags));
     ctx->ws_g = g;
     ctx->ws_og = og;
     ctx->ws_planes = wp;
@@ -415,23 +594,6 @@ int wstack_build(rg_context *ctx, const rg_wstack_params *p, const double *l, co
 constexpr int WS_NC = 4096;       // coarse buckets at most
 constexpr int WS_SORT_BLOCKS = 2048; // workgroups of the coarse passes at most
 constexpr int WS_FB_MAX = 13;     // keys per bucket at most 2^13 (one fine window)
-
-// exclusive prefix of v over the 256 threads of the workgroup; total to all
-__device__ inline uint32_t block_prefix256(uint32_t v, uint32_t &total, uint32_t *tmp)
-{
-    tmp[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        const uint32_t t = threadIdx.x >= (unsigned)off ? tmp[threadIdx.x - off] : 0u;
-        __syncthreads();
-        tmp[threadIdx.x] += t;
-        __syncthreads();
-    }
-    total = tmp[255];
-    const uint32_t incl = tmp[threadIdx.x];
-    __syncthreads();
-    return incl - v;
-}
 
 __global__ __launch_bounds__(256) void k_ws_coarse_count(WStackKeyer kf, int64_t n, int64_t chunk,
                                                          int nb, int fb, int nc, uint2 *kr,

@@ -49,8 +49,10 @@ def params(k, theta, lam):
                                h["w"].size)
 
 
-@pytest.mark.parametrize("lam,sky", [(640.0, "card"), (2560.0, "random")])
+@pytest.mark.parametrize("lam,sky", [(640.0, "card"), (2560.0, "random"), (1280.0, "many")])
 def test_predict_matches_oracle(ctx, kernels, lam, sky):
+    """Skies of up to 64 sources transform their source rows only and sum
+    the columns directly; larger ones (here 100 sources) run the 2-D FFT."""
     theta = 0.1
     h, d = kernels["host"], kernels["desc"]
     rng = np.random.default_rng(21)
@@ -58,9 +60,10 @@ def test_predict_matches_oracle(ctx, kernels, lam, sky):
         l, m = card(theta)
         flux = None
     else:
-        l = rng.uniform(-0.45 * theta, 0.45 * theta, 40)
-        m = rng.uniform(-0.45 * theta, 0.45 * theta, 40)
-        flux = rng.uniform(0.1, 1.0, 40)
+        ns = 40 if sky == "random" else 100
+        l = rng.uniform(-0.45 * theta, 0.45 * theta, ns)
+        m = rng.uniform(-0.45 * theta, 0.45 * theta, ns)
+        flux = rng.uniform(0.1, 1.0, ns)
     u, v, w = uvw(rng, theta, lam, h["w"].dw, 20000)
     stack, skipped = orc.wstack_build(theta, lam, l, m, flux, h["uv"].du, h["w"].dw, h["w"].size,
                                       h["lm"].x0, h["lm"].data, h["n"].data)
