@@ -294,9 +294,10 @@ def bench_wstack(ctx, steps, warmup, skip_cpu, cpu_seconds):
     """W-stacking predict at the reference test's workload (test/test_wstack.cpp:
     theta 0.1, lambda 20480, test-card sky, 40 M random visibilities with u, v
     in [-theta lambda/2, theta lambda/2) and w in [-dw, dw), the Sze-Tan
-    kernels of test/kernels/sze): stack build (sources, 4096^2 FFTs of every
-    plane) and degridding timed apart; degridding Mvis/s against the oracle's
-    OpenMP degridding of the same stack on a sample."""
+    kernels of test/kernels/sze): stack build (8 sources: their rows
+    transformed, columns summed directly into the 4096^2 planes) and
+    degridding (visibility sort included) timed apart; degridding Mvis/s
+    against the oracle's OpenMP degridding of the same stack on a sample."""
     import torch
 
     import rgamd

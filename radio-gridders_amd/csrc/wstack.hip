@@ -8,7 +8,8 @@
 // F^(t0 + k), F = exp(2 pi i dw (1 - sqrt(1 - l^2 - m^2))) on the inner
 // g x g region, t0 = -(w_planes / 2).  The sky is a sum of point sources at
 // their quantised pixels, so each plane's input is a zero plane plus one
-// value per source; one batched FFT then transforms every plane.
+// value per source: a few sources are transformed along their rows and then
+// summed directly over the columns, more sources by one batched 2-D FFT.
 //
 // Degridding: visibilities counting-sorted by uv tile, one wavefront per
 // visibility with lanes spread over the kernel's taps so that each load
