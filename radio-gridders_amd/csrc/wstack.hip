@@ -232,9 +232,10 @@ __global__ __launch_bounds__(256) void k_ws_sources(WsSky sky, int npts, double2
 
 // ---- few sources: rows transformed in a side buffer, columns summed directly --------------
 // A plane's input is nonzero only on the rows ys holding a source (the same
-// rows in every plane).  With at most WS_SPARSE_ROWS sources those rows are
-// numbered (slot[ys] = j, rows[j] = ys), filled into buf[k][j] and row
-// transformed there; the column transform is then the direct sum
+// rows in every plane).  With at most WS_SPARSE_ROWS sources (and og >= 8)
+// those rows are numbered (slot[ys] = j, rows[j] = ys), filled into
+// buf[k][j] and row transformed there; the column transform is then the
+// direct sum
 //   plane[k][y][x] = sum_j buf[k][j][x] w^(rows[j] y mod og),  w = exp(-2 pi i / og),
 // written once over the whole stack (no zero fill, no transposes).
 constexpr int WS_SPARSE_ROWS = 64;
@@ -300,7 +301,12 @@ __global__ __launch_bounds__(256) void k_ws_sources_rows(WsSky sky, int npts,
     }
 }
 
-// grid (og * og / 256, w_planes): thread -> (y, x) of plane blockIdx.y
+// grid (og * og / (256 WS_COL_ROWS), w_planes): thread -> rows y0 .. y0 +
+// WS_COL_ROWS - 1 of column x of plane blockIdx.y; per source row the
+// twiddle w^(row y) starts from the table at y0 and steps by w^row (at most
+// WS_COL_ROWS - 1 FP64 roundings)
+constexpr int WS_COL_ROWS = 8;
+
 __global__ __launch_bounds__(256) void k_ws_sparse_cols(const double2 *__restrict__ buf,
                                                         const int32_t *__restrict__ rows,
                                                         const uint32_t *__restrict__ nrows_p,
@@ -309,20 +315,35 @@ __global__ __launch_bounds__(256) void k_ws_sparse_cols(const double2 *__restric
                                                         double2 *stack)
 {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e >= (int64_t)og * og)
+    if (e >= (int64_t)og * (og / WS_COL_ROWS))
         return;
     const int k = (int)blockIdx.y;
-    const int y = (int)(e / og), x = (int)(e - (int64_t)y * og);
+    const int yg = (int)(e / og), x = (int)(e - (int64_t)yg * og);
+    const int y0 = yg * WS_COL_ROWS, mask = og - 1;
     const int nr = (int)*nrows_p;
     const double2 *b = buf + (size_t)k * maxr * og + x;
-    double re = 0.0, im = 0.0;
+    double re[WS_COL_ROWS], im[WS_COL_ROWS];
+#pragma unroll
+    for (int i = 0; i < WS_COL_ROWS; ++i)
+        re[i] = im[i] = 0.0;
     for (int j = 0; j < nr; ++j) {
         const double2 v = b[(size_t)j * og];
-        const double2 t = tw[(rows[j] * y) & (og - 1)];
-        re += v.x * t.x - v.y * t.y;
-        im += v.x * t.y + v.y * t.x;
+        const int r = rows[j];
+        double2 t = tw[(r * y0) & mask];
+        const double2 st = tw[r & mask];
+#pragma unroll
+        for (int i = 0; i < WS_COL_ROWS; ++i) {
+            re[i] += v.x * t.x - v.y * t.y;
+            im[i] += v.x * t.y + v.y * t.x;
+            const double tx = t.x * st.x - t.y * st.y;
+            t.y = t.x * st.y + t.y * st.x;
+            t.x = tx;
+        }
     }
-    stack[((size_t)k * og + y) * og + x] = make_double2(re, im);
+    double2 *dst = stack + ((size_t)k * og + y0) * og + x;
+#pragma unroll
+    for (int i = 0; i < WS_COL_ROWS; ++i)
+        dst[(size_t)i * og] = make_double2(re[i], im[i]);
 }
 
 // ---- degridding ------------------------------------------------------------------------------
@@ -525,7 +546,7 @@ int wstack_build(rg_context *ctx, const rg_wstack_params *p, const double *l, co
     sky.n_lm = corr_lm->size * corr_lm->oversampling;
     sky.n_n = corr_n->size * corr_n->oversampling;
     const unsigned sblocks = (unsigned)((npts + 255) / 256);
-    if (npts <= WS_SPARSE_ROWS) {
+    if (npts <= WS_SPARSE_ROWS && og >= WS_COL_ROWS) {
         // rows used, their slots, the row buffer (at most npts rows per plane)
         const int maxr = npts > 0 ? npts : 1;
         RG_TRY(ctx->scratch[9].ensure(sizeof(uint32_t) * (3 * (size_t)og + 1)));
@@ -548,7 +569,8 @@ int wstack_build(rg_context *ctx, const rg_wstack_params *p, const double *l, co
         RG_TRY(fft_rows(ctx, buf, RG_C128, og, (int64_t)wp * maxr, -1, s));
         const void *tw = nullptr;
         RG_TRY(get_twiddles(ctx, og, RG_C128, -1, &tw));
-        hipLaunchKernelGGL(k_ws_sparse_cols, dim3((unsigned)((plane + 255) / 256), (unsigned)wp),
+        hipLaunchKernelGGL(k_ws_sparse_cols,
+                           dim3((unsigned)((plane / WS_COL_ROWS + 255) / 256), (unsigned)wp),
                            dim3(256),
                            0, s, (const double2 *)buf, (const int32_t *)rows,
                            (const uint32_t *)nrows, og, maxr, (const double2 *)tw, stack);
