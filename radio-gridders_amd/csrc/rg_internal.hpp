@@ -113,7 +113,13 @@ struct rg_context {
     int64_t last_nvis = 0;
     // geometry of the W-stack held in scratch[17] (0 = none)
     int ws_g = 0, ws_og = 0, ws_planes = 0;
-    // generic scratch arrays
+    // Scratch arrays, reused by every call on the context (calls on one
+    // context run one after another on their stream):
+    //   0-6, 15  visibility sort (binning.hpp; W-stack sort: 0-5, 7, 8, 15)
+    //   7-11     W-towers batch buffers; 9-10 also the W-stack build's row
+    //            flags / row buffer, 11 the FFT transposes
+    //   13       channel offsets, 14 DFT, 16 uniform-weighting counts
+    //   17       the W-stack, kept between calls
     rg::DevBuf scratch[18];
     // channel offsets resident in scratch[13] (uploaded only when they change,
     // so a repeated multi-channel call issues no copy and can be captured)
