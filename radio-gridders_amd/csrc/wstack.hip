@@ -346,6 +346,26 @@ __global__ __launch_bounds__(256) void k_ws_sparse_cols(const double2 *__restric
         dst[(size_t)i * og] = make_double2(re[i], im[i]);
 }
 
+// Sum over the 16 lanes of a DPP row (lanes 16 r .. 16 r + 15), returned in
+// every lane of the row: rotations by 8 and 4 inside the row, then swaps of
+// lanes 2 and 1 apart inside quads; register moves only (a shuffle through
+// ds_bpermute would take an LDS instruction per 32-bit half).  All lanes of
+// the row must be active.
+__device__ inline double row16_sum(double x)
+{
+    auto step = [&](auto ctrl) {
+        constexpr int c = decltype(ctrl)::value;
+        const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), c, 0xf, 0xf, false);
+        const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), c, 0xf, 0xf, false);
+        x += __hiloint2double(hi, lo);
+    };
+    step(std::integral_constant<int, 0x128>{}); // row_ror:8
+    step(std::integral_constant<int, 0x124>{}); // row_ror:4
+    step(std::integral_constant<int, 0x4e>{});  // quad_perm [2, 3, 0, 1]
+    step(std::integral_constant<int, 0xb1>{});  // quad_perm [1, 0, 3, 2]
+    return x;
+}
+
 // ---- degridding ------------------------------------------------------------------------------
 // deconvolve_visibility_ (wstack_common.h:420-470) over the tile-sorted
 // records.  LV lanes per visibility: tap t = lane % aa_uv along u, pair
@@ -503,10 +523,15 @@ __global__ __launch_bounds__(256) void k_ws_degrid_tile(
                     im += sv.y * kk;
                 }
             }
+            if constexpr (LV == 16) {
+                re = row16_sum(re);
+                im = row16_sum(im);
+            } else {
 #pragma unroll
-            for (int o = LV / 2; o > 0; o >>= 1) {
-                re += __shfl_xor(re, o, LV);
-                im += __shfl_xor(im, o, LV);
+                for (int o = LV / 2; o > 0; o >>= 1) {
+                    re += __shfl_xor(re, o, LV);
+                    im += __shfl_xor(im, o, LV);
+                }
             }
             if (live && lane == 0)
                 out[cur.pos] = make_double2(re, im);
