@@ -470,10 +470,13 @@ __global__ __launch_bounds__(256) void k_ws_degrid_tile(
         // two-deep pipeline: round rd sums record rd with the weights loaded
         // in round rd - 1, loads the weights of record rd + 1 (read in round
         // rd - 1) and reads record rd + 2
+        // a lane's pairs are every plane pw with rows pv = h + b GROUPS (b < PV)
+        constexpr int PV = AUV / GROUPS;
+        static_assert(AUV % GROUPS == 0 && PER_LANE == AW * PV, "pair layout");
         struct Taps {
             uint32_t pos;
             int lr, lc; // local row / column of tap (-H, -H) + (0, t)
-            double gu, gw[AW], gv[PER_LANE < AUV ? PER_LANE : AUV];
+            double gu, gw[AW], gv[PV];
         };
         auto record = [&](int rd) {
             WsRec r{};
@@ -495,10 +498,9 @@ __global__ __launch_bounds__(256) void k_ws_degrid_tile(
 #pragma unroll
             for (int pw = 0; pw < AW; ++pw)
                 w.gw[pw] = conv_w[AW * ovw + pw];
-            // the rows a lane visits: pv = (h + j GROUPS) mod AUV
 #pragma unroll
-            for (int j = 0; j < (PER_LANE < AUV ? PER_LANE : AUV); ++j)
-                w.gv[j] = conv_uv[AUV * ovv + (h + j * GROUPS) % AUV];
+            for (int b = 0; b < PV; ++b)
+                w.gv[b] = conv_uv[AUV * ovv + h + b * GROUPS];
             return w;
         };
         WsRec r1 = record(1);
@@ -510,18 +512,22 @@ __global__ __launch_bounds__(256) void k_ws_degrid_tile(
             const Taps nxt = weights(r1);
             double re = 0.0, im = 0.0;
             if (live) {
-                const double2 *base = reg + cur.lr * R + cur.lc;
+                // gu sum_pw gw[pw] sum_pv gv[pv] stack[pw][pv] (the weights factor)
+                const double2 *base = reg + cur.lr * R + cur.lc + h * R;
 #pragma unroll
-                for (int j = 0; j < PER_LANE; ++j) {
-                    const int q = h + j * GROUPS;
-                    const int pw = q / AUV, pv = q % AUV; // dwi + AW/2, dvi + AUV/2
-                    constexpr int NV = PER_LANE < AUV ? PER_LANE : AUV;
-                    const double gv = cur.gw[pw] * cur.gv[j % NV];
-                    const double2 sv = base[(pw * R + pv) * R];
-                    const double kk = gv * cur.gu;
-                    re += sv.x * kk;
-                    im += sv.y * kk;
+                for (int pw = 0; pw < AW; ++pw) { // dwi + AW/2
+                    double ir = 0.0, ii = 0.0;
+#pragma unroll
+                    for (int b = 0; b < PV; ++b) { // dvi + AUV/2 = h + b GROUPS
+                        const double2 sv = base[(pw * R + b * GROUPS) * R];
+                        ir = fma(sv.x, cur.gv[b], ir);
+                        ii = fma(sv.y, cur.gv[b], ii);
+                    }
+                    re = fma(ir, cur.gw[pw], re);
+                    im = fma(ii, cur.gw[pw], im);
                 }
+                re *= cur.gu;
+                im *= cur.gu;
             }
             if constexpr (LV == 16) {
                 re = row16_sum(re);
