@@ -116,8 +116,9 @@ struct rg_context {
     // Scratch arrays, reused by every call on the context (calls on one
     // context run one after another on their stream):
     //   0-6, 15  visibility sort (binning.hpp; W-stack sort: 0-5, 7, 8, 15)
-    //   7-11     W-towers batch buffers; 9-10 also the W-stack build's row
-    //            flags / row buffer, 11 the FFT transposes
+    //   7-12     W-towers batch buffers (12: the padded kernel table); 9-10
+    //            also the W-stack build's row flags / row buffer, 11 the FFT
+    //            transposes
     //   13       channel offsets, 14 DFT, 16 uniform-weighting counts
     //   17       the W-stack, kept between calls
     rg::DevBuf scratch[18];

@@ -187,9 +187,12 @@ __device__ inline size_t gcb_index(int row, int col, int n)
 //   xy = gz | cy0 << 16, gz = (gx + S) z the footprint's first column in the
 //        band's row layout (S elements of left pad), cy0 = H b - gy (signed)
 //        the footprint row that band row 0 meets;
-//   kb = (kidx S^2 + cy0 S) z, byte offset of that footprint row's first tap
-//        in the kernel table (negative values are never dereferenced: a lane
-//        whose row the footprint misses reads no tap).
+//   kb = (kidx PS S + (min(cy0, S) + H - 1) S) z, byte offset of that
+//        footprint row's first tap in the padded kernel table (every S x S
+//        sub-kernel stored with H - 1 zero rows above and H below, PS =
+//        S + 2 H - 1 rows): band row l reads padded row min(cy0, S) + H - 1 + l,
+//        a zero row whenever the footprint misses it, always inside its own
+//        sub-kernel.
 template <typename T> struct BandRec {
     uint32_t xy;
     int32_t kb;
@@ -199,23 +202,44 @@ static_assert(sizeof(BandRec<float>) == sizeof(Rec<float>) &&
                   sizeof(BandRec<double>) == sizeof(Rec<double>),
               "band records replace binned records one for one");
 
-// xy of a padding record: cy0 out of reach of every band row
-constexpr uint32_t BAND_XY_NONE = 0x40000000u;
+
+__host__ __device__ constexpr int padded_rows(int S, int H) { return S + 2 * H - 1; }
 
 template <typename T>
-__device__ inline BandRec<T> band_record(const Rec<T> &r, int S, int Hb, int z)
+__device__ inline BandRec<T> band_record(const Rec<T> &r, int S, int H, int Hb, int z)
 {
     const int gx = (int)(r.pos & 0xffffu) - S, gy = (int)(r.pos >> 16) - S;
-    const int cy0 = Hb - gy;
+    const int cy0 = Hb - gy; // >= 1 - H for every record of band Hb / H's window
     BandRec<T> o;
     o.xy = (uint32_t)((gx + S) * z) | ((uint32_t)cy0 << 16);
-    o.kb = ((int32_t)r.kidx * S * S + cy0 * S) * z;
+    o.kb = ((int32_t)r.kidx * padded_rows(S, H) * S + (min(cy0, S) + H - 1) * S) * z;
     o.vis = r.vis;
     return o;
 }
 
-// Kernel taps through a buffer resource: a lane whose offset lies beyond the
-// table (the lanes a footprint misses get 0x80000000) reads zeros.
+// Kernel table for the band pass: sub-kernel k (S x S) at k PS S with H - 1
+// zero rows above and H below (PS = S + 2 H - 1 rows).
+template <typename T2>
+__global__ __launch_bounds__(256) void k_pad_kernels(const T2 *__restrict__ src, T2 *dst,
+                                                     int64_t nsub, int S, int H)
+{
+    const int PS = padded_rows(S, H);
+    const int64_t total = nsub * PS * S;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * 256) {
+        const int64_t k = e / (PS * S);
+        const int rr = (int)(e - k * PS * S);
+        const int row = rr / S - (H - 1), col = rr % S;
+        T2 v;
+        v.x = 0;
+        v.y = 0;
+        if (row >= 0 && row < S)
+            v = src[(k * S + row) * S + col];
+        dst[e] = v;
+    }
+}
+
+// Kernel taps through a buffer resource (offsets beyond the table read zeros).
 __device__ inline float2 tap_load(__amdgpu_buffer_rsrc_t r, uint32_t off, float2 *)
 {
     return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
@@ -224,7 +248,6 @@ __device__ inline double2 tap_load(__amdgpu_buffer_rsrc_t r, uint32_t off, doubl
 {
     return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
 }
-constexpr uint32_t TAP_NONE = 0x80000000u;
 // word 3 of a raw (unswizzled, stride 0) buffer resource on gfx9
 constexpr int BUF_RSRC_W3 = 0x00020000;
 
@@ -276,7 +299,7 @@ __global__ __launch_bounds__(256) void k_sort_items(const Rec<T> *__restrict__ i
                     rank += (kj < key || (kj == key && j < (int)threadIdx.x)) ? 1u : 0u;
                 }
                 if (have)
-                    dst[rank] = band_record(r, S, Hb, z);
+                    dst[rank] = band_record(r, S, H, Hb, z);
             }
             continue;
         }
@@ -318,7 +341,7 @@ __global__ __launch_bounds__(256) void k_sort_items(const Rec<T> *__restrict__ i
         for (uint32_t i = threadIdx.x; i < item.count; i += 256) {
             const Rec<T> r = src[i];
             const uint32_t d = atomicAdd(&bins[min(r.pos & 0xffffu, (uint32_t)nbins - 1)], 1u);
-            dst[d] = band_record(r, S, Hb, z);
+            dst[d] = band_record(r, S, H, Hb, z);
         }
     }
 }
@@ -401,7 +424,9 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
         if (base + threadIdx.x < item.count) {
             r = recs[item.start + base + threadIdx.x];
         } else {
-            r.xy = BAND_XY_NONE;
+            // padding: column n, after every footprint column (its cells lie
+            // in the row gaps), no visibility
+            r.xy = (uint32_t)((n + S) * z);
             r.kb = 0;
             r.vis.x = T(0);
             r.vis.y = T(0);
@@ -424,7 +449,7 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
     // per lane, in bytes: column residue, tap row, cell row
     const int uz = myU * z, tap_row = myV * S * z, cell_row = myV * BS * z;
     char *band_b = reinterpret_cast<char *>(band);
-    int cur = -1, carry = -1;
+    int cur = -1;
     T sr = T(0), si = T(0);
     auto flush = [&]() {
         T *c = reinterpret_cast<T *>(band_b + cur);
@@ -451,22 +476,19 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
         const int ng = min(RG, max(0, (int)nloc - grp * RG));
         const BandRec<T> *gr = s_rec + grp * RG;
         using Bt = RecBatch<T, U>;
-        // A lane whose row the footprint misses takes a zero tap and keeps its
-        // previous cell, so it adds nothing and never flushes: the update
-        // needs no second branch.
+        // A lane whose row the footprint misses reads a zero tap from the
+        // padding of the record's sub-kernel; its cell follows the footprint
+        // column like every other lane's, so it changes cell with them and
+        // flushes only when it has summed something.
         auto fetch = [&](uint32_t q0, Bt &bt) {
 #pragma unroll
             for (int q = 0; q < U; ++q) {
                 const BandRec<T> r = gr[q0 + q];
                 const int gz = (int)(r.xy & 0xffffu);
-                const int cy = ((int)r.xy >> 16) + myV;
                 const int cxz = (uz - gz) & ((S - 1) * z);
-                const bool in = (unsigned)cy < (unsigned)S;
-                carry = in ? cell_row + gz + cxz : carry;
-                bt.cell[q] = carry;
+                bt.cell[q] = cell_row + gz + cxz;
                 bt.v[q] = r.vis;
-                bt.k[q] = tap_load(ktab, in ? (uint32_t)(r.kb + tap_row + cxz) : TAP_NONE,
-                                   (T2 *)nullptr);
+                bt.k[q] = tap_load(ktab, (uint32_t)(r.kb + tap_row + cxz), (T2 *)nullptr);
             }
         };
         // A lane's cell never decreases along the records (they are sorted by
@@ -487,7 +509,7 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
 #pragma unroll
             for (int q = 0; q < U; ++q) {
                 if (bt.cell[q] != cur) {
-                    if (cur >= 0)
+                    if (cur >= 0 && (sr != T(0) || si != T(0)))
                         flush();
                     cur = bt.cell[q];
                     sr = T(0);
@@ -502,7 +524,7 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
         };
         pipelined_records<U, Bt>((uint32_t)((ng + U - 1) / U * U), fetch, update);
     }
-    if (cur >= 0)
+    if (cur >= 0 && (sr != T(0) || si != T(0)))
         flush();
 
     // destination: the task's subgrid band (column blocks), or a spill slot
@@ -1257,14 +1279,22 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
         bl.items = items_w;
         bl.item_start = item_start_w;
         bl.rank_of_slot = d_tab + o_ros;
-        bl.kern = wk->data;
         {
-            // taps are addressed by 32-bit byte offsets (buffer loads)
-            const int64_t kb = (int64_t)wk->plane_count * wk->oversampling * wk->oversampling * S * S *
-                               (int64_t)sizeof(T2);
+            // the band pass reads the table with zero rows around every
+            // sub-kernel; taps are addressed by 32-bit byte offsets
+            const int64_t nsub = (int64_t)wk->plane_count * wk->oversampling * wk->oversampling;
+            const int64_t kb = nsub * padded_rows(S, H) * S * (int64_t)sizeof(T2);
             if (kb >= ((int64_t)1 << 31))
                 return fail(RG_ERR_UNSUPPORTED, "batched W-towers kernel table too large (%lld bytes)",
                             (long long)kb);
+            RG_TRY(ctx->scratch[12].ensure((size_t)kb));
+            T2 *padded = ctx->scratch[12].as<T2>();
+            const int64_t total = nsub * padded_rows(S, H) * S;
+            hipLaunchKernelGGL(k_pad_kernels<T2>,
+                               dim3((unsigned)std::min<int64_t>((total + 255) / 256, 65536)),
+                               dim3(256), 0, s, (const T2 *)wk->data, padded, nsub, S, H);
+            RG_HIP(hipGetLastError());
+            bl.kern = padded;
             bl.kern_bytes = (uint32_t)kb;
         }
         bl.tw = tw_inv;
