@@ -80,22 +80,9 @@ struct WStackKeyer {
                       wp);
     }
 
-    __device__ int classify(int64_t i, uint32_t &key, uint32_t &pos, uint32_t &kidx) const
+    // the record of visibility i with cell c
+    __device__ static WsRec pack(int64_t i, const WsCell &c)
     {
-        WsCell c;
-        if (!cell(i, c, false))
-            return VIS_DROPPED;
-        key = key_of(c.us, c.vs, c.ws, ntx, wp);
-        pos = (uint32_t)i;
-        kidx = (uint32_t)c.us | ((uint32_t)c.vs << 16);
-        return VIS_OK;
-    }
-
-    // the record of visibility i (one classify accepted)
-    __device__ WsRec record(int64_t i) const
-    {
-        WsCell c;
-        cell(i, c);
         WsRec r;
         r.pos = (uint32_t)i;
         r.cell = (uint32_t)c.us | ((uint32_t)c.vs << 16);
@@ -632,8 +619,8 @@ int wstack_build(rg_context *ctx, const rg_wstack_params *p, const double *l, co
 // so degridding sorts in two levels with LDS counters only:
 //   coarse count   : workgroup b counts its chunk of the stream per coarse
 //                    bucket c = key >> fb (at most WS_NC buckets) in LDS; each
-//                    visibility keeps (key, rank in its workgroup's share of c);
-//                    hist[c][b] = the share
+//                    visibility's record and its rank in its workgroup's share
+//                    of c are written; hist[c][b] = the share
 //   bucket scan    : hist[c][.] to exclusive prefixes over the workgroups,
 //                    bucket totals to bucket starts
 //   coarse scatter : record of visibility i to bstart[c] + hist[c][b] + rank
@@ -650,8 +637,8 @@ constexpr int WS_SORT_BLOCKS = 2048; // workgroups of the coarse passes at most
 constexpr int WS_FB_MAX = 13;     // keys per bucket at most 2^13 (one fine window)
 
 __global__ __launch_bounds__(256) void k_ws_coarse_count(WStackKeyer kf, int64_t n, int64_t chunk,
-                                                         int nb, int fb, int nc, uint2 *kr,
-                                                         uint32_t *hist,
+                                                         int nb, int fb, int nc, uint32_t *rank,
+                                                         WsRec *rec, uint32_t *hist,
                                                          unsigned long long *stats)
 {
     __shared__ uint32_t h[WS_NC];
@@ -664,15 +651,16 @@ __global__ __launch_bounds__(256) void k_ws_coarse_count(WStackKeyer kf, int64_t
     const int64_t i0 = (int64_t)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
     uint32_t nd = 0;
     for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
-        uint32_t key = 0, pos, kidx;
-        uint2 e = make_uint2(KEY_NONE, 0u);
-        if (kf.classify(i, key, pos, kidx) == VIS_OK) {
-            e.x = key;
-            e.y = atomicAdd(&h[key >> fb], 1u);
+        WsCell cl;
+        uint32_t rk = KEY_NONE;
+        if (kf.cell(i, cl, true)) {
+            const uint32_t key = WStackKeyer::key_of(cl.us, cl.vs, cl.ws, kf.ntx, kf.wp);
+            rk = atomicAdd(&h[key >> fb], 1u);
+            rec[i] = WStackKeyer::pack(i, cl);
         } else {
             ++nd;
         }
-        kr[i] = e;
+        rank[i] = rk;
     }
     if (nd)
         atomicAdd(&dropped, nd);
@@ -728,20 +716,22 @@ __global__ __launch_bounds__(256) void k_ws_bucket_starts(uint32_t *bstart, int 
         bstart[nc] = total;
 }
 
-__global__ __launch_bounds__(256) void k_ws_coarse_scatter(WStackKeyer kf, int64_t n,
-                                                           int64_t chunk, int nb, int fb,
-                                                           const uint2 *__restrict__ kr,
+__global__ __launch_bounds__(256) void k_ws_coarse_scatter(int64_t n, int64_t chunk, int nb,
+                                                           int fb, int ntx, int wp,
+                                                           const uint32_t *__restrict__ rank,
+                                                           const WsRec *__restrict__ rec,
                                                            const uint32_t *__restrict__ hist,
                                                            const uint32_t *__restrict__ bstart,
                                                            WsRec *tmp)
 {
     const int64_t i0 = (int64_t)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
     for (int64_t i = i0 + threadIdx.x; i < i1; i += 256) {
-        const uint2 e = kr[i];
-        if (e.x == KEY_NONE)
+        const uint32_t rk = rank[i];
+        if (rk == KEY_NONE)
             continue;
-        const uint32_t c = e.x >> fb;
-        tmp[bstart[c] + hist[(size_t)c * nb + blockIdx.x] + e.y] = kf.record(i);
+        const WsRec r = rec[i];
+        const uint32_t c = WStackKeyer::key_of(r, ntx, wp) >> fb;
+        tmp[bstart[c] + hist[(size_t)c * nb + blockIdx.x] + rk] = r;
     }
 }
 
@@ -832,19 +822,23 @@ static int ws_sort(rg_context *ctx, const WStackKeyer &kf, int64_t n, int64_t nk
     RG_TRY(ctx->scratch[5].ensure(sizeof(WsRec) * n));                       // records
     RG_TRY(ctx->scratch[7].ensure(sizeof(WsRec) * n));                       // bucketed records
     RG_TRY(ctx->scratch[8].ensure(sizeof(uint32_t) * nkeys));                // per-key counts
-    RG_TRY(ctx->scratch[15].ensure(sizeof(uint2) * n)); // (key, bucket rank), then key ranks
+    RG_TRY(ctx->scratch[15].ensure(sizeof(uint32_t) * n)); // bucket ranks, then key ranks
     KernelTimer kt(ctx, RG_TIMER_BINNING, s);
     uint32_t *hist = ctx->scratch[0].as<uint32_t>(), *bst = ctx->scratch[3].as<uint32_t>();
     auto *stats = ctx->counters.as<unsigned long long>();
+    // the records in stream order go to the final record array (free until
+    // the fine scatter), their bucket ranks to scratch 15
+    uint32_t *rank = ctx->scratch[15].as<uint32_t>();
+    WsRec *rec0 = ctx->scratch[5].as<WsRec>();
     hipLaunchKernelGGL(k_ws_coarse_count, dim3((unsigned)nb), dim3(256), 0, s, kf, n, chunk, nb,
-                       fb, nc, ctx->scratch[15].as<uint2>(), hist, stats);
+                       fb, nc, rank, rec0, hist, stats);
     hipLaunchKernelGGL(k_ws_bucket_scan, dim3((unsigned)nc), dim3(256), 0, s, hist, nb, bst);
     hipLaunchKernelGGL(k_ws_bucket_starts, dim3(1), dim3(256), 0, s, bst, nc);
-    hipLaunchKernelGGL(k_ws_coarse_scatter, dim3((unsigned)nb), dim3(256), 0, s, kf, n, chunk, nb,
-                       fb, (const uint2 *)ctx->scratch[15].as<uint2>(), (const uint32_t *)hist,
-                       (const uint32_t *)bst, ctx->scratch[7].as<WsRec>());
+    hipLaunchKernelGGL(k_ws_coarse_scatter, dim3((unsigned)nb), dim3(256), 0, s, n, chunk, nb, fb,
+                       kf.ntx, kf.wp, (const uint32_t *)rank, (const WsRec *)rec0,
+                       (const uint32_t *)hist, (const uint32_t *)bst, ctx->scratch[7].as<WsRec>());
     uint32_t *cnt_key = ctx->scratch[8].as<uint32_t>();
-    uint32_t *rank = ctx->scratch[15].as<uint32_t>(); // the (key, rank) pairs are no longer read
+    // bucket ranks are no longer read: the key ranks reuse their array
     const WsRec *tmp = ctx->scratch[7].as<WsRec>();
     RG_TRY(zero_fill(cnt_key, sizeof(uint32_t) * nkeys, s));
     hipLaunchKernelGGL(k_ws_fine_rank, dim3((unsigned)((n + WS_FINE_CHUNK - 1) / WS_FINE_CHUNK)),
