@@ -187,12 +187,12 @@ __device__ inline size_t gcb_index(int row, int col, int n)
 //   xy = gz | cy0 << 16, gz = (gx + S) z the footprint's first column in the
 //        band's row layout (S elements of left pad), cy0 = H b - gy (signed)
 //        the footprint row that band row 0 meets;
-//   kb = (kidx PS S + (min(cy0, S) + H - 1) S) z, byte offset of that
-//        footprint row's first tap in the padded kernel table (every S x S
-//        sub-kernel stored with H - 1 zero rows above and H below, PS =
-//        S + 2 H - 1 rows): band row l reads padded row min(cy0, S) + H - 1 + l,
-//        a zero row whenever the footprint misses it, always inside its own
-//        sub-kernel.
+//   kb = ((H + kidx PS + min(cy0, S)) S) z, byte offset of that footprint
+//        row's first tap in the padded kernel table (H zero rows, then every
+//        S x S sub-kernel followed by H zero rows, PS = S + H): band row l
+//        reads row min(cy0, S) + l of the record's sub-kernel, in
+//        [1 - H, S + H - 1]; rows outside [0, S) fall in the zero rows after
+//        the sub-kernel or after the one before it (or in the leading rows).
 template <typename T> struct BandRec {
     uint32_t xy;
     int32_t kb;
@@ -203,7 +203,7 @@ static_assert(sizeof(BandRec<float>) == sizeof(Rec<float>) &&
               "band records replace binned records one for one");
 
 
-__host__ __device__ constexpr int padded_rows(int S, int H) { return S + 2 * H - 1; }
+__host__ __device__ constexpr int padded_rows(int S, int H) { return S + H; }
 
 template <typename T>
 __device__ inline BandRec<T> band_record(const Rec<T> &r, int S, int H, int Hb, int z)
@@ -212,29 +212,32 @@ __device__ inline BandRec<T> band_record(const Rec<T> &r, int S, int H, int Hb, 
     const int cy0 = Hb - gy; // >= 1 - H for every record of band Hb / H's window
     BandRec<T> o;
     o.xy = (uint32_t)((gx + S) * z) | ((uint32_t)cy0 << 16);
-    o.kb = ((int32_t)r.kidx * padded_rows(S, H) * S + (min(cy0, S) + H - 1) * S) * z;
+    o.kb = ((H + (int32_t)r.kidx * padded_rows(S, H) + min(cy0, S)) * S) * z;
     o.vis = r.vis;
     return o;
 }
 
-// Kernel table for the band pass: sub-kernel k (S x S) at k PS S with H - 1
-// zero rows above and H below (PS = S + 2 H - 1 rows).
+// Kernel table for the band pass: H zero rows, then sub-kernel k (S x S) at
+// row H + k PS followed by H zero rows (PS = S + H); (H + nsub PS) S taps.
 template <typename T2>
 __global__ __launch_bounds__(256) void k_pad_kernels(const T2 *__restrict__ src, T2 *dst,
                                                      int64_t nsub, int S, int H)
 {
     const int PS = padded_rows(S, H);
-    const int64_t total = nsub * PS * S;
+    const int64_t total = ((int64_t)H + nsub * PS) * S;
     for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
          e += (int64_t)gridDim.x * 256) {
-        const int64_t k = e / (PS * S);
-        const int rr = (int)(e - k * PS * S);
-        const int row = rr / S - (H - 1), col = rr % S;
+        const int64_t prow = e / S - H; // row counted from the first sub-kernel
+        const int col = (int)(e % S);
         T2 v;
         v.x = 0;
         v.y = 0;
-        if (row >= 0 && row < S)
-            v = src[(k * S + row) * S + col];
+        if (prow >= 0) {
+            const int64_t k = prow / PS;
+            const int row = (int)(prow - k * PS);
+            if (row < S)
+                v = src[(k * S + row) * S + col];
+        }
         dst[e] = v;
     }
 }
@@ -1283,13 +1286,13 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
             // the band pass reads the table with zero rows around every
             // sub-kernel; taps are addressed by 32-bit byte offsets
             const int64_t nsub = (int64_t)wk->plane_count * wk->oversampling * wk->oversampling;
-            const int64_t kb = nsub * padded_rows(S, H) * S * (int64_t)sizeof(T2);
+            const int64_t kb = ((int64_t)H + nsub * padded_rows(S, H)) * S * (int64_t)sizeof(T2);
             if (kb >= ((int64_t)1 << 31))
                 return fail(RG_ERR_UNSUPPORTED, "batched W-towers kernel table too large (%lld bytes)",
                             (long long)kb);
             RG_TRY(ctx->scratch[12].ensure((size_t)kb));
             T2 *padded = ctx->scratch[12].as<T2>();
-            const int64_t total = nsub * padded_rows(S, H) * S;
+            const int64_t total = ((int64_t)H + nsub * padded_rows(S, H)) * S;
             hipLaunchKernelGGL(k_pad_kernels<T2>,
                                dim3((unsigned)std::min<int64_t>((total + 255) / 256, 65536)),
                                dim3(256), 0, s, (const T2 *)wk->data, padded, nsub, S, H);
