@@ -1064,7 +1064,10 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
         RG_TRY(ctx->scratch[4].ensure(sizeof(Item) * std::max<int64_t>(nkeys + 1, max_items_w)));
 
     BinResult br;
-    RG_TRY((bin_visibilities<TowerKeyer, T>(ctx, kf, vis->n, nkeys, 0x40000000u, s, br)));
+    // (task, band) keys: 4 counter replicas at the reference's workload (the
+    // scans read every replica; 16 were slower, 1 << 20 counters in all)
+    RG_TRY((bin_visibilities<TowerKeyer, T>(ctx, kf, vis->n, nkeys, 0x40000000u, s, br,
+                                            (int64_t)1 << 20)));
 
     // visibilities per task slot (read back)
     RG_TRY(ctx->scratch[9].ensure(sizeof(uint32_t) * (2 * nslots + nkeys + 1)));
