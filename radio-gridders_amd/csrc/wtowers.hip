@@ -187,12 +187,17 @@ __device__ inline size_t gcb_index(int row, int col, int n)
 //   xy = gz | cy0 << 16, gz = (gx + S) z the footprint's first column in the
 //        band's row layout (S elements of left pad), cy0 = H b - gy (signed)
 //        the footprint row that band row 0 meets;
-//   kb = ((H + kidx PS + min(cy0, S)) S) z, byte offset of that footprint
-//        row's first tap in the padded kernel table (H zero rows, then every
-//        S x S sub-kernel followed by H zero rows, PS = S + H): band row l
-//        reads row min(cy0, S) + l of the record's sub-kernel, in
-//        [1 - H, S + H - 1]; rows outside [0, S) fall in the zero rows after
-//        the sub-kernel or after the one before it (or in the leading rows).
+//   kb, byte offset of that footprint row's first tap:
+//   - FP32 (padded table: H zero rows, then every S x S sub-kernel followed
+//     by H zero rows, PS = S + H): kb = ((H + kidx PS + min(cy0, S)) S) z;
+//     band row l reads row min(cy0, S) + l of the record's sub-kernel, in
+//     [1 - H, S + H - 1]; rows outside [0, S) fall in the zero rows after the
+//     sub-kernel or after the one before it (or in the leading rows);
+//   - FP64 (the table as given; its padded copy would outgrow an XCD's L2 at
+//     large supports): kb = (kidx S^2 + cy0 S) z, never dereferenced for a
+//     lane whose row the footprint misses (that lane reads no tap).
+template <typename T> constexpr bool padded_taps() { return sizeof(T) == 4; }
+
 template <typename T> struct BandRec {
     uint32_t xy;
     int32_t kb;
@@ -212,7 +217,10 @@ __device__ inline BandRec<T> band_record(const Rec<T> &r, int S, int H, int Hb, 
     const int cy0 = Hb - gy; // >= 1 - H for every record of band Hb / H's window
     BandRec<T> o;
     o.xy = (uint32_t)((gx + S) * z) | ((uint32_t)cy0 << 16);
-    o.kb = ((H + (int32_t)r.kidx * padded_rows(S, H) + min(cy0, S)) * S) * z;
+    if constexpr (padded_taps<T>())
+        o.kb = ((H + (int32_t)r.kidx * padded_rows(S, H) + min(cy0, S)) * S) * z;
+    else
+        o.kb = ((int32_t)r.kidx * S * S + cy0 * S) * z;
     o.vis = r.vis;
     return o;
 }
@@ -242,7 +250,8 @@ __global__ __launch_bounds__(256) void k_pad_kernels(const T2 *__restrict__ src,
     }
 }
 
-// Kernel taps through a buffer resource (offsets beyond the table read zeros).
+// Kernel taps through a buffer resource: offsets beyond the table read zeros
+// (TAP_NONE for a lane whose row the footprint misses, unpadded table).
 __device__ inline float2 tap_load(__amdgpu_buffer_rsrc_t r, uint32_t off, float2 *)
 {
     return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
@@ -251,6 +260,7 @@ __device__ inline double2 tap_load(__amdgpu_buffer_rsrc_t r, uint32_t off, doubl
 {
     return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
 }
+constexpr uint32_t TAP_NONE = 0x80000000u;
 // word 3 of a raw (unswizzled, stride 0) buffer resource on gfx9
 constexpr int BUF_RSRC_W3 = 0x00020000;
 
@@ -427,9 +437,10 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
         if (base + threadIdx.x < item.count) {
             r = recs[item.start + base + threadIdx.x];
         } else {
-            // padding: column n, after every footprint column (its cells lie
-            // in the row gaps), no visibility
-            r.xy = (uint32_t)((n + S) * z);
+            // padding, no visibility: padded table: column n, after every
+            // footprint column (its cells lie in the row gaps); otherwise a
+            // row out of reach of every band row (no tap, cell kept)
+            r.xy = padded_taps<T>() ? (uint32_t)((n + S) * z) : 0x40000000u;
             r.kb = 0;
             r.vis.x = T(0);
             r.vis.y = T(0);
@@ -452,7 +463,7 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
     // per lane, in bytes: column residue, tap row, cell row
     const int uz = myU * z, tap_row = myV * S * z, cell_row = myV * BS * z;
     char *band_b = reinterpret_cast<char *>(band);
-    int cur = -1;
+    int cur = -1, carry = -1; // carry: the kept cell of the unpadded form
     T sr = T(0), si = T(0);
     auto flush = [&]() {
         T *c = reinterpret_cast<T *>(band_b + cur);
@@ -479,19 +490,29 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
         const int ng = min(RG, max(0, (int)nloc - grp * RG));
         const BandRec<T> *gr = s_rec + grp * RG;
         using Bt = RecBatch<T, U>;
-        // A lane whose row the footprint misses reads a zero tap from the
-        // padding of the record's sub-kernel; its cell follows the footprint
-        // column like every other lane's, so it changes cell with them and
-        // flushes only when it has summed something.
+        // A lane whose row the footprint misses reads a zero tap: from the
+        // padding of the record's sub-kernel (its cell then follows the
+        // footprint column like every other lane's, so it changes cell with
+        // them and flushes only when it has summed something), or, unpadded,
+        // through an offset past the table, keeping its previous cell.
         auto fetch = [&](uint32_t q0, Bt &bt) {
 #pragma unroll
             for (int q = 0; q < U; ++q) {
                 const BandRec<T> r = gr[q0 + q];
                 const int gz = (int)(r.xy & 0xffffu);
                 const int cxz = (uz - gz) & ((S - 1) * z);
-                bt.cell[q] = cell_row + gz + cxz;
                 bt.v[q] = r.vis;
-                bt.k[q] = tap_load(ktab, (uint32_t)(r.kb + tap_row + cxz), (T2 *)nullptr);
+                if constexpr (padded_taps<T>()) {
+                    bt.cell[q] = cell_row + gz + cxz;
+                    bt.k[q] = tap_load(ktab, (uint32_t)(r.kb + tap_row + cxz), (T2 *)nullptr);
+                } else {
+                    const int cy = ((int)r.xy >> 16) + myV;
+                    const bool in = (unsigned)cy < (unsigned)S;
+                    carry = in ? cell_row + gz + cxz : carry;
+                    bt.cell[q] = carry;
+                    bt.k[q] = tap_load(ktab, in ? (uint32_t)(r.kb + tap_row + cxz) : TAP_NONE,
+                                       (T2 *)nullptr);
+                }
             }
         };
         // A lane's cell never decreases along the records (they are sorted by
@@ -1286,21 +1307,24 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
         bl.item_start = item_start_w;
         bl.rank_of_slot = d_tab + o_ros;
         {
-            // the band pass reads the table with zero rows around every
+            // the FP32 band pass reads the table with zero rows after every
             // sub-kernel; taps are addressed by 32-bit byte offsets
             const int64_t nsub = (int64_t)wk->plane_count * wk->oversampling * wk->oversampling;
-            const int64_t kb = ((int64_t)H + nsub * padded_rows(S, H)) * S * (int64_t)sizeof(T2);
+            const int64_t rows = padded_taps<T>() ? (int64_t)H + nsub * padded_rows(S, H) : nsub * S;
+            const int64_t kb = rows * S * (int64_t)sizeof(T2);
             if (kb >= ((int64_t)1 << 31))
                 return fail(RG_ERR_UNSUPPORTED, "batched W-towers kernel table too large (%lld bytes)",
                             (long long)kb);
-            RG_TRY(ctx->scratch[12].ensure((size_t)kb));
-            T2 *padded = ctx->scratch[12].as<T2>();
-            const int64_t total = ((int64_t)H + nsub * padded_rows(S, H)) * S;
-            hipLaunchKernelGGL(k_pad_kernels<T2>,
-                               dim3((unsigned)std::min<int64_t>((total + 255) / 256, 65536)),
-                               dim3(256), 0, s, (const T2 *)wk->data, padded, nsub, S, H);
-            RG_HIP(hipGetLastError());
-            bl.kern = padded;
+            bl.kern = wk->data;
+            if (padded_taps<T>()) {
+                RG_TRY(ctx->scratch[12].ensure((size_t)kb));
+                T2 *padded = ctx->scratch[12].as<T2>();
+                hipLaunchKernelGGL(k_pad_kernels<T2>,
+                                   dim3((unsigned)std::min<int64_t>((rows * S + 255) / 256, 65536)),
+                                   dim3(256), 0, s, (const T2 *)wk->data, padded, nsub, S, H);
+                RG_HIP(hipGetLastError());
+                bl.kern = padded;
+            }
             bl.kern_bytes = (uint32_t)kb;
         }
         bl.tw = tw_inv;
