@@ -666,6 +666,8 @@ __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
     constexpr int NB = n / H;
     __shared__ T2 lds[n * CB];
     __shared__ T2 tw[n];
+    __shared__ uint8_t s_band[NB]; // band b of the current floor has records
+    static_assert(NB <= NT, "one thread per band flag");
     for (int k = threadIdx.x; k < n; k += NT)
         tw[k] = twg[k];
     const int c = chunk_base + blockIdx.y;
@@ -699,8 +701,12 @@ __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
     int run = 0, wp_prev = 0;
     for (int k = first; k < first + count; ++k) {
         const T2 *g = G + (size_t)(k - t0) * n * n + (size_t)blockIdx.x * n * CB + line;
-        // band b of the task was written iff it has work items
-        const uint32_t *f = key_start + (size_t)task_slot[k] * NB;
+        // band b of the task was written iff it has work items (the previous
+        // floor's flags were read in its first stage, before its barrier)
+        if (threadIdx.x < NB) {
+            const uint32_t *f = key_start + (size_t)task_slot[k] * NB;
+            s_band[threadIdx.x] = f[threadIdx.x + 1] > f[threadIdx.x] ? 1 : 0;
+        }
         const int wp = task_wp[k];
         if (run > 0 && (run == SCREEN_RUN || wp != wp_prev + 1)) {
             close_run(wp_prev);
@@ -711,7 +717,7 @@ __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
             T2 val;
             val.x = T(0);
             val.y = T(0);
-            if (f[row / H + 1] > f[row / H])
+            if (s_band[row / H])
                 val = g[(size_t)row * CB];
             return val;
         };
@@ -724,7 +730,7 @@ __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
                 h[q] = val;
             }
         };
-        __syncthreads(); // the previous task's last stage has read lds
+        __syncthreads(); // the previous floor's last stage has read lds; band flags set
         fft_block_io<T2, LOGN, CB, CB, true>(lds, tw, load, emit);
         ++run;
         wp_prev = wp;
