@@ -870,8 +870,9 @@ int wstack_degrid(rg_context *ctx, const rg_wstack_params *p, const double *u, c
     if (ctx->ws_og != og || ctx->ws_planes != wp || ctx->ws_g != g)
         return fail(RG_ERR_PARAM, "no W-stack of this geometry has been built in the context");
     auto *cnt = ctx->counters.as<unsigned long long>();
-    // per call: visibilities not predicted (the sky counter is kept)
-    RG_TRY(zero_fill(cnt + CNT_DROPPED, sizeof(unsigned long long) * 2, s));
+    // per call: every counter but the build's skipped sources
+    static_assert(CNT_SKY_SKIPPED == CNT__N - 1, "per-call counters precede the sky counter");
+    RG_TRY(zero_fill(cnt, sizeof(unsigned long long) * CNT_SKY_SKIPPED, s));
     ctx->last_nvis = nvis;
     if (nvis == 0)
         return RG_OK;

@@ -230,3 +230,26 @@ def test_sort_handles_sparse_and_concentrated_streams(ctx, kernels, layout):
     assert np.abs(got - ref).max() < 1e-10 * np.abs(ref).max()
     st = ctx.stats()
     assert st.n_vis == nv and st.n_dropped == 0 and st.n_gridded == nv
+
+
+def test_empty_and_all_dropped_streams(ctx, kernels):
+    """No visibilities, and visibilities that all leave the stack: the sort
+    sees no records (empty buckets, no work items) and every output is 0."""
+    theta, lam = 0.1, 640.0
+    d = kernels["desc"]
+    p = params(kernels, theta, lam)
+    l, m = card(theta)
+    ctx.wstack_build(p, dev(l), dev(m), None, d["lm"], d["n"])
+    e = torch.zeros(0, dtype=torch.float64, device="cuda")
+    out0 = torch.zeros(0, dtype=torch.complex128, device="cuda")
+    ctx.wstack_degrid(p, e, e, e, d["uv"], d["w"], out0)
+    torch.cuda.synchronize()
+    assert ctx.stats().n_vis == 0 and ctx.stats().n_gridded == 0
+    nv = 3000
+    u = np.full(nv, 1e7)
+    out = torch.full((nv,), 5.0, dtype=torch.complex128, device="cuda")
+    ctx.wstack_degrid(p, dev(u), dev(np.zeros(nv)), dev(np.zeros(nv)), d["uv"], d["w"], out)
+    torch.cuda.synchronize()
+    assert torch.all(out == 0)
+    st = ctx.stats()
+    assert st.n_vis == nv and st.n_dropped == nv and st.n_gridded == 0
