@@ -59,9 +59,12 @@ def main():
     ap.add_argument("--dumps", type=int, default=64)
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-plain", action="store_true", help="skip the FP64 plain runs")
-    ap.add_argument("--only", choices=["all", "wproj", "wtowers", "channels", "wstack", "epic"],
+    ap.add_argument("--only", choices=["all", "wproj", "wtowers", "channels", "wstack", "epic",
+                                       "c1"],
                     default="all")
     ap.add_argument("--skip-epic", action="store_true", help="skip the EPIC gridding workload")
+    ap.add_argument("--skip-c1", action="store_true",
+                    help="skip the 27-station W-projection precision-form workload")
     ap.add_argument("--skip-wstack", action="store_true",
                     help="skip the W-stacking predict workload")
     ap.add_argument("--skip-channels", action="store_true",
@@ -152,7 +155,9 @@ def main():
             f"wall {wall / steps * 1e3:.1f} ms/step, timers {timers}")
         return {"ms_per_step": ms, "mvis_s": obs.n_vis / ms / 1e3, "timers": timers,
                 "gridded": st.n_gridded, "dropped": st.n_dropped, "bad_plane": st.n_bad_plane,
-                "items": st.n_items, "tasks": st.n_tasks, "towers": st.n_towers}
+                "items": st.n_items, "tasks": st.n_tasks, "towers": st.n_towers,
+                "records": st.n_records, "bands": st.n_bands, "chunks": st.n_chunks,
+                "_stats": st}
 
     def image(grid):
         img = grid.to(torch.complex128).clone()
@@ -196,16 +201,29 @@ def main():
         pp = rgamd.params(N, theta, rgamd.RG_MODE_PLAIN, subgrid=SUB, margin=MARGIN, winc=WINC)
         pb = rgamd.params(N, theta, rgamd.RG_MODE_BATCHED, subgrid=SUB, margin=MARGIN, winc=WINC,
                           tile=args.tower_band, floor_batch=args.floor_batch)
+        gb = torch.empty_like(g64)
+        r["fp64_batched"] = timed("W-towers FP64 batched", pb, fv128, k128_t, gb, K, W)
         if not args.skip_plain:
             r["fp64_plain"] = timed("W-towers FP64 plain", pp, fv128, k128_t, g64, 1, 0)
         else:
-            ctx.grid(pb, fv128, k128_t, g64)
+            g64.copy_(gb)
         r["fp32_batched"] = timed("W-towers FP32 batched", pb, fv64, k64_t, g32, K, W)
-        i32, i64 = image(g32), image(g64)
+        i32, i64, ib = image(g32), image(g64), image(gb)
         r["error_fp32_batched_vs_fp64"] = errs(i32, i64)
+        r["error_fp64_batched_vs_fp64"] = errs(ib, i64)
+        r["speedup_fp32_vs_fp64_batched"] = (r["fp64_batched"]["ms_per_step"] /
+                                            r["fp32_batched"]["ms_per_step"])
         dft_images["wtowers_fp64" + ("_plain" if not args.skip_plain else "_batched")] = i64
         dft_images["wtowers_fp32_batched"] = i32
         res["wtowers"] = r
+        del gb, ib
+
+    if args.only in ("all", "c1") and not args.skip_c1:
+        try:
+            res["c1_c3"] = bench_c1(ctx, K, W, args.skip_cpu, args.dft_pixels)
+        except Exception as e:  # reported, never hides the headline
+            log(f"[bench] 27-station workload failed: {e!r}")
+            res["c1_c3"] = {"error": repr(e)}
 
     if args.only in ("all", "channels") and not args.skip_channels:
         try:
@@ -250,15 +268,37 @@ def main():
     # the counter summaries were taken on the default 512-station, 64-dump workload
     default_workload = (args.stations, args.dumps) == (512, 64) and not args.tower_band \
         and not args.floor_batch
-    roof, job = roofline(head_kind, head, nv, S, N, SUB, default_workload,
-                         table_bytes=(wp_planes if head_kind == "wproj" else 11) * 64 * S * S * 8)
+    H = args.tower_band or 8
+    # FP32 tables: W-projection as given; W-towers band pass: the padded copy
+    # (H zero rows after every S x S sub-kernel and before the first)
+    tab_bytes = (wp_planes * 64 * S * S * 8 if head_kind == "wproj"
+                 else (H + 11 * 64 * (S + H)) * S * 8)
+    roof, job = roofline(head_kind, head, head["_stats"], nv, S, N, SUB, H, default_workload,
+                         tab_bytes)
 
     cpu = None
     if not args.skip_cpu:
         if head_kind == "wtowers":
-            cpu = cpu_baseline_wtowers(obs, vis128, tab_t, SUB, MARGIN, WINC, head["towers"])
+            cpu, cpu_grid = cpu_baseline_wtowers(obs, vis128, tab_t, SUB, MARGIN, WINC,
+                                                 head["towers"])
+            # the CPU restatement's image against the GPU FP64 image of the same inputs
+            gi = torch.from_numpy(cpu_grid).cuda()
+            del cpu_grid
+            ctx.grid_to_image(gi)
+            ref = dft_images.get("wtowers_fp64_plain", dft_images.get("wtowers_fp64_batched"))
+            cpu["error_gpu_fp64_vs_cpu"] = errs(ref, gi)
+            cpu["error_gpu_fp32_batched_vs_cpu"] = errs(dft_images["wtowers_fp32_batched"], gi)
+            del gi
+            log(f"[bench] CPU vs GPU images: {cpu['error_gpu_fp64_vs_cpu']}")
+            cpu["wprojection"] = cpu_baseline(obs, vis128, tab_p, wp_min, wp_step,
+                                              args.cpu_seconds)
         else:
             cpu = cpu_baseline(obs, vis128, tab_p, wp_min, wp_step, args.cpu_seconds)
+    for kind in res.values():
+        if isinstance(kind, dict):
+            for v in kind.values():
+                if isinstance(v, dict):
+                    v.pop("_stats", None)
 
     out = {
         "metric": METRIC,
@@ -281,6 +321,13 @@ def main():
                    "n_vis": nv, "grid": N, "path": head_kind},
         "speedup_vs_fp64_plain": (head["mvis_s"] / plain["mvis_s"]) if plain else None,
         "error_vs_fp64": res[head_kind]["error_fp32_batched_vs_fp64"],
+        "gate": ("max and RMS relative image error of FP32 batched against FP64 plain on "
+                 "identical inputs and kernels <= 1e-5 (the metric's gate); the error against "
+                 "the FP64 direct DFT is reported, not gated: it is set by the W-kernel's "
+                 "oversampling and support (about 7e-4 in the inner field for every path and "
+                 "precision alike), so no gridder with these kernels reaches 1e-5 against it"),
+        "gate_met": (res[head_kind]["error_fp32_batched_vs_fp64"]["max_rel"] <= 1e-5 and
+                     res[head_kind]["error_fp32_batched_vs_fp64"]["rms_rel"] <= 1e-5),
         "error_vs_dft": dft,
         "roofline": roof,
         "job_roofline": job,
@@ -288,6 +335,94 @@ def main():
         "results": res,
     }
     print(json.dumps(out), flush=True)
+
+
+def bench_c1(ctx, steps, warmup, skip_cpu, npix):
+    """27 stations on a random Y (10 km arms), 256 dumps over 4 h at dec -30
+    deg, 150 MHz, baselines < 10 km (about 90 000 visibilities), 10 sources,
+    2048^2 grid (theta 0.1, lambda 20480), W-projection with S16 x R8 kernels
+    at w-step 10.  Plain FP64 W-projection against the precision form (FP32
+    visibilities and tables, FP64 uvw-to-cell and plane indexing, FP32 tile
+    sums, plain and two-sum compensated), each image against the FP64 plain
+    image, the FP64 direct DFT on `npix` seeded pixels, and the full 2048^2
+    image of the oracle's FP64 CPU W-projection (src/reference/grid.c:187-202
+    restated), whose serial run is also timed."""
+    import torch
+
+    import rgamd
+    import synth
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import pyoracle as orc  # the checker: reference image and CPU timing only
+
+    obs = synth.config_c1()
+    N, theta, nv = obs.grid_size, obs.theta, obs.n_vis
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+    u, v, w = dev(obs.u), dev(obs.v), dev(obs.w)
+    vis128 = torch.empty(nv, dtype=torch.complex128, device="cuda")
+    rgamd.predict_points(u, v, w, dev(obs.l), dev(obs.m), dev(obs.flux), vis128)
+    fv128 = rgamd.flat_vis(u, v, w, vis128)
+    vis64 = vis128.to(torch.complex64)
+    fv64 = rgamd.flat_vis(u, v, w, vis64)
+    S, R, w_step = 16, 8, 10.0
+    w_min, P = synth.kernel_range(obs.w, w_step)
+    tab_h = rgamd.make_wkernels(P, R, S, theta, w_min, w_step)
+    tab = dev(tab_h)
+    tab32 = tab.to(torch.complex64)
+    k128, k64 = rgamd.wkern_desc(tab, w_min, w_step), rgamd.wkern_desc(tab32, w_min, w_step)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def run(p, fv, wk, dt, reps):
+        g = torch.empty((N, N), dtype=dt, device="cuda")
+        for _ in range(warmup):
+            ctx.grid(p, fv, wk, g)
+        torch.cuda.synchronize()
+        e0.record()
+        for _ in range(reps):
+            ctx.grid(p, fv, wk, g)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        img = g.to(torch.complex128)
+        ctx.grid_to_image(img)
+        return {"ms_per_step": ms, "mvis_s": nv / ms / 1e3}, img
+
+    out = {"n_vis": nv, "grid": N, "w_planes": P}
+    images = {}
+    out["fp64_plain"], images["fp64_plain"] = run(rgamd.params(N, theta, rgamd.RG_MODE_PLAIN),
+                                                  fv128, k128, torch.complex128, steps)
+    out["fp32_batched"], images["fp32_batched"] = run(rgamd.params(N, theta), fv64, k64,
+                                                      torch.complex64, steps)
+    out["fp32_batched_compensated"], images["fp32_batched_compensated"] = run(
+        rgamd.params(N, theta, compensated=1), fv64, k64, torch.complex128, steps)
+
+    def errs(a, b):
+        d = (a - b).abs()
+        return {"max_rel": float(d.max() / b.abs().max()),
+                "rms_rel": float(torch.sqrt((d ** 2).mean()) / torch.sqrt((b.abs() ** 2).mean()))}
+
+    # the full image of the oracle's FP64 CPU W-projection (serial, as grid.c)
+    t = time.perf_counter()
+    gref, _, bad = orc.grid_wprojection(N, theta, obs.u, obs.v, obs.w, vis128.cpu().numpy(),
+                                        tab_h, w_min, w_step, fast=True)
+    cpu_s = time.perf_counter() - t
+    iref = torch.from_numpy(gref).cuda()
+    ctx.grid_to_image(iref)
+    for name, img in images.items():
+        out[name]["error_vs_fp64_plain"] = errs(img, images["fp64_plain"])
+        out[name]["error_vs_cpu_image"] = errs(img, iref)
+    if npix > 0:
+        out["error_vs_dft"] = dft_errors(ctx, fv128, N, obs.lam, theta, S, images, npix)
+    if not skip_cpu:
+        _, host = host_threads()
+        out["cpu_baseline"] = {"value": nv / cpu_s / 1e6, "unit": "Mvis/s", "cores": 1,
+                               "kind": "port", "host": host,
+                               "sample": f"all {nv} visibilities, serial FP64 W-projection "
+                                         "(grid.c:187-202 has no OpenMP), -O3 -ffast-math"}
+    log(f"[bench] C1: plain FP64 {out['fp64_plain']['ms_per_step']:.3f} ms, FP32 batched "
+        f"{out['fp32_batched']['ms_per_step']:.3f} ms; FP32 vs CPU image "
+        f"{out['fp32_batched']['error_vs_cpu_image']}")
+    del tab, tab32, iref
+    return out
 
 
 def bench_wstack(ctx, steps, warmup, skip_cpu, cpu_seconds):
@@ -372,7 +507,7 @@ def bench_wstack(ctx, steps, warmup, skip_cpu, cpu_seconds):
     if not skip_cpu:
         # the oracle's OpenMP degridding of the same stack on a sample
         stack = ctx.wstack_stack(og, wp)
-        threads = min(16, os.cpu_count() or 1)
+        threads, host = host_threads()
         ns = 200_000
         su, sv, sw = u[:ns].cpu().numpy(), v[:ns].cpu().numpy(), w[:ns].cpu().numpy()
         t = time.perf_counter()
@@ -389,7 +524,7 @@ def bench_wstack(ctx, steps, warmup, skip_cpu, cpu_seconds):
                               nthreads=threads, fast=True)
         dt = (time.perf_counter() - t) / reps
         r["cpu_baseline"] = {"value": ns / dt / 1e6, "unit": "Mvis/s", "cores": threads,
-                             "kind": "port",
+                             "kind": "port", "host": host,
                              "sample": f"{ns} of the visibilities, degridding only, "
                                        f"{reps} repetitions"}
         log(f"[bench] CPU W-stack degrid: {r['cpu_baseline']['value']:.1f} Mvis/s on {threads} "
@@ -626,54 +761,92 @@ def bench_stress(ctx, steps, warmup, npix=4096):
     return out
 
 
-def roofline(kind, head, nv, S, N, n, default_workload, table_bytes):
-    """Dominant-kernel and whole-step rooflines under the algorithmic count:
-    per visibility 24 (uvw) + 8 (vis) + S^2 x 8 (taps) bytes and 8 S^2 flop;
-    W-towers per (tower, floor) task 6 n^2 x 8 bytes (scatter write, row pass
-    read + write, column pass read + subimage read + write) and
-    5 n^2 log2(n^2) + 8 n^2 flop, per tower 4 n^2 x 8 bytes and the same
-    flop; per run the N^2 grid written once and the kernel table read once.
-    The band kernel (scatter + row pass) is charged the visibility bytes and
-    3 n^2 x 8 per task, the column pass 3 n^2 x 8 per task."""
-    per_vis = 24 + 8 + S * S * 8
+def tower_work(st, nv, N, n, S, H, table_bytes, s_v=8, s_sub=8):
+    """Compulsory HBM bytes and useful flop of one batched W-towers step, per
+    kernel, from the call's own counts (st: rg_grid_stats of the step).  The
+    byte count charges what this design must move through HBM at least once:
+    the visibility stream and the sort's records, the band records, the
+    subgrid bands written by the band pass and read by the column pass, the
+    partial and tower subimages, the subimage FFTs, the fold and the kernel
+    table once.  Taps gathered from the (L2-resident) table are not charged."""
+    rec = 16  # Rec<float> / BandRec<float>
+    nn = n * n
+    lg = math.log2(n)
+    band = H * n * s_sub
+    b = {
+        # count: uvw in, (key, rank, pos, kidx) out; scatter: that + vis in, records out
+        "sort": nv * (24 + 16) + nv * (16 + s_v + rec),
+        # binned records of every band window in, band records out
+        "window_sort": st.n_records * rec * 2,
+        "band_pass": st.n_records * rec + st.n_bands * band + table_bytes,
+        "column_pass": st.n_bands * band + st.n_chunks * nn * s_sub,
+        "reduce": st.n_chunks * nn * s_sub + 2 * st.n_towers * nn * s_sub,
+        # forward FFT of every subimage (rows and columns, read + write), then
+        # the fold reads them and updates the FP64 master grid
+        "finish": 4 * st.n_towers * nn * s_sub + st.n_towers * nn * s_sub + 2 * N * N * 16,
+    }
+    f = {
+        "band_pass": nv * 8 * S * S + st.n_bands * H * 5 * n * lg,
+        "column_pass": st.n_tasks * (5 * nn * lg + 8 * nn),
+        "finish": st.n_towers * 5 * nn * math.log2(nn),
+    }
+    return b, f
+
+
+def wproj_work(st, nv, N, S, table_bytes, s_v=8, s_grid=8):
+    """Compulsory HBM bytes and useful flop of one batched W-projection step:
+    the visibility stream, the sort's records, the tile flushes (the grid,
+    once) and the kernel table once."""
+    rec = 16
+    b = {"sort": nv * (24 + 16) + nv * (16 + s_v + rec),
+         "tiles": st.n_records * rec + N * N * s_grid + table_bytes}
+    f = {"tiles": nv * 8 * S * S}
+    return b, f
+
+
+def roofline(kind, head, st, nv, S, N, n, H, default_workload, table_bytes):
+    """Dominant-kernel and whole-step rooflines.  `frac` (HBM) and `valu.frac`
+    use the compulsory bytes and useful flop of tower_work / wproj_work over
+    the kernel's average launch time; `effective` keeps the per-visibility
+    count 24 + 8 + S^2 x 8 bytes (every tap gather charged, also those
+    served from L2 / LDS) plus 3 n^2 x 8 bytes per (tower, floor) task, which
+    is not a floor on HBM traffic and can exceed the HBM peak."""
     timers = head["timers"]
     if kind == "wproj":
-        name, tkey, prof = "k_tiles (W-projection)", "wproj_tiles", "wproj_tiles_fp32_c2.txt"
-        byt = nv * per_vis
-        flop = nv * 8 * S * S
-        unit = {"visibility": per_vis}
-        job_b = nv * per_vis + N * N * 8 + table_bytes
-        job_f = nv * 8 * S * S
+        name, tkey, prof, part = ("k_tiles (W-projection)", "wproj_tiles",
+                                  "wproj_tiles_fp32_c2.txt", "tiles")
+        b, f = wproj_work(st, nv, N, S, table_bytes)
+        eff = nv * (24 + 8 + S * S * 8)
     else:
-        nt, ntow, n2 = head["tasks"], head["towers"], n * n
-        name, tkey, prof = ("k_tower_band_items (W-towers scatter + row IFFT)", "tower_tiles",
-                            "tower_band_items_fp32_c2.txt")
-        byt = nv * per_vis + nt * 3 * n2 * 8
-        unit = {"visibility": per_vis, "(tower, floor) task": 3 * n2 * 8}
-        fft = 5 * n2 * math.log2(n2) + 8 * n2
-        # the band kernel: the taps and the row half of each task's 2-D IFFT
-        flop = nv * 8 * S * S + nt * 5 * n2 * math.log2(n)
-        job_b = nv * per_vis + nt * 6 * n2 * 8 + ntow * 4 * n2 * 8 + N * N * 16 + table_bytes
-        job_f = nv * 8 * S * S + (nt + ntow) * fft
+        name, tkey, prof, part = ("k_tower_band_items (W-towers scatter + row IFFT)",
+                                  "tower_tiles", "tower_band_items_fp32_c2.txt", "band_pass")
+        b, f = tower_work(st, nv, N, n, S, H, table_bytes)
+        eff = nv * (24 + 8 + S * S * 8) + st.n_tasks * 3 * n * n * 8
     t = timers.get(tkey)
     roof = None
     if t:
         launches = t["launches_per_step"]
         avg_ms = t["ms_per_step"] / launches
-        ach = byt / launches / (avg_ms * 1e-3) / 1e9
+        byt = b[part] / launches
+        ach = byt / (avg_ms * 1e-3) / 1e9
         traffic = profiled_traffic(prof) if default_workload else None
+        tf = f[part] / launches / (avg_ms * 1e-3) / 1e12
         roof = {"kernel": name, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
-                "bytes_per_launch": byt / launches, "bytes_per_unit": unit,
-                "launches_per_step": launches, "avg_launch_ms": avg_ms,
-                "measured_GBps": (traffic / (avg_ms * 1e-3) / 1e9) if traffic else None}
-        # the VALU roof alongside (no MFMA: not a dense contraction)
-        tf = flop / launches / (avg_ms * 1e-3) / 1e12
-        roof["valu"] = {"achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                        "frac": tf / FP32_PEAK_TFLOPS, "flop_per_launch": flop / launches}
+                "bytes_per_launch": byt, "launches_per_step": launches, "avg_launch_ms": avg_ms,
+                "measured_GBps": (traffic / (avg_ms * 1e-3) / 1e9) if traffic else None,
+                "valu": {"achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": tf / FP32_PEAK_TFLOPS, "flop_per_launch": f[part] / launches},
+                "effective": {"GBps": eff / launches / (avg_ms * 1e-3) / 1e9,
+                              "bytes_per_launch": eff / launches,
+                              "bytes_per_unit": {"visibility": 24 + 8 + S * S * 8}
+                              | ({} if kind == "wproj" else {"task": 3 * n * n * 8})}}
+    job_b, job_f = sum(b.values()), sum(f.values())
     t_roof = max(job_b / (HBM_PEAK_GBS * 1e9), job_f / (FP32_PEAK_TFLOPS * 1e12))
-    job = {"bytes": job_b, "flop": job_f, "t_roof_ms": t_roof * 1e3,
-           "frac": t_roof * 1e3 / head["ms_per_step"]}
+    job = {"bytes": job_b, "flop": job_f, "bytes_by_kernel": b, "flop_by_kernel": f,
+           "t_roof_ms": t_roof * 1e3, "frac": t_roof * 1e3 / head["ms_per_step"],
+           "counts": {k: getattr(st, k) for k in ("n_records", "n_bands", "n_chunks", "n_tasks",
+                                                  "n_towers", "n_items")}}
     return roof, job
 
 
@@ -681,8 +854,10 @@ def dft_errors(ctx, fv, N, lam, theta, S, images, npix, seed=4, inner_only=False
     """Images against the FP64 direct DFT on `npix` seeded pixels (of the whole
     image, or of the inner field only), after the layout map (DFT pixel (x, y)
     is gridder pixel ((N - x) % N, (N - y) % N)) and division by the taper
-    A(l) A(m).  Reported for the whole sample and its inner-field part
-    |l|, |m| < theta / 4."""
+    A(l) A(m).  Reported over the inner field |l|, |m| < theta / 4 and over
+    the valid field, the sampled pixels where A(l) A(m) >= 1e-2 (the taper
+    division amplifies rounding by up to 100 there; beyond it, where A -> 0
+    at the field edge, the quotient says nothing about the gridder)."""
     import torch
 
     import rgamd
@@ -707,26 +882,52 @@ def dft_errors(ctx, fv, N, lam, theta, S, images, npix, seed=4, inner_only=False
     taper = rgamd.pswf_eval(math.pi * S / 4, 2.0 * (np.arange(N) - N // 2) / N)
     r, c = (N - xs) % N, (N - ys) % N
     inner = (np.abs(r - N // 2) < N // 8) & (np.abs(c - N // 2) < N // 8)
-    res = {"pixels": int(len(idx)), "dft_seconds": t_dft}
+    valid = taper[c] * taper[r] >= 1e-2
+    res = {"pixels": int(len(idx)), "pixels_valid_field": int(valid.sum()),
+           "pixels_inner_field": int(inner.sum()), "dft_seconds": t_dft}
+
+    def rel(e, sel):
+        return (float(e[sel].max() / np.abs(d[sel]).max()),
+                float(np.sqrt(np.mean(e[sel] ** 2)) / np.sqrt(np.mean(np.abs(d[sel]) ** 2))))
+
     for name, img in images.items():
         g = img.cpu().numpy()[r, c] / (taper[c] * taper[r])
         e = np.abs(g - d)
-        res[name] = {"max_rel": float(e.max() / np.abs(d).max()),
-                     "rms_rel": float(np.sqrt(np.mean(e ** 2)) / np.sqrt(np.mean(np.abs(d) ** 2))),
-                     "inner_max_rel": float(e[inner].max() / np.abs(d[inner]).max()),
-                     "inner_rms_rel": float(np.sqrt(np.mean(e[inner] ** 2)) /
-                                            np.sqrt(np.mean(np.abs(d[inner]) ** 2)))}
+        vm, vr = rel(e, valid)
+        im, ir = rel(e, inner)
+        res[name] = {"valid_max_rel": vm, "valid_rms_rel": vr, "inner_max_rel": im,
+                     "inner_rms_rel": ir}
     log(f"[bench] DFT oracle ({len(idx)} pixels, {t_dft:.1f}s): {res}")
     return res
 
 
+def host_threads():
+    """Host threads for the CPU baselines: every core this process may run on
+    (its CPU affinity), capped by OMP_NUM_THREADS when that is set (a batch
+    system's per-job core share), and the facts reported with each CPU
+    number."""
+    nproc = os.cpu_count() or 1
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = nproc
+    omp = os.environ.get("OMP_NUM_THREADS")
+    threads = aff
+    if omp and omp.strip().isdigit() and int(omp) > 0:
+        threads = min(aff, int(omp))
+    return threads, {"nproc": nproc, "affinity_cpus": aff, "OMP_NUM_THREADS": omp,
+                     "threads": threads, "cpu": _cpu_model()}
+
+
 def cpu_baseline(obs, vis128, tab_p, w_min, w_step, seconds):
-    """The oracle's OpenMP W-projection (fast build) on host cores, on a bounded
-    prefix of the same visibility stream."""
+    """The oracle's OpenMP W-projection (fast build; grid rows partitioned
+    over the threads, race-free) on all available host threads, on a bounded
+    prefix of the same visibility stream.  (The reference's CPU W-projection,
+    grid.c:187-202, is serial.)"""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle as orc
 
-    threads = min(16, os.cpu_count() or 1)
+    threads, host = host_threads()
     vis = vis128.cpu().numpy()
     table = tab_p.cpu().numpy()
     N = obs.grid_size
@@ -742,34 +943,36 @@ def cpu_baseline(obs, vis128, tab_p, w_min, w_step, seconds):
     dt = time.perf_counter() - t
     log(f"[bench] CPU W-projection: {ns} vis in {dt:.2f}s on {threads} threads")
     return {"value": ns / dt / 1e6, "unit": "Mvis/s", "cores": threads, "kind": "port",
-            "sample": f"first {ns} visibilities of the stream, W-projection (OpenMP, row-partitioned, "
-                      f"-O3 -ffast-math), 2048^2 grid",
-            "cpu": _cpu_model()}
+            "sample": f"first {ns} visibilities of the stream, W-projection (OpenMP, grid rows "
+                      f"partitioned over {threads} threads, all busy; -O3 -ffast-math), 2048^2 grid",
+            "host": host}
 
 
 def cpu_baseline_wtowers(obs, vis128, tab_t, n, margin, winc, ntowers):
-    """The oracle's OpenMP W-towers (fast build, OpenMP over towers, as
-    grid.c:414) on host cores over the whole visibility stream (W-towers time
-    is dominated by the per-(tower, floor) FFTs, so a prefix of the stream is
-    not representative).  At most one thread per active tower does work."""
+    """The oracle's OpenMP W-towers (fast build, OpenMP over towers with
+    schedule(dynamic), as grid.c:413-416) on all available host threads over
+    the whole visibility stream (W-towers time is dominated by the
+    per-(tower, floor) FFTs, so a prefix of the stream is not representative).
+    Only the active towers can use threads: at most min(threads, towers) are
+    busy.  Returns the record and the CPU grid."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import pyoracle as orc
 
-    threads = min(16, os.cpu_count() or 1)
-    ns = min(obs.n_vis, 8_000_000)
+    threads, host = host_threads()
+    ns = obs.n_vis
     vis = vis128[:ns].cpu().numpy()
     table = tab_t.cpu().numpy()
     t = time.perf_counter()
-    orc.grid_wtowers(obs.grid_size, obs.theta, obs.u[:ns], obs.v[:ns], obs.w[:ns], vis, table,
-                     -5.0, 1.0, n, margin, winc, nthreads=threads, fast=True)
+    g, _, _ = orc.grid_wtowers(obs.grid_size, obs.theta, obs.u[:ns], obs.v[:ns], obs.w[:ns], vis,
+                               table, -5.0, 1.0, n, margin, winc, nthreads=threads, fast=True)
     dt = time.perf_counter() - t
-    log(f"[bench] CPU W-towers: {ns} vis in {dt:.2f}s on {threads} threads")
     busy = max(1, min(threads, ntowers))
-    return {"value": ns / dt / 1e6, "unit": "Mvis/s", "cores": busy, "kind": "port",
-            "sample": (f"{'all' if ns == obs.n_vis else 'first'} {ns} visibilities of the workload, "
-                       f"W-towers (OpenMP over towers: {threads} threads, {ntowers} active towers; "
-                       "-O3 -ffast-math, iterative radix-2 FFT), 2048^2 grid"),
-            "cpu": _cpu_model()}
+    log(f"[bench] CPU W-towers: {ns} vis in {dt:.2f}s on {threads} threads ({busy} busy)")
+    return ({"value": ns / dt / 1e6, "unit": "Mvis/s", "cores": busy, "kind": "port",
+             "sample": (f"all {ns} visibilities of the workload, W-towers (OpenMP over towers: "
+                        f"{threads} threads started, {ntowers} active towers, so {busy} busy; "
+                        "-O3 -ffast-math, the oracle's own radix-2 FFT: FFTW is absent), 2048^2 grid"),
+             "host": host, "seconds": dt}, g)
 
 
 def _cpu_model():

@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define RG_ABI_VERSION 1
+#define RG_ABI_VERSION 2
 
 /* status codes */
 #define RG_OK 0
@@ -104,9 +104,15 @@ typedef struct rg_grid_stats {
     int64_t n_gridded;   /* visibilities gridded */
     int64_t n_dropped;   /* outside the grid / tower set (dropped, as the reference's bounds check) */
     int64_t n_bad_plane; /* W-plane index outside the table (skipped) */
-    int64_t n_items;     /* batched work items (W-projection) */
+    int64_t n_items;     /* batched work items (W-projection tiles, W-towers bands) */
     int64_t n_tasks;     /* non-empty (tower, floor) tasks (W-towers) */
     int64_t n_towers;    /* non-empty towers (W-towers) */
+    int64_t n_records;   /* records of the batched work queue (W-projection: sorted
+                            visibilities; W-towers: band records, one per band window a
+                            visibility falls in) */
+    int64_t n_bands;     /* batched W-towers: subgrid bands written (windows with records) */
+    int64_t n_chunks;    /* batched W-towers: column-pass chunks (runs of <= 32 floors of
+                            one tower), each writing one partial subimage */
 } rg_grid_stats;
 
 typedef struct rg_context rg_context;
@@ -118,7 +124,9 @@ int rg_abi_version(void);
  * memory and twiddle tables; one context per host thread. */
 int rg_context_create(rg_context **ctx, int device);
 int rg_context_destroy(rg_context *ctx);
-/* Stats of the last rg_grid on this context (synchronises the stream of that call). */
+/* Stats of the last rg_grid on this context (synchronises the stream of that call).
+ * Returns RG_ERR_RANGE (with `out` filled) when that call overran one of its
+ * internal capacity bounds, i.e. its result is incomplete. */
 int rg_context_stats(rg_context *ctx, rg_grid_stats *out);
 
 /* Kernel timers: when enabled, librg records HIP events on the launch stream
@@ -128,8 +136,8 @@ int rg_context_stats(rg_context *ctx, rg_grid_stats *out);
 #define RG_TIMER_WPROJ_TILES 0  /* batched W-projection tile kernel */
 #define RG_TIMER_WPROJ_PLAIN 1  /* plain W-projection scatter */
 #define RG_TIMER_BINNING 2      /* count + scan + scatter of the sort */
-#define RG_TIMER_TOWER_TILES 3  /* batched W-towers subgrid scatter */
-#define RG_TIMER_TOWER_ROWS 4   /* batched W-towers row IFFT pass */
+#define RG_TIMER_TOWER_TILES 3  /* batched W-towers band pass: subgrid scatter + row IFFT */
+#define RG_TIMER_TOWER_REDUCE 4 /* batched W-towers sum of partial images into subimages */
 #define RG_TIMER_TOWER_COLS 5   /* batched W-towers column IFFT + screen pass */
 #define RG_TIMER_TOWER_FINISH 6 /* W-towers forward FFTs + fold */
 #define RG_TIMER__COUNT 7

@@ -296,9 +296,16 @@ int rg_context_stats(rg_context *ctx, rg_grid_stats *out)
     out->n_items = (int64_t)c[CNT_ITEMS];
     out->n_tasks = (int64_t)c[CNT_TASKS];
     out->n_towers = (int64_t)c[CNT_TOWERS];
+    out->n_records = (int64_t)c[CNT_RECORDS];
+    out->n_bands = (int64_t)c[CNT_BANDS];
+    out->n_chunks = (int64_t)c[CNT_CHUNKS];
     // every visibility is gridded, dropped (off the grid / tower set) or has
     // no W-plane
     out->n_gridded = ctx->last_nvis - out->n_dropped - out->n_bad_plane;
+    if (c[CNT_OVERFLOW])
+        return fail(RG_ERR_RANGE,
+                    "the last call overran %llu internal capacity bound(s): its result is "
+                    "incomplete", (unsigned long long)c[CNT_OVERFLOW]);
     return RG_OK;
 }
 

@@ -161,14 +161,17 @@ struct Baseline {
 };
 
 // load_vis_group (hdf5.c:64-143): shape checks and the baseline-length filter
-// on the first uvw sample, in metres, [min_len, max_len).
-static bool read_vis_group(h5::Api *a, h5::hid_t g, Baseline &bl, double min_len, double max_len,
-                           bool filter, int64_t *total)
+// on the first uvw sample, in metres, [min_len, max_len).  Returns 1 when the
+// group is kept, 0 when it is filtered out or holds no time samples (the
+// reference reads the first uvw sample of an empty group out of bounds), and
+// a negative RG_ERR_* code when its datasets are malformed or a read fails.
+static int read_vis_group(h5::Api *a, h5::hid_t g, Baseline &bl, double min_len, double max_len,
+                          bool filter, int64_t *total)
 {
     using namespace h5;
     hid_t fds = a->dopen2(g, "frequency", P_DEFAULT), tds = a->dopen2(g, "time", P_DEFAULT);
     hid_t uds = a->dopen2(g, "uvw", P_DEFAULT), vds = a->dopen2(g, "vis", P_DEFAULT);
-    bool good = fds >= 0 && tds >= 0 && uds >= 0 && vds >= 0;
+    int st = (fds >= 0 && tds >= 0 && uds >= 0 && vds >= 0) ? 1 : RG_ERR_SHAPE;
     hsize_t fdim = 0, tdim = 0, udim[2] = {0, 0}, vdim[3] = {0, 0, 0};
     auto shape = [&](hid_t ds, int nd, hsize_t *dims, size_t esz) {
         hid_t sp = a->dget_space(ds), ty = a->dget_type(ds);
@@ -178,34 +181,41 @@ static bool read_vis_group(h5::Api *a, h5::hid_t g, Baseline &bl, double min_len
         a->tclose(ty);
         return r;
     };
-    if (good)
-        good = shape(fds, 1, &fdim, 8) && shape(tds, 1, &tdim, 8) && shape(uds, 2, udim, 8) &&
-               udim[0] == tdim && udim[1] == 3 && shape(vds, 3, vdim, 16) && vdim[0] == tdim &&
-               vdim[1] == fdim && vdim[2] == 1;
-    if (good) {
+    if (st == 1 &&
+        !(shape(fds, 1, &fdim, 8) && shape(tds, 1, &tdim, 8) && shape(uds, 2, udim, 8) &&
+          udim[0] == tdim && udim[1] == 3 && shape(vds, 3, vdim, 16) && vdim[0] == tdim &&
+          vdim[1] == fdim && vdim[2] == 1))
+        st = RG_ERR_SHAPE;
+    if (st == 1 && (tdim == 0 || fdim == 0))
+        st = 0; // nothing to grid
+    if (st == 1) {
         if (total)
             *total += (int64_t)(tdim * fdim);
         bl.T = (int)tdim;
         bl.F = (int)fdim;
         bl.uvw.resize(tdim * 3);
-        a->dread(uds, *a->t_ieee_f64le, S_ALL, S_ALL, P_DEFAULT, bl.uvw.data());
-        const double len = std::sqrt(bl.uvw[0] * bl.uvw[0] + bl.uvw[1] * bl.uvw[1]);
-        if (filter && (len < min_len || len >= max_len)) {
-            good = false;
+        if (a->dread(uds, *a->t_ieee_f64le, S_ALL, S_ALL, P_DEFAULT, bl.uvw.data()) < 0) {
+            st = RG_ERR_IO;
         } else {
-            bl.time.resize(tdim);
-            bl.freq.resize(fdim);
-            bl.vis.resize(tdim * fdim * 2);
-            a->dread(tds, *a->t_ieee_f64le, S_ALL, S_ALL, P_DEFAULT, bl.time.data());
-            a->dread(fds, *a->t_ieee_f64le, S_ALL, S_ALL, P_DEFAULT, bl.freq.data());
-            a->dread(vds, a->cpx, S_ALL, S_ALL, P_DEFAULT, bl.vis.data());
+            const double len = std::sqrt(bl.uvw[0] * bl.uvw[0] + bl.uvw[1] * bl.uvw[1]);
+            if (filter && (len < min_len || len >= max_len)) {
+                st = 0;
+            } else {
+                bl.time.resize(tdim);
+                bl.freq.resize(fdim);
+                bl.vis.resize(tdim * fdim * 2);
+                if (a->dread(tds, *a->t_ieee_f64le, S_ALL, S_ALL, P_DEFAULT, bl.time.data()) < 0 ||
+                    a->dread(fds, *a->t_ieee_f64le, S_ALL, S_ALL, P_DEFAULT, bl.freq.data()) < 0 ||
+                    a->dread(vds, a->cpx, S_ALL, S_ALL, P_DEFAULT, bl.vis.data()) < 0)
+                    st = RG_ERR_IO;
+            }
         }
     }
     if (fds >= 0) a->dclose(fds);
     if (tds >= 0) a->dclose(tds);
     if (uds >= 0) a->dclose(uds);
     if (vds >= 0) a->dclose(vds);
-    return good;
+    return st;
 }
 
 static int load_vis(const char *filename, double min_len, double max_len,
@@ -248,8 +258,11 @@ static int load_vis(const char *filename, double min_len, double max_len,
         // single-sample baseline per row, filter per row.
         Baseline data;
         int64_t total = 0;
-        if (!read_vis_group(a, g, data, -DBL_MAX, DBL_MAX, false, &total)) {
-            status = fail(RG_ERR_SHAPE, "malformed legacy visibility group in %s", filename);
+        const int rs = read_vis_group(a, g, data, -DBL_MAX, DBL_MAX, false, &total);
+        if (rs < 0) {
+            status = fail(rs, "malformed or unreadable legacy visibility group in %s", filename);
+        } else if (rs == 0) {
+            // no rows: no visibilities
         } else {
             hid_t d1 = a->dopen2(g, "antenna1", P_DEFAULT), d2 = a->dopen2(g, "antenna2", P_DEFAULT);
             std::vector<int64_t> a1(data.T), a2(data.T);
@@ -305,8 +318,14 @@ static int load_vis(const char *filename, double min_len, double max_len,
                     Baseline b;
                     b.a1 = a1;
                     b.a2 = a2;
-                    if (read_vis_group(a, g2, b, min_len, max_len, true, nullptr))
+                    // a group of the wrong shape is skipped as the reference
+                    // does (hdf5.c:88-95); a failed read is an error
+                    const int rs = read_vis_group(a, g2, b, min_len, max_len, true, nullptr);
+                    if (rs > 0)
                         out.push_back(std::move(b));
+                    else if (rs < 0 && rs != RG_ERR_SHAPE)
+                        status = fail(rs, "reading baseline %d-%d of %s failed", a1, a2,
+                                      filename);
                     a->gclose(g2);
                 }
                 a->gclose(g1);
@@ -782,7 +801,7 @@ int rg_write_vis_hdf5(const char *filename, int32_t n_bl, const int32_t *a1, con
     Api *a = api();
     if (!a)
         return fail(RG_ERR_UNSUPPORTED, "libhdf5 (>= 1.10) could not be loaded");
-    if (!filename || n_bl < 0 || T <= 0 || F <= 0 || (n_bl > 0 && (!a1 || !a2 || !uvw || !vis)) ||
+    if (!filename || n_bl < 0 || T < 0 || F <= 0 || (n_bl > 0 && (!a1 || !a2 || !uvw || !vis)) ||
         !time || !freq)
         return fail(RG_ERR_PARAM, "invalid arguments");
     hid_t f = a->fcreate(filename, ACC_TRUNC, P_DEFAULT, P_DEFAULT);

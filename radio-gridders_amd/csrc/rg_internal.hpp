@@ -142,8 +142,11 @@ enum Counter {
     CNT_TOWERS = 4,
     CNT_RECORDS = 5, // records written by the sort (one per key a visibility reaches)
     CNT_MAX_ITEMS = 6,
-    CNT_SKY_SKIPPED = 7, // W-stack predict: sky points outside the image or tables
-    CNT__N = 8
+    CNT_OVERFLOW = 7,    // capacity overruns inside a call (results incomplete): must stay 0
+    CNT_BANDS = 8,       // W-towers band windows with records (bands written)
+    CNT_CHUNKS = 9,      // W-towers column-pass chunks
+    CNT_SKY_SKIPPED = 10, // W-stack predict: sky points outside the image or tables
+    CNT__N = 11
 };
 
 int ensure_counters(rg_context *ctx);

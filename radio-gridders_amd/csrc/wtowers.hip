@@ -363,18 +363,24 @@ __global__ __launch_bounds__(256) void k_sort_items(const Rec<T> *__restrict__ i
 // Band b of a task takes the records of keys b - mk + 1 .. b (every footprint
 // that can reach rows [H b, H b + H)); they are contiguous in the binned
 // record array.  Per band key: window length and start.
+// Band windows with records are counted (the bands the pass writes).
 __global__ __launch_bounds__(256) void k_window_len(const uint32_t *__restrict__ key_start,
                                                     int64_t nkeys, int nb, int mk, uint32_t *len,
-                                                    uint32_t *wstart)
+                                                    uint32_t *wstart, unsigned long long *stats)
 {
     const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nkeys)
-        return;
-    const int b = (int)(k % nb);
-    const int64_t k0 = k - min(b, mk - 1);
-    const uint32_t w0 = key_start[k0];
-    len[k] = key_start[k + 1] - w0;
-    wstart[k] = w0;
+    uint32_t l = 0;
+    if (k < nkeys) {
+        const int b = (int)(k % nb);
+        const int64_t k0 = k - min(b, mk - 1);
+        const uint32_t w0 = key_start[k0];
+        l = key_start[k + 1] - w0;
+        len[k] = l;
+        wstart[k] = w0;
+    }
+    const unsigned long long any = __ballot(l > 0);
+    if (__lane_id() == 0 && any)
+        atomicAdd(&stats[CNT_BANDS], (unsigned long long)__popcll(any));
 }
 
 // ---- batched band pass -------------------------------------------------------------------
@@ -399,7 +405,8 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
     const int32_t *__restrict__ rank_of_slot, int t0,
     const typename Vec2<T>::type *__restrict__ kern, uint32_t kern_bytes,
     const typename Vec2<T>::type *__restrict__ twg, typename Vec2<T>::type *G,
-    typename Vec2<T>::type *spill, uint32_t spill_cap, uint32_t *spill_ctr, uint32_t *item_spill)
+    typename Vec2<T>::type *spill, uint32_t spill_cap, uint32_t *spill_ctr, uint32_t *item_spill,
+    unsigned long long *stats)
 {
     using T2 = typename Vec2<T>::type;
     constexpr int n = 1 << LOGN;
@@ -565,10 +572,13 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
             s_slot = sl;
         }
         __syncthreads();
-        // the capacity bounds every batch (at most 2 R / K parts); the check
-        // only keeps the stores inside the buffer
-        if (s_slot >= spill_cap)
+        // the capacity bounds every batch (at most 2 R / K parts); an overrun
+        // would lose this part, so it is counted as an error of the call
+        if (s_slot >= spill_cap) {
+            if (threadIdx.x == 0)
+                atomicAdd(&stats[CNT_OVERFLOW], 1ull);
             return;
+        }
         dst = spill + (size_t)s_slot * H * n;
     }
     __syncthreads();
@@ -917,6 +927,7 @@ struct BandLaunch {
     void *G, *spill;
     uint32_t spill_cap;
     uint32_t *spill_ctr, *item_spill;
+    unsigned long long *stats;
     uint32_t blocks; // bound on the batch's items
 };
 
@@ -929,7 +940,7 @@ static int launch_bands_h(const BandLaunch &a, hipStream_t s)
                        0, s, (const BandRec<T> *)a.recs, a.items, a.item_start, a.key_begin,
                        a.key_end, a.rank_of_slot, a.t0, (const T2 *)a.kern, a.kern_bytes,
                        (const T2 *)a.tw, (T2 *)a.G, (T2 *)a.spill, a.spill_cap, a.spill_ctr,
-                       a.item_spill);
+                       a.item_spill, a.stats);
     RG_HIP(hipGetLastError());
     hipLaunchKernelGGL((k_band_combine<T2, H, LOGN>), dim3(a.blocks), dim3(256), 0, s, a.items,
                        a.item_start, a.key_begin, a.key_end, a.rank_of_slot, a.t0, (T2 *)a.G,
@@ -1083,6 +1094,13 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
     kf.mk = H > 0 ? (S - 2) / H + 2 : 1; // bands an S-row footprint can reach
     const int64_t nslots = (int64_t)cc * cc * nwp;
     const int64_t nkeys = nslots * kf.nk;
+    // band records are addressed by 32-bit offsets: the expanded array holds
+    // up to mk records per visibility
+    if (!plain && (int64_t)kf.mk * vis->n >= ((int64_t)1 << 32))
+        return fail(RG_ERR_PARAM,
+                    "W-towers call too large: %lld visibilities x %d bands per footprint exceed "
+                    "2^32 band records; split the stream",
+                    (long long)vis->n, kf.mk);
     // The sort makes one item per key (plain mode reads whole tasks).  Batched
     // band work items of at most K records per band window are built after it.
     const uint32_t K = p->vis_per_item > 0 ? (uint32_t)p->vis_per_item : 2048u;
@@ -1179,6 +1197,11 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
     int max_chunks = 0;
     for (size_t b = 0; b + 1 < batch_chunk0.size(); ++b)
         max_chunks = std::max(max_chunks, batch_chunk0[b + 1] - batch_chunk0[b]);
+    if (!plain) {
+        const unsigned long long nch = (unsigned long long)chunk_first.size();
+        RG_HIP(hipMemcpyAsync(ctx->counters.as<unsigned long long>() + CNT_CHUNKS, &nch,
+                              sizeof(nch), hipMemcpyHostToDevice, s));
+    }
 
     std::vector<int32_t> tables;
     const size_t o_rot = 0, o_slot = o_rot + (size_t)cc * cc, o_wp = o_slot + ntasks,
@@ -1297,7 +1320,8 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
             uint32_t *len = ctx->scratch[0].as<uint32_t>();
             uint32_t *prefix = ctx->scratch[3].as<uint32_t>();
             hipLaunchKernelGGL(k_window_len, dim3((unsigned)((nkeys + 255) / 256)), dim3(256), 0, s,
-                               (const uint32_t *)br.vis_start, nkeys, NB, kf.mk, len, d_wstart);
+                               (const uint32_t *)br.vis_start, nkeys, NB, kf.mk, len, d_wstart,
+                               ctx->counters.as<unsigned long long>());
             RG_HIP(hipGetLastError());
             RG_TRY(items_from_lengths(ctx, len, nkeys, K, prefix, item_start_w, items_w, s));
             hipLaunchKernelGGL((k_sort_items<T>),
@@ -1338,6 +1362,7 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
         bl.spill = spill;
         bl.spill_cap = (uint32_t)max_spill;
         bl.item_spill = item_spill;
+        bl.stats = ctx->counters.as<unsigned long long>();
         int b = 0;
         for (int t0 = 0; t0 < ntasks; t0 += Q, ++b) {
             const int t1 = std::min(ntasks, t0 + Q);
@@ -1362,7 +1387,7 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
             }
             const int rb = batch_run0[b], nr = batch_run0[b + 1] - batch_run0[b];
             {
-                KernelTimer kt(ctx, RG_TIMER_TOWER_ROWS, s);
+                KernelTimer kt(ctx, RG_TIMER_TOWER_REDUCE, s);
                 int64_t gx = (int64_t)((nn + 255) / 256);
                 if (gx > 1024)
                     gx = 1024;

@@ -33,7 +33,7 @@ RG_DFT_MIXED = 1
 RG_IMAGE_GPU = 0
 RG_IMAGE_CPU = 1
 
-TIMERS = {"wproj_tiles": 0, "wproj_plain": 1, "binning": 2, "tower_tiles": 3, "tower_rows": 4,
+TIMERS = {"wproj_tiles": 0, "wproj_plain": 1, "binning": 2, "tower_tiles": 3, "tower_reduce": 4,
           "tower_cols": 5, "tower_finish": 6}
 
 
@@ -74,7 +74,8 @@ class WStackParams(C.Structure):
 class GridStats(C.Structure):
     _fields_ = [("n_vis", C.c_int64), ("n_gridded", C.c_int64), ("n_dropped", C.c_int64),
                 ("n_bad_plane", C.c_int64), ("n_items", C.c_int64), ("n_tasks", C.c_int64),
-                ("n_towers", C.c_int64)]
+                ("n_towers", C.c_int64), ("n_records", C.c_int64), ("n_bands", C.c_int64),
+                ("n_chunks", C.c_int64)]
 
 
 # (name, restype, argtypes) of every exported symbol of include/rg.h

@@ -31,7 +31,7 @@ def test_every_declared_symbol_is_exported_and_bound():
 
 def test_version_and_error_codes():
     L = rgamd.lib()
-    assert L.rg_abi_version() == 1
+    assert L.rg_abi_version() == 2
     out = np.empty(16, np.complex128)
     assert L.rg_wkern_generate(out.ctypes.data, 7, 1, 2, 2, 0.1, 0.0, 1.0, 0, 0.0, 1) == rgamd.RG_ERR_PARAM
     assert b"type" in L.rg_last_error()
@@ -45,3 +45,20 @@ def test_null_context_rejected():
     p = rgamd.params(64, 0.1)
     assert L.rg_grid(None, C.byref(p), None, None, None, 0, None) == rgamd.RG_ERR_PARAM
     assert L.rg_context_destroy(None) == rgamd.RG_OK
+
+
+def _struct_fields(name):
+    src = open(os.path.join(REPO, "include", "rg.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name), src, flags=re.S).group(1)
+    return [m.group(2) for m in re.finditer(r"\b([a-z0-9_]+)\s+\**\s*([a-z0-9_]+)\s*;", body)]
+
+
+@pytest.mark.parametrize("cname,pyname", [("rg_grid_stats", "GridStats"),
+                                          ("rg_grid_params", "GridParams"),
+                                          ("rg_flat_vis", "FlatVis"), ("rg_wkern", "WKern")])
+def test_struct_layouts_match_the_header(cname, pyname):
+    """The ctypes structures mirror the header's field order (a mismatch would
+    shift every later field)."""
+    ours = [f[0] for f in getattr(rgamd, pyname)._fields_]
+    assert _struct_fields(cname) == ours

@@ -70,3 +70,18 @@ def test_missing_and_foreign_files(tmp_path):
     with pytest.raises(rgamd.RGError) as e:
         rgamd.load_flat_vis(str(p))
     assert e.value.code == rgamd.RG_ERR_IO
+
+
+def test_empty_baseline_groups_are_skipped(tmp_path):
+    """Groups with no time samples hold nothing to grid: they are skipped
+    instead of having their first uvw sample read (which the reference reads
+    out of bounds, hdf5.c:104-106); the other baselines of a file still load."""
+    path = str(tmp_path / "empty.h5")
+    a = np.array([0, 0], np.int32)
+    b = np.array([1, 2], np.int32)
+    rgamd.write_vis_hdf5(path, a, b, np.zeros(0), np.array([1.5e8]), np.zeros((2, 0, 3)),
+                         np.zeros((2, 0, 1), np.complex128))
+    u, v, w, vis = rgamd.load_flat_vis(path)
+    assert len(u) == 0 and len(vis) == 0
+    u, v, w, vis = rgamd.load_flat_vis(path, 10.0, 1000.0)  # the filter reads no sample either
+    assert len(u) == 0

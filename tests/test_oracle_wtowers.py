@@ -102,3 +102,44 @@ def test_single_visibility_fringe_through_screens():
     assert np.abs(got - expect)[inner].max() < 1e-3
     wrong = np.exp(2j * np.pi * (u * L + v * M2 - w * (nn - 1)))
     assert np.abs(got - wrong)[inner].max() > 1e-2
+
+
+def test_fringes_on_several_floors_of_one_tower():
+    """Three visibilities of one tower on floors -2, 3 and 7 (gaps of 5 and 4
+    floors, off-floor residual w): the image is the closed-form sum of their
+    fringes.  This pins the inter-floor step subimg = F^(wp - last_wp) subimg +
+    IFFT(subgrid) (grid.c:473-480) and the w = 0 transfer F^-last_wp
+    (grid.c:489-496): a power taken from the wrong floor puts a phase error of
+    2 pi k winc (1 - n) on the earlier floors' fringes."""
+    N, theta, S, R, n, M, winc = 128, 0.1, 16, 8, 64, 16, 10.0
+    table = rgamd.make_wkernels(11, R, S, theta, -5.0, 1.0)
+    u = np.array([2.375, 1.125, 3.0]) / theta
+    v = np.array([-3.5, -2.25, -4.125]) / theta
+    w = np.array([-21.0, 31.0, 68.5])  # floors -2, 3, 7 (residuals -1, 1, -1.5)
+    vis = np.array([1.0, 0.7 - 0.4j, -0.5 + 0.9j])
+    chunk = n - M
+    assert len({(orc.tower_index(a, theta, N, chunk), orc.tower_index(b, theta, N, chunk))
+                for a, b in zip(u, v)}) == 1
+    assert [orc.floor_index(x, winc) for x in w] == [-2, 3, 7]
+    g, _, bad = orc.grid_wtowers(N, theta, u, v, w, vis, table, -5.0, 1.0, n, M, winc)
+    assert bad == 0
+    img = orc.grid_to_image(g)
+    c = np.pi * S / 4
+    lm = theta * (np.arange(N) - N // 2) / N
+    A = np.array([rgamd.pswf(c, 2 * x / theta) for x in lm])
+    L, M2 = np.meshgrid(lm, lm)
+    nn = np.sqrt(1 - L ** 2 - M2 ** 2)
+    inner = (np.abs(L) < 0.3 * theta) & (np.abs(M2) < 0.3 * theta)
+    got = img / np.outer(A, A)
+
+    def fringes(wv):
+        return sum(a * np.exp(2j * np.pi * (x * L + y * M2 + z * (nn - 1)))
+                   for a, x, y, z in zip(vis, u, v, wv))
+
+    # kernel interpolation error ~1e-3 per unit fringe (R = 8, 16 taps)
+    assert np.abs(got - fringes(w))[inner].max() < 5e-3
+    # each visibility's floor term moved by one floor is caught (>= 0.047)
+    for k in range(3):
+        ws = w.copy()
+        ws[k] += winc
+        assert np.abs(got - fringes(ws))[inner].max() > 2e-2
