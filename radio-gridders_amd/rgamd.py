@@ -176,7 +176,8 @@ def _stream(stream):
 
 
 def flat_vis(u, v, w, vis) -> FlatVis:
-    """Descriptor of device tensors: u, v, w float64; vis complex128 or complex64."""
+    """Descriptor of device tensors: u, v, w float64; vis complex128 or complex64.
+    The descriptor keeps the tensors alive: it holds their raw addresses."""
     import torch
     assert u.dtype == torch.float64 and v.dtype == torch.float64 and w.dtype == torch.float64
     assert vis.dtype in (torch.complex128, torch.complex64)
@@ -184,18 +185,23 @@ def flat_vis(u, v, w, vis) -> FlatVis:
     assert v.numel() == n and w.numel() == n and vis.numel() == n
     for t in (u, v, w, vis):
         assert t.is_contiguous() and t.is_cuda
-    return FlatVis(_ptr(u), _ptr(v), _ptr(w), _ptr(vis), n,
-                   RG_C128 if vis.dtype == torch.complex128 else RG_C64, 0)
+    d = FlatVis(_ptr(u), _ptr(v), _ptr(w), _ptr(vis), n,
+                RG_C128 if vis.dtype == torch.complex128 else RG_C64, 0)
+    d._keep = (u, v, w, vis)
+    return d
 
 
 def wkern_desc(table, w_min: float, w_step: float) -> WKern:
-    """Descriptor of a device table [P][R][R][S][S] (complex128 or complex64)."""
+    """Descriptor of a device table [P][R][R][S][S] (complex128 or complex64);
+    keeps the table alive."""
     import torch
     assert table.dim() == 5 and table.is_contiguous() and table.is_cuda
     P, R, R2, S, S2 = table.shape
     assert R == R2 and S == S2
     typ = RG_C128 if table.dtype == torch.complex128 else RG_C64
-    return WKern(_ptr(table), typ, P, R, S, float(w_min), float(w_step))
+    d = WKern(_ptr(table), typ, P, R, S, float(w_min), float(w_step))
+    d._keep = table
+    return d
 
 
 def params(grid_size, theta, mode=RG_MODE_BATCHED, subgrid=0, margin=0, winc=0.0, tile=0,
