@@ -162,6 +162,23 @@ int rg_grid_channels(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
                      const int64_t *offsets, int32_t n_chan, const rg_wkern *wk,
                      void *grids, int32_t grid_type, void *stream);
 
+/* Allocation-free batched W-projection (no counterpart in the reference): for
+ * callers that cannot allocate device memory -- code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc.  Same parameters and result
+ * as rg_grid with RG_MODE_BATCHED and subgrid_size 0 (FP32 or FP64 kernels),
+ * on the current HIP device, with no context: the sort's working memory is a
+ * 64 MiB device array inside librg (one per device, held until the process
+ * ends), enough for about 2.0 M visibilities (FP32 kernels) or 1.4 M (FP64)
+ * per call on a 2048^2 grid; a call that needs more returns RG_ERR_PARAM.
+ * accumulate = 0 zeroes the grid first (as rg_grid), 1 adds into it, so a
+ * longer stream can be split over calls.  Calls on one device share that
+ * array: calls that overlap in time on one device (other streams or host
+ * threads) race.  No call counters are reported (rg_context_stats does not
+ * see these calls); visibilities off the grid or outside the table's w range
+ * are skipped as in rg_grid. */
+int rg_grid_noalloc(const rg_grid_params *p, const rg_flat_vis *vis, const rg_wkern *wk,
+                    void *grid, int32_t grid_type, int32_t accumulate, void *stream);
+
 /* shift -> unnormalised inverse FFT -> shift, in place (grid_cuda.cu:907-918,
  * fft_shift_kernel common_kernels.cuh:17-32). */
 int rg_grid_to_image(rg_context *ctx, void *grid, int32_t type, int32_t n, void *stream);

@@ -152,6 +152,40 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t *count,
     }
 }
 
+int bin_plan(int64_t n, int64_t nkeys, uint32_t per_item, int64_t max_counters, size_t rec_bytes,
+             BinPlan &pl)
+{
+    // record indices are 32-bit
+    if (n >= (int64_t)0x7fffffff)
+        return fail(RG_ERR_PARAM, "visibility stream too long for one call (%lld)", (long long)n);
+    if (nkeys <= 0 || nkeys >= (int64_t)0x7fffffff)
+        return fail(RG_ERR_PARAM, "invalid key space %lld", (long long)nkeys);
+    if (per_item == 0)
+        return fail(RG_ERR_PARAM, "work items need at least one record");
+    pl.nblocks_scan = (nkeys + SCAN_TILE - 1) / SCAN_TILE;
+    if (pl.nblocks_scan > 1024 * 64)
+        return fail(RG_ERR_PARAM, "key space too large (%lld)", (long long)nkeys);
+    pl.max_items = nkeys + n / per_item + 1;
+    // counter replicas: up to 16, at most max_counters counters in all
+    // (replicas spread the atomics of hot keys; a stream without hot keys
+    // is better served by fewer counters that stay in cache)
+    pl.rp.stride = (nkeys + 1 + 63) / 64 * 64;
+    uint32_t nrep = 16;
+    while (nrep > 1 && (int64_t)nrep * pl.rp.stride > max_counters)
+        nrep >>= 1;
+    pl.rp.mask = nrep - 1;
+    const int64_t n1 = n > 0 ? n : 1;
+    pl.bytes[BIN_COUNT] = sizeof(uint32_t) * nrep * pl.rp.stride;
+    pl.bytes[BIN_VIS_START] = sizeof(uint32_t) * (nkeys + 1);
+    pl.bytes[BIN_ITEM_START] = sizeof(uint32_t) * (nkeys + 1);
+    pl.bytes[BIN_CURSOR] = sizeof(uint32_t) * nrep * pl.rp.stride;
+    pl.bytes[BIN_ITEMS] = sizeof(Item) * pl.max_items;
+    pl.bytes[BIN_RECS] = rec_bytes * n1;
+    pl.bytes[BIN_BSUMS] = sizeof(uint2) * (pl.nblocks_scan + 1);
+    pl.bytes[BIN_KRP] = sizeof(uint4) * n1;
+    return RG_OK;
+}
+
 int items_from_lengths(rg_context *ctx, const uint32_t *len, int64_t nkeys, uint32_t per_item,
                        uint32_t *prefix, uint32_t *item_start, Item *items, hipStream_t s)
 {

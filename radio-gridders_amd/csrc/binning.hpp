@@ -146,8 +146,7 @@ __global__ __launch_bounds__(SCAN_BLOCK) void k_scan_down(const uint32_t *count,
                                                           uint32_t *item_start, uint32_t *cursor,
                                                           Item *items);
 
-// Host driver for count + scan + items + scatter.  Scratch layout is
-// managed by the caller through the context's scratch slots 0..5.
+// Host driver for count + scan + items + scatter.
 struct BinResult {
     uint32_t *vis_start;  // [nkeys + 1] (last entry written by the scan)
     uint32_t *item_start; // [nkeys + 1]
@@ -156,6 +155,43 @@ struct BinResult {
     int64_t max_items;
 };
 
+// Device arrays of one sort and their sizes (bin_plan).
+enum BinArray {
+    BIN_COUNT = 0,      // per-replica key counters
+    BIN_VIS_START = 1,  // first record of every key
+    BIN_ITEM_START = 2, // first work item of every key
+    BIN_CURSOR = 3,     // per-replica scatter starts
+    BIN_ITEMS = 4,      // work items
+    BIN_RECS = 5,       // sorted records
+    BIN_BSUMS = 6,      // scan block totals
+    BIN_KRP = 7,        // per visibility: key, rank, position, tap index
+    BIN__N = 8
+};
+
+struct BinPlan {
+    Replicas rp;
+    int64_t nblocks_scan, max_items;
+    size_t bytes[BIN__N];
+};
+
+struct BinBufs {
+    uint32_t *count, *vis_start, *item_start, *cursor;
+    Item *items;
+    void *recs;
+    uint2 *bsums;
+    uint4 *krp;
+    unsigned long long *stats; // CNT__N call counters
+};
+
+// Validates the sort's sizes and fills the plan (no device work).
+int bin_plan(int64_t n, int64_t nkeys, uint32_t per_item, int64_t max_counters, size_t rec_bytes,
+             BinPlan &pl);
+// The sort into caller-provided arrays of the plan's sizes; timer_ctx (may be
+// NULL) receives the RG_TIMER_BINNING events.
+template <typename Keyer, typename T>
+int bin_run(const Keyer &kf, int64_t n, int64_t nkeys, uint32_t per_item, const BinPlan &pl,
+            const BinBufs &b, rg_context *timer_ctx, hipStream_t s, BinResult &out);
+// bin_run into the context's scratch slots 0-6 and 15.
 template <typename Keyer, typename T>
 int bin_visibilities(rg_context *ctx, const Keyer &kf, int64_t n, int64_t nkeys, uint32_t per_item,
                      hipStream_t s, BinResult &out, int64_t max_counters = (int64_t)1 << 22);

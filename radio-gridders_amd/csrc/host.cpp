@@ -369,6 +369,23 @@ int rg_grid_channels(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
     return grid_common(ctx, p, vis, offsets, n_chan, wk, grids, grid_type, stream);
 }
 
+int rg_grid_noalloc(const rg_grid_params *p, const rg_flat_vis *vis, const rg_wkern *wk,
+                    void *grid, int32_t grid_type, int32_t accumulate, void *stream)
+{
+    RG_TRY(check_vis(vis));
+    RG_TRY(check_wkern(wk));
+    RG_TRY(check_params(p, wk));
+    if (!grid)
+        return fail(RG_ERR_PARAM, "grid is NULL");
+    if (grid_type != RG_C128 && grid_type != RG_C64)
+        return fail(RG_ERR_PARAM, "unknown grid type %d", grid_type);
+    if (p->subgrid_size != 0 || p->mode != RG_MODE_BATCHED)
+        return fail(RG_ERR_UNSUPPORTED,
+                    "rg_grid_noalloc grids batched W-projection only (subgrid_size 0, "
+                    "RG_MODE_BATCHED)");
+    return wproj_noalloc(p, vis, wk, grid, grid_type, accumulate != 0, (hipStream_t)stream);
+}
+
 int rg_grid_to_image(rg_context *ctx, void *grid, int32_t type, int32_t n, void *stream)
 {
     if (!ctx || !grid)

@@ -170,6 +170,10 @@ int wproj_plain(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis
 int wproj_batched(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis,
                   const int64_t *offsets, int n_chan, const rg_wkern *wk, void *grid,
                   int grid_type, hipStream_t s);
+// batched W-projection into the library's static scratch (no allocation)
+int wproj_noalloc(const rg_grid_params *p, const rg_flat_vis *vis, const rg_wkern *wk, void *grid,
+                  int grid_type, int accumulate, hipStream_t s);
+size_t noalloc_scratch_bytes();
 int wtowers_run(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis,
                 const rg_wkern *wk, void *grid, int grid_type, hipStream_t s);
 // in-place FFT of nrows contiguous rows of length n (power of two 8..4096)

@@ -11,6 +11,7 @@
 // the sort (FP64, same formulas), then accumulated per tile in LDS by the
 // Romein kernel of tiles.hpp.
 #include <algorithm>
+#include <mutex>
 
 #include "binning_impl.hpp"
 #include "tiles.hpp"
@@ -192,8 +193,8 @@ struct WprojKeyer {
 };
 
 template <typename T, int S, int TILE, bool COMP, typename GT>
-static int launch_tiles(const BinResult &br, rg_context *ctx, const rg_wkern *wk, void *grid,
-                        int N, int ntx, int ntiles, hipStream_t s)
+static int launch_tiles(const BinResult &br, rg_context *timer_ctx, unsigned long long *stats,
+                        const rg_wkern *wk, void *grid, int N, int ntx, int ntiles, hipStream_t s)
 {
     TileSink sink;
     sink.base = grid;
@@ -202,36 +203,97 @@ static int launch_tiles(const BinResult &br, rg_context *ctx, const rg_wkern *wk
     sink.ntiles = ntiles;
     sink.group_map = nullptr;
     sink.group_offset = 0;
-    KernelTimer kt(ctx, RG_TIMER_WPROJ_TILES, s);
+    KernelTimer kt(timer_ctx, RG_TIMER_WPROJ_TILES, s);
     if (br.max_items > 0)
         hipLaunchKernelGGL((k_tiles<T, S, TILE, COMP, GT>), dim3((unsigned)br.max_items),
                            dim3(S * S), 0, s, (const Rec<T> *)br.recs, (const Item *)br.items, 0u,
-                           0u, (const unsigned long long *)ctx->counters.as<unsigned long long>(),
+                           0u, (const unsigned long long *)stats,
                            (const typename Vec2<T>::type *)wk->data, sink);
     RG_HIP(hipGetLastError());
     return RG_OK;
 }
 
 template <typename T, int S, int TILE>
-static int dispatch_comp_grid(const BinResult &br, rg_context *ctx, const rg_wkern *wk,
-                              void *grid, int grid_type, bool comp, int N, int ntx, int ntiles,
-                              hipStream_t s)
+static int dispatch_comp_grid(const BinResult &br, rg_context *timer_ctx,
+                              unsigned long long *stats, const rg_wkern *wk, void *grid,
+                              int grid_type, bool comp, int N, int ntx, int ntiles, hipStream_t s)
 {
     if constexpr (std::is_same<T, double>::value) {
         if (grid_type != RG_C128)
             return fail(RG_ERR_PARAM, "FP64 kernels need an RG_C128 grid");
-        return launch_tiles<double, S, TILE, false, double>(br, ctx, wk, grid, N, ntx, ntiles, s);
+        return launch_tiles<double, S, TILE, false, double>(br, timer_ctx, stats, wk, grid, N, ntx,
+                                                            ntiles, s);
     } else {
         if (grid_type == RG_C128)
-            return comp ? launch_tiles<float, S, TILE, true, double>(br, ctx, wk, grid, N, ntx,
-                                                                      ntiles, s)
-                        : launch_tiles<float, S, TILE, false, double>(br, ctx, wk, grid, N, ntx,
-                                                                       ntiles, s);
-        return comp ? launch_tiles<float, S, TILE, true, float>(br, ctx, wk, grid, N, ntx, ntiles,
-                                                                 s)
-                    : launch_tiles<float, S, TILE, false, float>(br, ctx, wk, grid, N, ntx,
-                                                                  ntiles, s);
+            return comp ? launch_tiles<float, S, TILE, true, double>(br, timer_ctx, stats, wk, grid,
+                                                                      N, ntx, ntiles, s)
+                        : launch_tiles<float, S, TILE, false, double>(br, timer_ctx, stats, wk,
+                                                                       grid, N, ntx, ntiles, s);
+        return comp ? launch_tiles<float, S, TILE, true, float>(br, timer_ctx, stats, wk, grid, N,
+                                                                 ntx, ntiles, s)
+                    : launch_tiles<float, S, TILE, false, float>(br, timer_ctx, stats, wk, grid, N,
+                                                                  ntx, ntiles, s);
     }
+}
+
+template <typename T>
+static int dispatch_tiles(const BinResult &br, rg_context *timer_ctx, unsigned long long *stats,
+                          const rg_wkern *wk, void *grid, int grid_type, bool comp, int N,
+                          int tile, int ntx, int ntiles, hipStream_t s)
+{
+    if (wk->size == 16)
+        return tile == 32 ? dispatch_comp_grid<T, 16, 32>(br, timer_ctx, stats, wk, grid,
+                                                          grid_type, comp, N, ntx, ntiles, s)
+                          : dispatch_comp_grid<T, 16, 48>(br, timer_ctx, stats, wk, grid,
+                                                          grid_type, comp, N, ntx, ntiles, s);
+    return tile == 32 ? dispatch_comp_grid<T, 8, 32>(br, timer_ctx, stats, wk, grid, grid_type,
+                                                     comp, N, ntx, ntiles, s)
+                      : dispatch_comp_grid<T, 8, 48>(br, timer_ctx, stats, wk, grid, grid_type,
+                                                     comp, N, ntx, ntiles, s);
+}
+
+// Tile geometry and sort parameters of batched W-projection.
+struct WprojShape {
+    int tile, ntx, ntiles;
+    uint32_t per_item;
+};
+
+static int wproj_shape(const rg_grid_params *p, const rg_wkern *wk, WprojShape &sh)
+{
+    const int S = wk->size;
+    sh.tile = p->tile > 0 ? p->tile : 32;
+    if (sh.tile != 32 && sh.tile != 48)
+        return fail(RG_ERR_UNSUPPORTED, "batched tile must be 32 or 48 (got %d)", sh.tile);
+    if (S != 8 && S != 16)
+        return fail(RG_ERR_UNSUPPORTED, "batched W-projection supports kernel sizes 8 and 16 (got %d)", S);
+    sh.ntx = (p->grid_size + S + sh.tile - 1) / sh.tile;
+    sh.ntiles = sh.ntx * sh.ntx;
+    sh.per_item = p->vis_per_item > 0 ? (uint32_t)p->vis_per_item : 2048u;
+    return RG_OK;
+}
+
+static WprojKeyer wproj_keyer(const rg_grid_params *p, const rg_flat_vis *vis, const rg_wkern *wk,
+                              const WprojShape &sh, const int64_t *d_off, int n_chan)
+{
+    WprojKeyer kf;
+    kf.u = vis->u;
+    kf.v = vis->v;
+    kf.w = vis->w;
+    kf.vis = vis->vis;
+    kf.vis_type = vis->vis_type;
+    kf.N = p->grid_size;
+    kf.S = wk->size;
+    kf.R = wk->oversampling;
+    kf.P = wk->plane_count;
+    kf.theta = p->theta;
+    kf.w_min = wk->w_min;
+    kf.w_step = wk->w_step;
+    kf.tile = sh.tile;
+    kf.ntx = sh.ntx;
+    kf.ntiles = sh.ntiles;
+    kf.chan_off = d_off;
+    kf.nchan = n_chan;
+    return kf;
 }
 
 template <typename T>
@@ -239,15 +301,9 @@ static int wproj_batched_t(rg_context *ctx, const rg_grid_params *p, const rg_fl
                            const int64_t *offsets, int n_chan, const rg_wkern *wk, void *grid,
                            int grid_type, hipStream_t s)
 {
-    const int N = p->grid_size, S = wk->size;
-    const int tile = p->tile > 0 ? p->tile : 32;
-    if (tile != 32 && tile != 48)
-        return fail(RG_ERR_UNSUPPORTED, "batched tile must be 32 or 48 (got %d)", tile);
-    if (S != 8 && S != 16)
-        return fail(RG_ERR_UNSUPPORTED, "batched W-projection supports kernel sizes 8 and 16 (got %d)", S);
-    const int ntx = (N + S + tile - 1) / tile;
-    const int ntiles = ntx * ntx;
-    const uint32_t per_item = p->vis_per_item > 0 ? (uint32_t)p->vis_per_item : 2048u;
+    const int N = p->grid_size;
+    WprojShape sh;
+    RG_TRY(wproj_shape(p, wk, sh));
 
     const int64_t *d_off = nullptr;
     if (n_chan > 1) {
@@ -267,38 +323,12 @@ static int wproj_batched_t(rg_context *ctx, const rg_grid_params *p, const rg_fl
     const size_t gbytes = (size_t)(grid_type == RG_C128 ? 16 : 8) * N * N * n_chan;
     RG_TRY(zero_fill(grid, gbytes, s));
 
-    WprojKeyer kf;
-    kf.u = vis->u;
-    kf.v = vis->v;
-    kf.w = vis->w;
-    kf.vis = vis->vis;
-    kf.vis_type = vis->vis_type;
-    kf.N = N;
-    kf.S = S;
-    kf.R = wk->oversampling;
-    kf.P = wk->plane_count;
-    kf.theta = p->theta;
-    kf.w_min = wk->w_min;
-    kf.w_step = wk->w_step;
-    kf.tile = tile;
-    kf.ntx = ntx;
-    kf.ntiles = ntiles;
-    kf.chan_off = d_off;
-    kf.nchan = n_chan;
-
+    const WprojKeyer kf = wproj_keyer(p, vis, wk, sh, d_off, n_chan);
     BinResult br;
-    RG_TRY((bin_visibilities<WprojKeyer, T>(ctx, kf, vis->n, (int64_t)ntiles * n_chan, per_item, s,
-                                            br)));
-    const bool comp = p->compensated != 0;
-    if (S == 16)
-        return tile == 32 ? dispatch_comp_grid<T, 16, 32>(br, ctx, wk, grid, grid_type, comp, N, ntx,
-                                                          ntiles, s)
-                          : dispatch_comp_grid<T, 16, 48>(br, ctx, wk, grid, grid_type, comp, N, ntx,
-                                                          ntiles, s);
-    return tile == 32 ? dispatch_comp_grid<T, 8, 32>(br, ctx, wk, grid, grid_type, comp, N, ntx,
-                                                     ntiles, s)
-                      : dispatch_comp_grid<T, 8, 48>(br, ctx, wk, grid, grid_type, comp, N, ntx,
-                                                     ntiles, s);
+    RG_TRY((bin_visibilities<WprojKeyer, T>(ctx, kf, vis->n, (int64_t)sh.ntiles * n_chan,
+                                            sh.per_item, s, br)));
+    return dispatch_tiles<T>(br, ctx, ctx->counters.as<unsigned long long>(), wk, grid, grid_type,
+                             p->compensated != 0, N, sh.tile, sh.ntx, sh.ntiles, s);
 }
 
 int wproj_batched(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis,
@@ -310,4 +340,96 @@ int wproj_batched(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *v
     return wproj_batched_t<double>(ctx, p, vis, offsets, n_chan, wk, grid, grid_type, s);
 }
 
+// ---- allocation-free batched W-projection ------------------------------------------------
+// One file-scope device array serves every call on a device: the sort's
+// counters, key starts, work items, records and per-visibility keys, and the
+// call counters.  Host code never passes the array's name to a kernel: its
+// device address on the current device comes from hipGetSymbolAddress, looked
+// up once per device and kept.  Calls on one device share the array, so calls
+// that overlap in time on one device (several streams or host threads) race.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[NOALLOC_MAX_DEVICES];
+
+static int noalloc_scratch(void **out)
+{
+    int dev = 0;
+    RG_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+        return fail(RG_ERR_UNSUPPORTED, "device %d beyond the %d devices of the static scratch",
+                    dev, NOALLOC_MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        RG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (!ptr)
+            return fail(RG_ERR_HIP, "no device address for the static scratch on device %d", dev);
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return RG_OK;
+}
+
+size_t noalloc_scratch_bytes() { return NOALLOC_SCRATCH_BYTES; }
+
+template <typename T>
+static int wproj_noalloc_t(const rg_grid_params *p, const rg_flat_vis *vis, const rg_wkern *wk,
+                           void *grid, int grid_type, int accumulate, hipStream_t s)
+{
+    const int N = p->grid_size;
+    WprojShape sh;
+    RG_TRY(wproj_shape(p, wk, sh));
+    BinPlan pl;
+    RG_TRY(bin_plan(vis->n, sh.ntiles, sh.per_item, (int64_t)1 << 22, sizeof(Rec<T>), pl));
+    // carve the arrays from the static scratch, 256-byte aligned
+    size_t off[BIN__N + 1];
+    size_t at = (sizeof(unsigned long long) * CNT__N + 255) / 256 * 256; // call counters first
+    for (int k = 0; k < BIN__N; ++k) {
+        off[k] = at;
+        at += (pl.bytes[k] + 255) / 256 * 256;
+    }
+    off[BIN__N] = at;
+    if (at > NOALLOC_SCRATCH_BYTES)
+        return fail(RG_ERR_PARAM,
+                    "rg_grid_noalloc: %lld visibilities on a %d^2 grid need %zu bytes of scratch; "
+                    "the static scratch holds %zu: split the stream over calls with accumulate = 1",
+                    (long long)vis->n, N, at, NOALLOC_SCRATCH_BYTES);
+    void *base = nullptr;
+    RG_TRY(noalloc_scratch(&base));
+    char *b0 = static_cast<char *>(base);
+    BinBufs b;
+    b.stats = reinterpret_cast<unsigned long long *>(b0);
+    b.count = reinterpret_cast<uint32_t *>(b0 + off[BIN_COUNT]);
+    b.vis_start = reinterpret_cast<uint32_t *>(b0 + off[BIN_VIS_START]);
+    b.item_start = reinterpret_cast<uint32_t *>(b0 + off[BIN_ITEM_START]);
+    b.cursor = reinterpret_cast<uint32_t *>(b0 + off[BIN_CURSOR]);
+    b.items = reinterpret_cast<Item *>(b0 + off[BIN_ITEMS]);
+    b.recs = b0 + off[BIN_RECS];
+    b.bsums = reinterpret_cast<uint2 *>(b0 + off[BIN_BSUMS]);
+    b.krp = reinterpret_cast<uint4 *>(b0 + off[BIN_KRP]);
+    // the previous call's counters are still there: start from zero
+    RG_TRY(zero_fill(b.stats, sizeof(unsigned long long) * CNT__N, s));
+    if (!accumulate)
+        RG_TRY(zero_fill(grid, (size_t)(grid_type == RG_C128 ? 16 : 8) * N * N, s));
+    if (vis->n == 0)
+        return RG_OK;
+    const WprojKeyer kf = wproj_keyer(p, vis, wk, sh, nullptr, 1);
+    BinResult br;
+    RG_TRY((bin_run<WprojKeyer, T>(kf, vis->n, sh.ntiles, sh.per_item, pl, b, nullptr, s, br)));
+    return dispatch_tiles<T>(br, nullptr, b.stats, wk, grid, grid_type, p->compensated != 0, N,
+                             sh.tile, sh.ntx, sh.ntiles, s);
+}
+
+int wproj_noalloc(const rg_grid_params *p, const rg_flat_vis *vis, const rg_wkern *wk, void *grid,
+                  int grid_type, int accumulate, hipStream_t s)
+{
+    if (wk->type == RG_C64)
+        return wproj_noalloc_t<float>(p, vis, wk, grid, grid_type, accumulate, s);
+    return wproj_noalloc_t<double>(p, vis, wk, grid, grid_type, accumulate, s);
+}
+
 } // namespace rg
+

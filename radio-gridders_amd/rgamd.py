@@ -94,6 +94,8 @@ SIGNATURES = [
     ("rg_grid", C.c_int, [_P, C.POINTER(GridParams), C.POINTER(FlatVis), C.POINTER(WKern), _P, _I, _P]),
     ("rg_grid_channels", C.c_int, [_P, C.POINTER(GridParams), C.POINTER(FlatVis), C.POINTER(C.c_int64),
                                    _I, C.POINTER(WKern), _P, _I, _P]),
+    ("rg_grid_noalloc", C.c_int, [C.POINTER(GridParams), C.POINTER(FlatVis), C.POINTER(WKern), _P, _I,
+                                  _I, _P]),
     ("rg_grid_to_image", C.c_int, [_P, _P, _I, _I, _P]),
     ("rg_fft2d", C.c_int, [_P, _P, _I, _I, _I, _I, _P]),
     ("rg_fft_shift", C.c_int, [_P, _I, _I, _P]),
@@ -157,6 +159,13 @@ def lib():
 def check(code: int):
     if code != RG_OK:
         raise RGError(code, lib().rg_last_error().decode())
+
+
+def grid_noalloc(p: "GridParams", fv: "FlatVis", wk: "WKern", grid, accumulate=False, stream=None):
+    """rg_grid_noalloc: batched W-projection on the current device through
+    librg's static scratch (no context, no allocation)."""
+    check(lib().rg_grid_noalloc(C.byref(p), C.byref(fv), C.byref(wk), _ptr(grid),
+                                grid_type_of(grid), 1 if accumulate else 0, _stream(stream)))
 
 
 def _ptr(t) -> int:

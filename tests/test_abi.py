@@ -62,3 +62,28 @@ def test_struct_layouts_match_the_header(cname, pyname):
     shift every later field)."""
     ours = [f[0] for f in getattr(rgamd, pyname)._fields_]
     assert _struct_fields(cname) == ours
+
+
+def _dummy_call(n, mode=1, subgrid=0):
+    """rg_grid_noalloc with host buffers standing in for device arrays: only
+    calls that the host-side checks reject are made (nothing is read)."""
+    buf = np.zeros(64, np.complex128)
+    ptr = buf.ctypes.data
+    fv = rgamd.FlatVis(ptr, ptr, ptr, ptr, n, rgamd.RG_C128, 0)
+    wk = rgamd.WKern(ptr, rgamd.RG_C128, 1, 8, 16, -5.0, 1.0)
+    p = rgamd.params(2048, 0.1, mode=mode, subgrid=subgrid, margin=16 if subgrid else 0,
+                     winc=10.0 if subgrid else 0.0)
+    return rgamd.lib().rg_grid_noalloc(C.byref(p), C.byref(fv), C.byref(wk), ptr, rgamd.RG_C128,
+                                       0, None), buf
+
+
+def test_noalloc_rejects_streams_beyond_its_scratch():
+    code, _ = _dummy_call(50_000_000)
+    assert code == rgamd.RG_ERR_PARAM
+    msg = rgamd.lib().rg_last_error().decode()
+    assert "static scratch" in msg and "accumulate" in msg
+
+
+def test_noalloc_is_batched_wprojection_only():
+    assert _dummy_call(10, mode=rgamd.RG_MODE_PLAIN)[0] == rgamd.RG_ERR_UNSUPPORTED
+    assert _dummy_call(10, subgrid=512)[0] == rgamd.RG_ERR_UNSUPPORTED
