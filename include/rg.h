@@ -95,7 +95,7 @@ typedef struct rg_grid_params {
     int32_t compensated;    /* batched FP32: 0 plain FP32 tile sums (default), 1 two-sum
                                compensated tile sums (worth it only into an RG_C128 grid) */
     int32_t floor_batch;    /* batched W-towers: (tower, floor) tasks per batch (0 = as many as
-                               fit 8 GiB of subgrids) */
+                               fit 16 GiB of subgrids) */
 } rg_grid_params;
 
 /* Counters of the last rg_grid call (read back synchronously). */
@@ -128,6 +128,14 @@ int rg_context_destroy(rg_context *ctx);
  * Returns RG_ERR_RANGE (with `out` filled) when that call overran one of its
  * internal capacity bounds, i.e. its result is incomplete. */
 int rg_context_stats(rg_context *ctx, rg_grid_stats *out);
+/* Graph capture: a gridding call made on a capturing stream freezes the
+ * context, since the captured graph holds the addresses of its scratch
+ * memory and channel offsets.  While frozen, a call that would need more
+ * scratch memory or other channel offsets fails with RG_ERR_PARAM instead of
+ * reallocating or rewriting what the graph reads.  rg_context_thaw declares
+ * that no graph captured from this context will be replayed again and lifts
+ * the restriction (no counterpart in the reference, which has no graphs). */
+int rg_context_thaw(rg_context *ctx);
 
 /* Kernel timers: when enabled, librg records HIP events on the launch stream
  * around each launch of the timed kernels below.  rg_context_timer
@@ -156,8 +164,20 @@ int rg_grid(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis,
 
 /* Multi-channel batched gridding: channel c owns visibilities
  * [offsets[c], offsets[c+1]) of `vis` and the grid at
- * grid + c * N * N elements.  All channels are gridded by one work queue
- * (W-projection only).  offsets is a HOST array of n_chan + 1 entries. */
+ * grid + c * N * N elements.  All channels are gridded by one work queue:
+ * W-projection tiles, or W-towers (subgrid_size > 0) tasks of every
+ * (channel, tower, floor) with one RG_C128 master grid per channel.  offsets
+ * is a HOST array of n_chan + 1 entries.
+ *
+ * Batched W-towers calls (one or many channels) size their key space and
+ * buffers by a plan kept in the context: the first call of a shape reads the
+ * floor range and task counts back; later calls of the same shape (grid,
+ * subgrid, kernels, parameters, channel count, no longer stream) issue no
+ * host read-back before their end, where an eager call checks that its data
+ * fitted the plan (and re-plans and redoes itself otherwise).  Such a call
+ * can be captured in a HIP graph after an eager call of its shape; a replay
+ * whose data outgrew the plan is reported by rg_context_stats
+ * (RG_ERR_RANGE). */
 int rg_grid_channels(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis,
                      const int64_t *offsets, int32_t n_chan, const rg_wkern *wk,
                      void *grids, int32_t grid_type, void *stream);

@@ -44,7 +44,9 @@ template <typename T> struct Rec {
     typename Vec2<T>::type vis;
 };
 
-enum VisStatus { VIS_OK = 0, VIS_DROPPED = 1, VIS_BAD_PLANE = 2 };
+// VIS_OVERFLOW: outside a capacity the call was sized for (counted as dropped
+// and as an overrun of the call)
+enum VisStatus { VIS_OK = 0, VIS_DROPPED = 1, VIS_BAD_PLANE = 2, VIS_OVERFLOW = 3 };
 
 // Returns, for the calling lane, old counter value + rank inside its run (when
 // `want` is set).  A run is a maximal stretch of consecutive active lanes with
@@ -102,13 +104,17 @@ __global__ __launch_bounds__(256) void k_bin_count(Keyer kf, int64_t n, uint32_t
     if (i < n)
         krp[i] = make_uint4(ok ? key : KEY_NONE, rank, pos, kidx);
     // per-wave status counts
-    const unsigned long long dropped = __ballot(i < n && st == VIS_DROPPED);
+    const unsigned long long dropped =
+        __ballot(i < n && (st == VIS_DROPPED || st == VIS_OVERFLOW));
     const unsigned long long badp = __ballot(i < n && st == VIS_BAD_PLANE);
+    const unsigned long long over = __ballot(i < n && st == VIS_OVERFLOW);
     if (__lane_id() == 0) {
         if (dropped)
             atomicAdd(&stats[CNT_DROPPED], (unsigned long long)__popcll(dropped));
         if (badp)
             atomicAdd(&stats[CNT_BAD_PLANE], (unsigned long long)__popcll(badp));
+        if (over)
+            atomicAdd(&stats[CNT_OVERFLOW], (unsigned long long)__popcll(over));
     }
 }
 

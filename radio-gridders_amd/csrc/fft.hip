@@ -10,11 +10,14 @@ namespace rg {
 // ---- row pass ------------------------------------------------------------------
 // Transforms rows [blockIdx.x*B, +B) of nrows rows of length n stored
 // contiguously.  row_flags (optional): bit r set = row r non-zero; rows with
-// the bit clear are left untouched (they are zero and stay zero).
+// the bit clear are left untouched (they are zero and stay zero).  dev_batch
+// (optional, device): only the first *dev_batch arrays of n rows are
+// transformed.
 template <typename T2, int LOGN, int B, bool INV>
 __global__ __launch_bounds__(B *((1 << LOGN) / 8)) void k_fft_rows(T2 *data, int64_t nrows,
                                                                    const T2 *__restrict__ twg,
-                                                                   const uint32_t *row_flags)
+                                                                   const uint32_t *row_flags,
+                                                                   const int32_t *dev_batch)
 {
     constexpr int n = 1 << LOGN;
     constexpr int NT = B * (n / 8);
@@ -30,6 +33,8 @@ __global__ __launch_bounds__(B *((1 << LOGN) / 8)) void k_fft_rows(T2 *data, int
         tw = tws;
     }
     const int64_t row0 = (int64_t)blockIdx.x * B;
+    if (dev_batch && row0 >= (int64_t)*dev_batch * n)
+        return; // the block's rows (B divides n) lie in an inactive array
     // block-uniform skip when every row of the block is empty
     bool any = true;
     if (row_flags) {
@@ -66,11 +71,15 @@ __global__ __launch_bounds__(B *((1 << LOGN) / 8)) void k_fft_rows(T2 *data, int
 }
 
 // ---- column pass (n <= 512) ----------------------------------------------------
-// Block (cb, a): columns [cb*CB, +CB) of array a (n x n, contiguous).
+// Block (cb, a): columns [cb*CB, +CB) of array a (n x n, contiguous); arrays
+// a >= *dev_batch (when given) are left alone.
 template <typename T2, int LOGN, int CB, bool INV>
 __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_fft_cols(T2 *data,
-                                                                    const T2 *__restrict__ twg)
+                                                                    const T2 *__restrict__ twg,
+                                                                    const int32_t *dev_batch)
 {
+    if (dev_batch && (int)blockIdx.y >= *dev_batch)
+        return;
     constexpr int n = 1 << LOGN;
     constexpr int NT = CB * (n / 8);
     __shared__ T2 lds[n * CB];
@@ -93,8 +102,10 @@ __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_fft_cols(T2 *data,
 // ---- transpose -------------------------------------------------------------------
 template <typename T2>
 __global__ __launch_bounds__(256) void k_transpose(const T2 *__restrict__ in, T2 *__restrict__ out,
-                                                   int n)
+                                                   int n, const int32_t *dev_batch)
 {
+    if (dev_batch && (int)blockIdx.z >= *dev_batch)
+        return;
     __shared__ T2 tile[32][33];
     const size_t off = (size_t)blockIdx.z * n * n;
     const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
@@ -180,76 +191,78 @@ int get_twiddles(rg_context *ctx, int n, int type, int sign, const void **out)
 // ---- dispatch ----------------------------------------------------------------------
 
 template <typename T2, int LOGN, bool INV>
-static int rows_t(T2 *data, int64_t nrows, const T2 *tw, const uint32_t *flags, hipStream_t s)
+static int rows_t(T2 *data, int64_t nrows, const T2 *tw, const uint32_t *flags,
+                  const int32_t *dev_batch, hipStream_t s)
 {
     constexpr int n = 1 << LOGN;
     constexpr int TPL = n / 8;
     constexpr int B = TPL >= 256 ? 1 : 256 / TPL;
     const int64_t blocks = (nrows + B - 1) / B;
     hipLaunchKernelGGL((k_fft_rows<T2, LOGN, B, INV>), dim3((unsigned)blocks), dim3(B * TPL), 0, s,
-                       data, nrows, tw, flags);
+                       data, nrows, tw, flags, dev_batch);
     RG_HIP(hipGetLastError());
     return RG_OK;
 }
 
 template <typename T2, int LOGN, bool INV>
-static int cols_t(T2 *data, int batch, const T2 *tw, hipStream_t s)
+static int cols_t(T2 *data, int batch, const T2 *tw, const int32_t *dev_batch, hipStream_t s)
 {
     constexpr int n = 1 << LOGN;
     constexpr int CB = sizeof(T2) == 8 ? 16 : 8;
     constexpr int cb = CB > n ? n : CB;
     hipLaunchKernelGGL((k_fft_cols<T2, LOGN, cb, INV>), dim3(n / cb, batch), dim3(cb * (n / 8)), 0,
-                       s, data, tw);
+                       s, data, tw, dev_batch);
     RG_HIP(hipGetLastError());
     return RG_OK;
 }
 
 template <typename T2, bool INV>
 int fft_rows_dispatch(T2 *data, int n, int64_t nrows, const T2 *tw, const uint32_t *flags,
-                      hipStream_t s)
+                      const int32_t *dev_batch, hipStream_t s)
 {
     switch (n) {
-    case 8: return rows_t<T2, 3, INV>(data, nrows, tw, flags, s);
-    case 16: return rows_t<T2, 4, INV>(data, nrows, tw, flags, s);
-    case 32: return rows_t<T2, 5, INV>(data, nrows, tw, flags, s);
-    case 64: return rows_t<T2, 6, INV>(data, nrows, tw, flags, s);
-    case 128: return rows_t<T2, 7, INV>(data, nrows, tw, flags, s);
-    case 256: return rows_t<T2, 8, INV>(data, nrows, tw, flags, s);
-    case 512: return rows_t<T2, 9, INV>(data, nrows, tw, flags, s);
-    case 1024: return rows_t<T2, 10, INV>(data, nrows, tw, flags, s);
-    case 2048: return rows_t<T2, 11, INV>(data, nrows, tw, flags, s);
-    case 4096: return rows_t<T2, 12, INV>(data, nrows, tw, flags, s);
+    case 8: return rows_t<T2, 3, INV>(data, nrows, tw, flags, dev_batch, s);
+    case 16: return rows_t<T2, 4, INV>(data, nrows, tw, flags, dev_batch, s);
+    case 32: return rows_t<T2, 5, INV>(data, nrows, tw, flags, dev_batch, s);
+    case 64: return rows_t<T2, 6, INV>(data, nrows, tw, flags, dev_batch, s);
+    case 128: return rows_t<T2, 7, INV>(data, nrows, tw, flags, dev_batch, s);
+    case 256: return rows_t<T2, 8, INV>(data, nrows, tw, flags, dev_batch, s);
+    case 512: return rows_t<T2, 9, INV>(data, nrows, tw, flags, dev_batch, s);
+    case 1024: return rows_t<T2, 10, INV>(data, nrows, tw, flags, dev_batch, s);
+    case 2048: return rows_t<T2, 11, INV>(data, nrows, tw, flags, dev_batch, s);
+    case 4096: return rows_t<T2, 12, INV>(data, nrows, tw, flags, dev_batch, s);
     default: return fail(RG_ERR_PARAM, "FFT size %d unsupported (power of two 8..4096)", n);
     }
 }
 
 template <typename T2, bool INV>
-static int fft_cols_dispatch(T2 *data, int n, int batch, const T2 *tw, hipStream_t s)
+static int fft_cols_dispatch(T2 *data, int n, int batch, const T2 *tw, const int32_t *dev_batch,
+                             hipStream_t s)
 {
     switch (n) {
-    case 8: return cols_t<T2, 3, INV>(data, batch, tw, s);
-    case 16: return cols_t<T2, 4, INV>(data, batch, tw, s);
-    case 32: return cols_t<T2, 5, INV>(data, batch, tw, s);
-    case 64: return cols_t<T2, 6, INV>(data, batch, tw, s);
-    case 128: return cols_t<T2, 7, INV>(data, batch, tw, s);
-    case 256: return cols_t<T2, 8, INV>(data, batch, tw, s);
-    case 512: return cols_t<T2, 9, INV>(data, batch, tw, s);
+    case 8: return cols_t<T2, 3, INV>(data, batch, tw, dev_batch, s);
+    case 16: return cols_t<T2, 4, INV>(data, batch, tw, dev_batch, s);
+    case 32: return cols_t<T2, 5, INV>(data, batch, tw, dev_batch, s);
+    case 64: return cols_t<T2, 6, INV>(data, batch, tw, dev_batch, s);
+    case 128: return cols_t<T2, 7, INV>(data, batch, tw, dev_batch, s);
+    case 256: return cols_t<T2, 8, INV>(data, batch, tw, dev_batch, s);
+    case 512: return cols_t<T2, 9, INV>(data, batch, tw, dev_batch, s);
     default: return fail(RG_ERR_PARAM, "column pass size %d unsupported", n);
     }
 }
 
 template int fft_rows_dispatch<float2, true>(float2 *, int, int64_t, const float2 *,
-                                             const uint32_t *, hipStream_t);
+                                             const uint32_t *, const int32_t *, hipStream_t);
 template int fft_rows_dispatch<double2, true>(double2 *, int, int64_t, const double2 *,
-                                              const uint32_t *, hipStream_t);
+                                             const uint32_t *, const int32_t *, hipStream_t);
 template int fft_rows_dispatch<float2, false>(float2 *, int, int64_t, const float2 *,
-                                              const uint32_t *, hipStream_t);
+                                             const uint32_t *, const int32_t *, hipStream_t);
 template int fft_rows_dispatch<double2, false>(double2 *, int, int64_t, const double2 *,
-                                               const uint32_t *, hipStream_t);
+                                             const uint32_t *, const int32_t *, hipStream_t);
 
 template <typename T2>
 static int fft2d_t(rg_context *ctx, T2 *data, int type, int n, int batch, int sign,
-                   const uint32_t *row_flags, hipStream_t s)
+                   const uint32_t *row_flags, const int32_t *dev_batch, hipStream_t s)
 {
     const void *twv = nullptr;
     RG_TRY(get_twiddles(ctx, n, type, sign, &twv));
@@ -258,11 +271,11 @@ static int fft2d_t(rg_context *ctx, T2 *data, int type, int n, int batch, int si
     const int64_t nrows = (int64_t)n * batch;
     if (n <= 512) {
         if (inv) {
-            RG_TRY((fft_rows_dispatch<T2, true>(data, n, nrows, tw, row_flags, s)));
-            return fft_cols_dispatch<T2, true>(data, n, batch, tw, s);
+            RG_TRY((fft_rows_dispatch<T2, true>(data, n, nrows, tw, row_flags, dev_batch, s)));
+            return fft_cols_dispatch<T2, true>(data, n, batch, tw, dev_batch, s);
         }
-        RG_TRY((fft_rows_dispatch<T2, false>(data, n, nrows, tw, row_flags, s)));
-        return fft_cols_dispatch<T2, false>(data, n, batch, tw, s);
+        RG_TRY((fft_rows_dispatch<T2, false>(data, n, nrows, tw, row_flags, dev_batch, s)));
+        return fft_cols_dispatch<T2, false>(data, n, batch, tw, dev_batch, s);
     }
     // rows, transpose, rows, transpose back
     RG_TRY(ctx->scratch[11].ensure(sizeof(T2) * (size_t)n * n * batch));
@@ -271,10 +284,11 @@ static int fft2d_t(rg_context *ctx, T2 *data, int type, int n, int batch, int si
     for (int pass = 0; pass < 2; ++pass) {
         const uint32_t *flags = pass == 0 ? row_flags : nullptr;
         if (inv)
-            RG_TRY((fft_rows_dispatch<T2, true>(data, n, nrows, tw, flags, s)));
+            RG_TRY((fft_rows_dispatch<T2, true>(data, n, nrows, tw, flags, dev_batch, s)));
         else
-            RG_TRY((fft_rows_dispatch<T2, false>(data, n, nrows, tw, flags, s)));
-        hipLaunchKernelGGL(k_transpose<T2>, tg, dim3(256), 0, s, (const T2 *)data, tmp, n);
+            RG_TRY((fft_rows_dispatch<T2, false>(data, n, nrows, tw, flags, dev_batch, s)));
+        hipLaunchKernelGGL(k_transpose<T2>, tg, dim3(256), 0, s, (const T2 *)data, tmp, n,
+                           dev_batch);
         RG_HIP(hipGetLastError());
         std::swap(data, tmp);
     }
@@ -292,25 +306,27 @@ int fft_rows(rg_context *ctx, void *data, int type, int n, int64_t nrows, int si
     RG_TRY(get_twiddles(ctx, n, type, sign, &tw));
     if (type == RG_C128)
         return sign > 0 ? fft_rows_dispatch<double2, true>((double2 *)data, n, nrows,
-                                                           (const double2 *)tw, nullptr, s)
+                                                           (const double2 *)tw, nullptr, nullptr, s)
                         : fft_rows_dispatch<double2, false>((double2 *)data, n, nrows,
-                                                            (const double2 *)tw, nullptr, s);
+                                                            (const double2 *)tw, nullptr, nullptr,
+                                                            s);
     return sign > 0 ? fft_rows_dispatch<float2, true>((float2 *)data, n, nrows, (const float2 *)tw,
-                                                      nullptr, s)
+                                                      nullptr, nullptr, s)
                     : fft_rows_dispatch<float2, false>((float2 *)data, n, nrows,
-                                                       (const float2 *)tw, nullptr, s);
+                                                       (const float2 *)tw, nullptr, nullptr, s);
 }
 
 int fft2d(rg_context *ctx, void *data, int type, int n, int batch, int sign, hipStream_t s,
-          const uint32_t *row_flags)
+          const uint32_t *row_flags, const int32_t *dev_batch)
 {
     if (n < 8 || n > 4096 || (n & (n - 1)))
         return fail(RG_ERR_PARAM, "FFT size %d unsupported (power of two 8..4096)", n);
     if (batch < 1 || (sign != 1 && sign != -1))
         return fail(RG_ERR_PARAM, "invalid FFT batch/sign");
     if (type == RG_C128)
-        return fft2d_t<double2>(ctx, (double2 *)data, type, n, batch, sign, row_flags, s);
-    return fft2d_t<float2>(ctx, (float2 *)data, type, n, batch, sign, row_flags, s);
+        return fft2d_t<double2>(ctx, (double2 *)data, type, n, batch, sign, row_flags, dev_batch,
+                                s);
+    return fft2d_t<float2>(ctx, (float2 *)data, type, n, batch, sign, row_flags, dev_batch, s);
 }
 
 } // namespace rg

@@ -2,6 +2,7 @@
 // prolate spheroidal taper and the W-kernel table generator.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <cstdarg>
@@ -41,6 +42,11 @@ int DevBuf::ensure(size_t n)
 {
     if (n <= bytes && p)
         return RG_OK;
+    if (frozen && *frozen)
+        return fail(RG_ERR_PARAM,
+                    "this call needs %zu bytes of a scratch buffer that a captured graph uses "
+                    "(%zu bytes): the context cannot reallocate it until rg_context_thaw",
+                    n, bytes);
     release();
     size_t want = n < 256 ? 256 : n;
     hipError_t e = hipMalloc(&p, want);
@@ -196,6 +202,35 @@ using namespace rg;
 
 // ---- C ABI -----------------------------------------------------------------------------
 
+namespace rg {
+
+int channel_offsets(rg_context *ctx, const int64_t *offsets, int n_chan, hipStream_t s,
+                    const int64_t **d_off)
+{
+    *d_off = nullptr;
+    if (n_chan <= 1)
+        return RG_OK;
+    // resident copy of the offsets: a call with the offsets of the previous
+    // multi-channel call issues no copy (and so can be graph-captured)
+    const bool same = ctx->chan_off.size() == (size_t)n_chan + 1 &&
+                      std::equal(ctx->chan_off.begin(), ctx->chan_off.end(), offsets);
+    if (!same) {
+        if (ctx->frozen)
+            return fail(RG_ERR_PARAM,
+                        "a captured graph reads this context's channel offsets: calls with other "
+                        "offsets need rg_context_thaw (or another context)");
+        RG_TRY(ctx->scratch[13].ensure(sizeof(int64_t) * (n_chan + 1)));
+        RG_HIP(hipMemcpyAsync(ctx->scratch[13].p, offsets, sizeof(int64_t) * (n_chan + 1),
+                              hipMemcpyHostToDevice, s));
+        RG_HIP(hipStreamSynchronize(s)); // the host array may go away after the call
+        ctx->chan_off.assign(offsets, offsets + n_chan + 1);
+    }
+    *d_off = ctx->scratch[13].as<int64_t>();
+    return RG_OK;
+}
+
+} // namespace rg
+
 extern "C" {
 
 const char *rg_last_error(void) { return g_last_error.c_str(); }
@@ -216,6 +251,8 @@ int rg_context_create(rg_context **ctx, int device)
     RG_HIP(hipSetDevice(device));
     rg_context *c = new rg_context();
     c->device = device;
+    for (auto &buf : c->scratch)
+        buf.frozen = &c->frozen;
     int st = ensure_counters(c);
     if (st != RG_OK) {
         delete c;
@@ -309,6 +346,14 @@ int rg_context_stats(rg_context *ctx, rg_grid_stats *out)
     return RG_OK;
 }
 
+int rg_context_thaw(rg_context *ctx)
+{
+    if (!ctx)
+        return fail(RG_ERR_PARAM, "ctx is NULL");
+    ctx->frozen = false;
+    return RG_OK;
+}
+
 static int grid_common(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis,
                        const int64_t *offsets, int32_t n_chan, const rg_wkern *wk, void *grid,
                        int32_t grid_type, void *stream)
@@ -324,24 +369,29 @@ static int grid_common(rg_context *ctx, const rg_grid_params *p, const rg_flat_v
         return fail(RG_ERR_PARAM, "unknown grid type %d", grid_type);
     hipStream_t s = (hipStream_t)stream;
     RG_HIP(hipSetDevice(ctx->device));
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    RG_HIP(hipStreamIsCapturing(s, &cap));
+    const bool capturing = cap == hipStreamCaptureStatusActive;
+    if (capturing)
+        ctx->frozen = true; // the graph will hold this context's scratch addresses
     RG_TRY(zero_fill(ctx->counters.p, sizeof(unsigned long long) * CNT__N, s));
     ctx->last_stream = s;
     ctx->last_nvis = vis->n;
+    if (p->mode == RG_MODE_PLAIN && n_chan != 1)
+        return fail(RG_ERR_UNSUPPORTED, "plain mode grids one channel per call");
+    const int64_t *d_off = nullptr;
+    RG_TRY(channel_offsets(ctx, offsets, n_chan, s, &d_off));
     if (p->subgrid_size == 0) {
         ctx->last_kind = 1;
         if (p->mode == RG_MODE_PLAIN) {
-            if (n_chan != 1)
-                return fail(RG_ERR_UNSUPPORTED, "plain mode grids one channel per call");
             if (grid_type != RG_C128)
                 return fail(RG_ERR_PARAM, "plain mode writes an RG_C128 grid");
             return wproj_plain(ctx, p, vis, wk, grid, s);
         }
-        return wproj_batched(ctx, p, vis, offsets, n_chan, wk, grid, grid_type, s);
+        return wproj_batched(ctx, p, vis, d_off, n_chan, wk, grid, grid_type, s);
     }
     ctx->last_kind = 2;
-    if (n_chan != 1)
-        return fail(RG_ERR_UNSUPPORTED, "multi-channel batching is W-projection only");
-    return wtowers_run(ctx, p, vis, wk, grid, grid_type, s);
+    return wtowers_run(ctx, p, vis, d_off, n_chan, wk, grid, grid_type, s, capturing);
 }
 
 int rg_grid(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis, const rg_wkern *wk,

@@ -85,6 +85,9 @@ __host__ __device__ inline int floor_index(double w, double winc)
 struct DevBuf {
     void *p = nullptr;
     size_t bytes = 0;
+    // set for a context's scratch slots: while *frozen (a captured graph holds
+    // the slot's address) ensure() fails instead of reallocating
+    const bool *frozen = nullptr;
     int ensure(size_t n);
     void release();
     template <typename T> T *as() const { return reinterpret_cast<T *>(p); }
@@ -99,6 +102,19 @@ struct PinnedBuf {
 
 // Twiddle table key: (n, type, sign)
 using TwKey = std::tuple<int, int, int>;
+
+// Capacities of batched W-towers calls of one shape (csrc/wtowers.hip).
+struct WtPlan {
+    bool valid = false;
+    int N = 0, n = 0, M = 0, S = 0, R = 0, P = 0, H = 0, type = 0, nchan = 0;
+    uint32_t K = 0;
+    int floor_batch = 0;
+    double theta = 0, winc = 0, w_min = 0, w_step = 0;
+    int64_t nvis = 0;   // longest stream the plan was made for
+    int nwp_cap = 0;    // floors per tower slot
+    int ntasks_cap = 0, nchunks_cap = 0, ntowers_cap = 0;
+    int nitems_cap = 0; // band work items
+};
 
 } // namespace rg
 
@@ -121,10 +137,17 @@ struct rg_context {
     //            transposes
     //   13       channel offsets, 14 DFT, 16 uniform-weighting counts
     //   17       the W-stack, kept between calls
-    rg::DevBuf scratch[18];
+    //   8        W-towers floor range; 18 W-towers task tables
+    rg::DevBuf scratch[19];
     // channel offsets resident in scratch[13] (uploaded only when they change,
     // so a repeated multi-channel call issues no copy and can be captured)
     std::vector<int64_t> chan_off;
+    // set by a call on a capturing stream: the graph holds scratch addresses,
+    // so no slot may be reallocated and the offsets not rewritten until
+    // rg_context_thaw
+    bool frozen = false;
+    // batched W-towers capacities of the last shape
+    rg::WtPlan wt_plan;
     std::map<rg::TwKey, rg::DevBuf> twiddles;
     // kernel timers (RG_TIMER_*): recorded (start, stop) event pairs
     bool timing = false;
@@ -168,20 +191,28 @@ int get_twiddles(rg_context *ctx, int n, int type, int sign, const void **out);
 int wproj_plain(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis,
                 const rg_wkern *wk, void *grid, hipStream_t s);
 int wproj_batched(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis,
-                  const int64_t *offsets, int n_chan, const rg_wkern *wk, void *grid,
+                  const int64_t *d_off, int n_chan, const rg_wkern *wk, void *grid,
                   int grid_type, hipStream_t s);
 // batched W-projection into the library's static scratch (no allocation)
 int wproj_noalloc(const rg_grid_params *p, const rg_flat_vis *vis, const rg_wkern *wk, void *grid,
                   int grid_type, int accumulate, hipStream_t s);
 size_t noalloc_scratch_bytes();
+// d_off: device channel offsets (nchan + 1, NULL for one channel); capturing:
+// the stream is being captured (no read-backs, no allocation)
 int wtowers_run(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis,
-                const rg_wkern *wk, void *grid, int grid_type, hipStream_t s);
+                const int64_t *d_off, int nchan, const rg_wkern *wk, void *grid, int grid_type,
+                hipStream_t s, bool capturing);
+// device copy of host channel offsets, kept in the context (scratch[13])
+int channel_offsets(rg_context *ctx, const int64_t *offsets, int n_chan, hipStream_t s,
+                    const int64_t **d_off);
 // in-place FFT of nrows contiguous rows of length n (power of two 8..4096)
 int fft_rows(rg_context *ctx, void *data, int type, int n, int64_t nrows, int sign, hipStream_t s);
 // row_flags (optional): bit r set for every row r (of the n * batch rows) that
-// may be nonzero on input; the first row pass skips the others, which must be 0
+// may be nonzero on input; the first row pass skips the others, which must be 0.
+// dev_batch (optional, device int): only the first *dev_batch <= batch arrays
+// are transformed (the launch is sized for batch).
 int fft2d(rg_context *ctx, void *data, int type, int n, int batch, int sign, hipStream_t s,
-          const uint32_t *row_flags = nullptr);
+          const uint32_t *row_flags = nullptr, const int32_t *dev_batch = nullptr);
 int fft_shift(void *data, int type, int n, int batch, hipStream_t s);
 int zero_fill(void *p, size_t bytes, hipStream_t s);
 int weight_vis(rg_context *ctx, const double *u, const double *v, int64_t n, void *vis,

@@ -298,28 +298,13 @@ static WprojKeyer wproj_keyer(const rg_grid_params *p, const rg_flat_vis *vis, c
 
 template <typename T>
 static int wproj_batched_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis,
-                           const int64_t *offsets, int n_chan, const rg_wkern *wk, void *grid,
+                           const int64_t *d_off, int n_chan, const rg_wkern *wk, void *grid,
                            int grid_type, hipStream_t s)
 {
     const int N = p->grid_size;
     WprojShape sh;
     RG_TRY(wproj_shape(p, wk, sh));
 
-    const int64_t *d_off = nullptr;
-    if (n_chan > 1) {
-        // resident copy of the offsets: a call with the offsets of the previous
-        // multi-channel call issues no copy (and so can be graph-captured)
-        const bool same = ctx->chan_off.size() == (size_t)n_chan + 1 &&
-                          std::equal(ctx->chan_off.begin(), ctx->chan_off.end(), offsets);
-        if (!same) {
-            RG_TRY(ctx->scratch[13].ensure(sizeof(int64_t) * (n_chan + 1)));
-            RG_HIP(hipMemcpyAsync(ctx->scratch[13].p, offsets, sizeof(int64_t) * (n_chan + 1),
-                                  hipMemcpyHostToDevice, s));
-            RG_HIP(hipStreamSynchronize(s)); // the host array may go away after the call
-            ctx->chan_off.assign(offsets, offsets + n_chan + 1);
-        }
-        d_off = ctx->scratch[13].as<int64_t>();
-    }
     const size_t gbytes = (size_t)(grid_type == RG_C128 ? 16 : 8) * N * N * n_chan;
     RG_TRY(zero_fill(grid, gbytes, s));
 
@@ -332,12 +317,12 @@ static int wproj_batched_t(rg_context *ctx, const rg_grid_params *p, const rg_fl
 }
 
 int wproj_batched(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis,
-                  const int64_t *offsets, int n_chan, const rg_wkern *wk, void *grid,
+                  const int64_t *d_off, int n_chan, const rg_wkern *wk, void *grid,
                   int grid_type, hipStream_t s)
 {
     if (wk->type == RG_C64)
-        return wproj_batched_t<float>(ctx, p, vis, offsets, n_chan, wk, grid, grid_type, s);
-    return wproj_batched_t<double>(ctx, p, vis, offsets, n_chan, wk, grid, grid_type, s);
+        return wproj_batched_t<float>(ctx, p, vis, d_off, n_chan, wk, grid, grid_type, s);
+    return wproj_batched_t<double>(ctx, p, vis, d_off, n_chan, wk, grid, grid_type, s);
 }
 
 // ---- allocation-free batched W-projection ------------------------------------------------

@@ -30,6 +30,8 @@ namespace rg {
 
 // ---- binning -------------------------------------------------------------------------
 
+// Tower ids: channel c, tower (cx, cy) -> (c * cc + cy) * cc + cx.  Task slots:
+// tower id * nwp + (wp - wp_min), nwp the floor capacity of the call.
 struct TowerKeyer {
     const double *u, *v, *w;
     const void *vis;
@@ -38,13 +40,29 @@ struct TowerKeyer {
     double theta, w_min, w_step;
     int n, chunk, cc;
     double winc;
-    int wp_min, nwp;
+    const int *range; // device: wp_min of the call (range[0])
+    int nwp;          // floors per tower slot (capacity); farther floors overflow
     // band_h == 0: key = task.  Otherwise keys are task * nk + b for the
     // nk = n / band_h bands of band_h subgrid rows, and a visibility's key is
     // the first band its footprint rows [gy, gy + S) reach inside [0, n); the
     // footprint reaches at most mk consecutive bands.
     // pos is always the raw subgrid corner + S: (gx + S) | (gy + S) << 16.
     int band_h, nk, mk;
+    const int64_t *chan_off; // channel c owns [chan_off[c], chan_off[c + 1])
+    int nchan;
+
+    __device__ int chan_of(int64_t i) const
+    {
+        int lo = 0, hi = nchan - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (chan_off[mid] <= i)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        return lo;
+    }
 
     __device__ int classify(int64_t i, uint32_t &key, uint32_t &pos, uint32_t &kidx) const
     {
@@ -54,8 +72,11 @@ struct TowerKeyer {
         if (cx < 0 || cx >= cc || cy < 0 || cy >= cc)
             return VIS_DROPPED;
         const int wp = floor_index(wi, winc);
-        if (wp < wp_min || wp >= wp_min + nwp)
+        const int wp_min = range[0];
+        if (wp < wp_min)
             return VIS_DROPPED;
+        if (wp >= wp_min + nwp)
+            return VIS_OVERFLOW;
         const int x_min = chunk * cx - N / 2;
         const int y_min = chunk * cy - N / 2;
         const double u_mid = (double)(x_min + chunk / 2) / theta;
@@ -68,7 +89,8 @@ struct TowerKeyer {
         const int p = w_plane(wi - w_mid, w_min, w_step);
         if (p < 0 || p >= P)
             return VIS_BAD_PLANE;
-        const uint32_t task = (uint32_t)((cy * cc + cx) * nwp + (wp - wp_min));
+        const int c = nchan > 1 ? chan_of(i) : 0;
+        const uint32_t task = (uint32_t)(((c * cc + cy) * cc + cx) * nwp + (wp - wp_min));
         key = band_h > 0 ? task * (uint32_t)nk + (uint32_t)(max(gy, 0) / band_h) : task;
         pos = (uint32_t)(gx + S) | ((uint32_t)(gy + S) << 16);
         kidx = (uint32_t)((p * R + yf) * R + xf);
@@ -133,7 +155,7 @@ __global__ void k_range_init(int *range)
     range[1] = INT_MIN;
 }
 
-// per task slot: visibility count and first work item
+// per task slot: visibility count and first work item (plain mode)
 __global__ __launch_bounds__(256) void k_task_info(const uint32_t *vis_start,
                                                    const uint32_t *item_start, int64_t nslots,
                                                    int ntiles, uint32_t *nvis, uint32_t *item0)
@@ -143,6 +165,185 @@ __global__ __launch_bounds__(256) void k_task_info(const uint32_t *vis_start,
         return;
     nvis[t] = vis_start[(t + 1) * ntiles] - vis_start[t * ntiles];
     item0[t] = item_start[t * ntiles];
+}
+
+// ---- task tables (batched mode, built on the device) ---------------------------------------
+// Tasks are the non-empty slots in slot order (tower id, then floor), so the
+// tasks of a tower are consecutive.  Each tower's run of tasks is cut into
+// chunks of at most TASKS_PER_CHUNK (one column-pass partial image each);
+// active towers are ranked in tower-id order (the subimage index).
+constexpr int TASKS_PER_CHUNK = 32;
+
+struct TaskTables {
+    int32_t *tower_ntasks;   // [ntw]
+    int32_t *tower_task_off; // [ntw]
+    int32_t *tower_chunk_off; // [ntw]
+    int32_t *rank_of_tower;  // [ntw], -1 inactive
+    int32_t *tower_of_rank;  // [ntw]
+    int32_t *rank_chunk0;    // [ntw] first chunk of rank r
+    int32_t *rank_nchunk;    // [ntw]
+    int32_t *task_slot;      // [ntasks_cap]
+    int32_t *task_wp;        // [ntasks_cap]
+    int32_t *task_rank;      // [ntasks_cap] tower rank of the task
+    int32_t *chunk_first;    // [nchunks_cap]
+    int32_t *chunk_count;    // [nchunks_cap]
+    int32_t *rank_of_slot;   // [nslots] task index of a non-empty slot
+    // totals[0] tasks, [1] towers, [2] chunks (each clipped to its capacity)
+    int32_t *totals;
+    int ntw, nwp, nk; // tower ids, floors per tower, keys per task
+    int ntasks_cap, nchunks_cap, ntowers_cap;
+};
+
+__device__ inline bool slot_nonempty(const uint32_t *key_start, int64_t s, int nk)
+{
+    return key_start[(s + 1) * nk] > key_start[s * nk];
+}
+
+// block t: number of non-empty slots of tower t
+__global__ __launch_bounds__(256) void k_tower_count(const uint32_t *__restrict__ key_start,
+                                                     TaskTables tt)
+{
+    __shared__ int wsum[4];
+    const int t = blockIdx.x;
+    int c = 0;
+    for (int f = threadIdx.x; f < tt.nwp; f += 256)
+        c += slot_nonempty(key_start, (int64_t)t * tt.nwp + f, tt.nk) ? 1 : 0;
+    for (int o = 32; o > 0; o >>= 1)
+        c += __shfl_xor(c, o);
+    if (__lane_id() == 0)
+        wsum[threadIdx.x / 64] = c;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        tt.tower_ntasks[t] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+// one block: offsets of every tower's tasks and chunks, tower ranks, totals
+__global__ __launch_bounds__(1024) void k_tower_scan(TaskTables tt, unsigned long long *stats)
+{
+    __shared__ int s_task[1024], s_chunk[1024], s_act[1024];
+    int run_task = 0, run_chunk = 0, run_act = 0;
+    for (int base = 0; base < tt.ntw; base += 1024) {
+        const int t = base + (int)threadIdx.x;
+        const int nt = t < tt.ntw ? tt.tower_ntasks[t] : 0;
+        const int nc = (nt + TASKS_PER_CHUNK - 1) / TASKS_PER_CHUNK;
+        const int act = nt > 0 ? 1 : 0;
+        s_task[threadIdx.x] = nt;
+        s_chunk[threadIdx.x] = nc;
+        s_act[threadIdx.x] = act;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            int a = 0, c = 0, r = 0;
+            if ((int)threadIdx.x >= o) {
+                a = s_task[threadIdx.x - o];
+                c = s_chunk[threadIdx.x - o];
+                r = s_act[threadIdx.x - o];
+            }
+            __syncthreads();
+            s_task[threadIdx.x] += a;
+            s_chunk[threadIdx.x] += c;
+            s_act[threadIdx.x] += r;
+            __syncthreads();
+        }
+        if (t < tt.ntw) {
+            const int toff = run_task + s_task[threadIdx.x] - nt;
+            const int coff = run_chunk + s_chunk[threadIdx.x] - nc;
+            const int rk = run_act + s_act[threadIdx.x] - act;
+            // a tower ranked beyond the subimage capacity is left out (the
+            // call reports an overrun)
+            const bool keep = act && rk < tt.ntowers_cap;
+            tt.tower_task_off[t] = toff;
+            tt.tower_chunk_off[t] = coff;
+            tt.rank_of_tower[t] = keep ? rk : -1;
+            if (keep) {
+                tt.tower_of_rank[rk] = t;
+                tt.rank_chunk0[rk] = coff;
+                tt.rank_nchunk[rk] = nc;
+            }
+        }
+        run_task += s_task[1023];
+        run_chunk += s_chunk[1023];
+        run_act += s_act[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        stats[CNT_TASKS] = (unsigned long long)run_task;
+        stats[CNT_TOWERS] = (unsigned long long)run_act;
+        stats[CNT_CHUNKS] = (unsigned long long)run_chunk;
+        const bool over = run_task > tt.ntasks_cap || run_chunk > tt.nchunks_cap ||
+                          run_act > tt.ntowers_cap;
+        if (over)
+            atomicAdd(&stats[CNT_OVERFLOW], 1ull);
+        tt.totals[0] = min(run_task, tt.ntasks_cap);
+        tt.totals[1] = min(run_act, tt.ntowers_cap);
+        tt.totals[2] = min(run_chunk, tt.nchunks_cap);
+    }
+}
+
+// block t: the task entries and chunks of tower t (block scan over its slots)
+__global__ __launch_bounds__(256) void k_tower_tasks(const uint32_t *__restrict__ key_start,
+                                                     TaskTables tt,
+                                                     const int *__restrict__ range)
+{
+    __shared__ int wsum[4];
+    const int t = blockIdx.x;
+    const int nt = tt.tower_ntasks[t];
+    if (nt == 0)
+        return;
+    const int toff = tt.tower_task_off[t], rk = tt.rank_of_tower[t];
+    const int wp_min = range[0];
+    int run = 0;
+    for (int base = 0; base < tt.nwp; base += 256) {
+        const int f = base + (int)threadIdx.x;
+        const int64_t s = (int64_t)t * tt.nwp + f;
+        const bool ne = f < tt.nwp && slot_nonempty(key_start, s, tt.nk);
+        const unsigned long long m = __ballot(ne);
+        const int lane = __lane_id(), wv = threadIdx.x / 64;
+        const int in_wave = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0)
+            wsum[wv] = __popcll(m);
+        __syncthreads();
+        int before = run;
+        for (int q = 0; q < wv; ++q)
+            before += wsum[q];
+        if (ne) {
+            const int k = toff + before + in_wave;
+            if (k < tt.ntasks_cap) {
+                tt.task_slot[k] = (int32_t)s;
+                tt.task_wp[k] = wp_min + f;
+                tt.task_rank[k] = rk;
+                tt.rank_of_slot[s] = k;
+            }
+        }
+        run += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        __syncthreads();
+    }
+    const int coff = tt.tower_chunk_off[t];
+    const int nc = (nt + TASKS_PER_CHUNK - 1) / TASKS_PER_CHUNK;
+    for (int c = threadIdx.x; c < nc; c += 256) {
+        if (coff + c < tt.nchunks_cap) {
+            const int first = toff + c * TASKS_PER_CHUNK;
+            tt.chunk_first[coff + c] = first;
+            tt.chunk_count[coff + c] =
+                max(0, min(min(TASKS_PER_CHUNK, nt - c * TASKS_PER_CHUNK), tt.ntasks_cap - first));
+        }
+    }
+}
+
+// Tasks [t0, t1) of batch b (Q tasks per batch); false when the batch is empty.
+struct BatchRef {
+    const int32_t *task_slot;
+    const int32_t *totals;
+    int Q;
+};
+
+__device__ inline bool batch_tasks(const BatchRef &br, int b, int &t0, int &t1)
+{
+    const int nt = br.totals[0];
+    t0 = b * br.Q;
+    if (t0 >= nt)
+        return false;
+    t1 = min(nt, t0 + br.Q);
+    return true;
 }
 
 // ---- screen ------------------------------------------------------------------------------
@@ -401,8 +602,8 @@ __global__ __launch_bounds__(256) void k_window_len(const uint32_t *__restrict__
 template <typename T, int S, int H, int LOGN, int GRP>
 __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
     const BandRec<T> *__restrict__ recs, const Item *__restrict__ items,
-    const uint32_t *__restrict__ item_start, uint32_t key_begin, uint32_t key_end,
-    const int32_t *__restrict__ rank_of_slot, int t0,
+    const uint32_t *__restrict__ item_start, BatchRef bref, int batch,
+    const int32_t *__restrict__ rank_of_slot,
     const typename Vec2<T>::type *__restrict__ kern, uint32_t kern_bytes,
     const typename Vec2<T>::type *__restrict__ twg, typename Vec2<T>::type *G,
     typename Vec2<T>::type *spill, uint32_t spill_cap, uint32_t *spill_ctr, uint32_t *item_spill,
@@ -428,222 +629,238 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
     __shared__ T2 tw[n];
     __shared__ BandRec<T> s_rec[RS];
     __shared__ uint32_t s_slot;
-    const uint32_t ib = item_start[key_begin], ie = item_start[key_end];
+    // the batch's tasks and the work items of their keys
+    int t0, t1;
+    if (!batch_tasks(bref, batch, t0, t1))
+        return;
+    const uint32_t ib = item_start[(uint32_t)bref.task_slot[t0] * (uint32_t)NB];
+    const uint32_t ie = item_start[(uint32_t)(bref.task_slot[t1 - 1] + 1) * (uint32_t)NB];
+    // one workgroup per item: the grid must cover the batch's items
+    if (blockIdx.x == 0 && threadIdx.x == 0 && ie - ib > gridDim.x)
+        atomicAdd(&stats[CNT_OVERFLOW], 1ull);
     const uint32_t it = ib + blockIdx.x;
     if (it >= ie)
         return;
-    const Item item = items[it];
-    const uint32_t tslot = item.key / (uint32_t)NB;
-    const int b = (int)(item.key - tslot * (uint32_t)NB);
-    const int j = rank_of_slot[tslot] - t0;
-
-    // thread t < RS stages record t of each RS-record stage; the next
-    // stage's record is loaded while the current stage is processed
-    auto stage_rec = [&](uint32_t base) {
-        BandRec<T> r;
-        if (base + threadIdx.x < item.count) {
-            r = recs[item.start + base + threadIdx.x];
-        } else {
-            // padding, no visibility: padded table: column n, after every
-            // footprint column (its cells lie in the row gaps); otherwise a
-            // row out of reach of every band row (no tap, cell kept)
-            r.xy = padded_taps<T>() ? (uint32_t)((n + S) * z) : 0x40000000u;
-            r.kb = 0;
-            r.vis.x = T(0);
-            r.vis.y = T(0);
-        }
-        return r;
-    };
-    BandRec<T> next;
-    if (threadIdx.x < RS)
-        next = stage_rec(0);
-    for (int e = threadIdx.x; e < BANDE; e += NT) {
-        band[e].x = T(0);
-        band[e].y = T(0);
-    }
     for (int e = threadIdx.x; e < n; e += NT)
         tw[e] = twg[e];
-    const __amdgpu_buffer_rsrc_t ktab =
-        __builtin_amdgcn_make_buffer_rsrc((void *)kern, 0, (int)kern_bytes, BUF_RSRC_W3);
-    const int lane = threadIdx.x % NL, grp = threadIdx.x / NL;
-    const int myU = lane % S, myV = lane / S;
-    // per lane, in bytes: column residue, tap row, cell row
-    const int uz = myU * z, tap_row = myV * S * z, cell_row = myV * BS * z;
-    char *band_b = reinterpret_cast<char *>(band);
-    int cur = -1, carry = -1; // carry: the kept cell of the unpadded form
-    T sr = T(0), si = T(0);
-    auto flush = [&]() {
-        T *c = reinterpret_cast<T *>(band_b + cur);
-        if (GRP > 1) {
-            atomicAdd(c, sr);
-            atomicAdd(c + 1, si);
-        } else {
-            c[0] += sr;
-            c[1] += si;
+    {
+        const Item item = items[it];
+        const uint32_t tslot = item.key / (uint32_t)NB;
+        const int b = (int)(item.key - tslot * (uint32_t)NB);
+        const int j = rank_of_slot[tslot] - t0;
+
+        // thread t < RS stages record t of each RS-record stage; the next
+        // stage's record is loaded while the current stage is processed
+        auto stage_rec = [&](uint32_t base) {
+            BandRec<T> r;
+            if (base + threadIdx.x < item.count) {
+                r = recs[item.start + base + threadIdx.x];
+            } else {
+                // padding, no visibility: padded table: column n, after every
+                // footprint column (its cells lie in the row gaps); otherwise a
+                // row out of reach of every band row (no tap, cell kept)
+                r.xy = padded_taps<T>() ? (uint32_t)((n + S) * z) : 0x40000000u;
+                r.kb = 0;
+                r.vis.x = T(0);
+                r.vis.y = T(0);
+            }
+            return r;
+        };
+        BandRec<T> next;
+        if (threadIdx.x < RS)
+            next = stage_rec(0);
+        for (int e = threadIdx.x; e < BANDE; e += NT) {
+            band[e].x = T(0);
+            band[e].y = T(0);
         }
-    };
-    // U records at a time: U tap loads in flight, then the U updates in order.
-    constexpr int U = 8;
-    static_assert(RG % U == 0, "record batch must be a multiple of the unroll");
-    for (uint32_t base = 0; base < item.count; base += RS) {
-        const uint32_t nloc = min((uint32_t)RS, item.count - base);
-        __syncthreads();
-        if (threadIdx.x < RS) {
-            s_rec[threadIdx.x] = next;
-            if (base + RS < item.count)
-                next = stage_rec(base + RS);
-        }
-        __syncthreads();
-        const int ng = min(RG, max(0, (int)nloc - grp * RG));
-        const BandRec<T> *gr = s_rec + grp * RG;
-        using Bt = RecBatch<T, U>;
-        // A lane whose row the footprint misses reads a zero tap: from the
-        // padding of the record's sub-kernel (its cell then follows the
-        // footprint column like every other lane's, so it changes cell with
-        // them and flushes only when it has summed something), or, unpadded,
-        // through an offset past the table, keeping its previous cell.
-        auto fetch = [&](uint32_t q0, Bt &bt) {
-#pragma unroll
-            for (int q = 0; q < U; ++q) {
-                const BandRec<T> r = gr[q0 + q];
-                const int gz = (int)(r.xy & 0xffffu);
-                const int cxz = (uz - gz) & ((S - 1) * z);
-                bt.v[q] = r.vis;
-                if constexpr (padded_taps<T>()) {
-                    bt.cell[q] = cell_row + gz + cxz;
-                    bt.k[q] = tap_load(ktab, (uint32_t)(r.kb + tap_row + cxz), (T2 *)nullptr);
-                } else {
-                    const int cy = ((int)r.xy >> 16) + myV;
-                    const bool in = (unsigned)cy < (unsigned)S;
-                    carry = in ? cell_row + gz + cxz : carry;
-                    bt.cell[q] = carry;
-                    bt.k[q] = tap_load(ktab, in ? (uint32_t)(r.kb + tap_row + cxz) : TAP_NONE,
-                                       (T2 *)nullptr);
-                }
+        const __amdgpu_buffer_rsrc_t ktab =
+            __builtin_amdgcn_make_buffer_rsrc((void *)kern, 0, (int)kern_bytes, BUF_RSRC_W3);
+        const int lane = threadIdx.x % NL, grp = threadIdx.x / NL;
+        const int myU = lane % S, myV = lane / S;
+        // per lane, in bytes: column residue, tap row, cell row
+        const int uz = myU * z, tap_row = myV * S * z, cell_row = myV * BS * z;
+        char *band_b = reinterpret_cast<char *>(band);
+        int cur = -1, carry = -1; // carry: the kept cell of the unpadded form
+        T sr = T(0), si = T(0);
+        auto flush = [&]() {
+            T *c = reinterpret_cast<T *>(band_b + cur);
+            if (GRP > 1) {
+                atomicAdd(c, sr);
+                atomicAdd(c + 1, si);
+            } else {
+                c[0] += sr;
+                c[1] += si;
             }
         };
-        // A lane's cell never decreases along the records (they are sorted by
-        // footprint column, and the cell is the first column >= gx of the
-        // lane's residue), so no lane changes cell in a batch iff every lane's
-        // last cell is its current one: then the batch is plain FMAs.
-        auto update = [&](const Bt &bt) {
-            if (__all(bt.cell[U - 1] == cur)) {
-#pragma unroll
+        // U records at a time: U tap loads in flight, then the U updates in order.
+        constexpr int U = 8;
+        static_assert(RG % U == 0, "record batch must be a multiple of the unroll");
+        for (uint32_t base = 0; base < item.count; base += RS) {
+            const uint32_t nloc = min((uint32_t)RS, item.count - base);
+            __syncthreads();
+            if (threadIdx.x < RS) {
+                s_rec[threadIdx.x] = next;
+                if (base + RS < item.count)
+                    next = stage_rec(base + RS);
+            }
+            __syncthreads();
+            const int ng = min(RG, max(0, (int)nloc - grp * RG));
+            const BandRec<T> *gr = s_rec + grp * RG;
+            using Bt = RecBatch<T, U>;
+            // A lane whose row the footprint misses reads a zero tap: from the
+            // padding of the record's sub-kernel (its cell then follows the
+            // footprint column like every other lane's, so it changes cell with
+            // them and flushes only when it has summed something), or, unpadded,
+            // through an offset past the table, keeping its previous cell.
+            auto fetch = [&](uint32_t q0, Bt &bt) {
+    #pragma unroll
                 for (int q = 0; q < U; ++q) {
+                    const BandRec<T> r = gr[q0 + q];
+                    const int gz = (int)(r.xy & 0xffffu);
+                    const int cxz = (uz - gz) & ((S - 1) * z);
+                    bt.v[q] = r.vis;
+                    if constexpr (padded_taps<T>()) {
+                        bt.cell[q] = cell_row + gz + cxz;
+                        bt.k[q] = tap_load(ktab, (uint32_t)(r.kb + tap_row + cxz), (T2 *)nullptr);
+                    } else {
+                        const int cy = ((int)r.xy >> 16) + myV;
+                        const bool in = (unsigned)cy < (unsigned)S;
+                        carry = in ? cell_row + gz + cxz : carry;
+                        bt.cell[q] = carry;
+                        bt.k[q] = tap_load(ktab, in ? (uint32_t)(r.kb + tap_row + cxz) : TAP_NONE,
+                                           (T2 *)nullptr);
+                    }
+                }
+            };
+            // A lane's cell never decreases along the records (they are sorted by
+            // footprint column, and the cell is the first column >= gx of the
+            // lane's residue), so no lane changes cell in a batch iff every lane's
+            // last cell is its current one: then the batch is plain FMAs.
+            auto update = [&](const Bt &bt) {
+                if (__all(bt.cell[U - 1] == cur)) {
+    #pragma unroll
+                    for (int q = 0; q < U; ++q) {
+                        sr = fma(bt.k[q].x, bt.v[q].x, sr);
+                        sr = fma(bt.k[q].y, bt.v[q].y, sr);
+                        si = fma(bt.k[q].x, bt.v[q].y, si);
+                        si = fma(-bt.k[q].y, bt.v[q].x, si);
+                    }
+                    return;
+                }
+    #pragma unroll
+                for (int q = 0; q < U; ++q) {
+                    if (bt.cell[q] != cur) {
+                        if (cur >= 0 && (sr != T(0) || si != T(0)))
+                            flush();
+                        cur = bt.cell[q];
+                        sr = T(0);
+                        si = T(0);
+                    }
+                    // conj(k) * v
                     sr = fma(bt.k[q].x, bt.v[q].x, sr);
                     sr = fma(bt.k[q].y, bt.v[q].y, sr);
                     si = fma(bt.k[q].x, bt.v[q].y, si);
                     si = fma(-bt.k[q].y, bt.v[q].x, si);
                 }
+            };
+            pipelined_records<U, Bt>((uint32_t)((ng + U - 1) / U * U), fetch, update);
+        }
+        if (cur >= 0 && (sr != T(0) || si != T(0)))
+            flush();
+
+        // destination: the task's subgrid band (column blocks), or a spill slot
+        // (H rows of n) for a split key
+        const uint32_t k0 = item_start[item.key], k1 = item_start[item.key + 1];
+        T2 *dst;
+        const bool whole = k1 - k0 == 1;
+        if (whole) {
+            dst = G + (size_t)j * n * n;
+        } else {
+            if (threadIdx.x == 0) {
+                const uint32_t sl = atomicAdd(spill_ctr, 1u);
+                item_spill[it - ib] = sl;
+                s_slot = sl;
+            }
+            __syncthreads();
+            // the capacity bounds every batch (at most 2 R / K parts); an overrun
+            // would lose this part, so it is counted as an error of the call
+            if (s_slot >= spill_cap) {
+                if (threadIdx.x == 0)
+                    atomicAdd(&stats[CNT_OVERFLOW], 1ull);
                 return;
             }
-#pragma unroll
-            for (int q = 0; q < U; ++q) {
-                if (bt.cell[q] != cur) {
-                    if (cur >= 0 && (sr != T(0) || si != T(0)))
-                        flush();
-                    cur = bt.cell[q];
-                    sr = T(0);
-                    si = T(0);
-                }
-                // conj(k) * v
-                sr = fma(bt.k[q].x, bt.v[q].x, sr);
-                sr = fma(bt.k[q].y, bt.v[q].y, sr);
-                si = fma(bt.k[q].x, bt.v[q].y, si);
-                si = fma(-bt.k[q].y, bt.v[q].x, si);
-            }
-        };
-        pipelined_records<U, Bt>((uint32_t)((ng + U - 1) / U * U), fetch, update);
-    }
-    if (cur >= 0 && (sr != T(0) || si != T(0)))
-        flush();
-
-    // destination: the task's subgrid band (column blocks), or a spill slot
-    // (H rows of n) for a split key
-    const uint32_t k0 = item_start[item.key], k1 = item_start[item.key + 1];
-    T2 *dst;
-    const bool whole = k1 - k0 == 1;
-    if (whole) {
-        dst = G + (size_t)j * n * n;
-    } else {
-        if (threadIdx.x == 0) {
-            const uint32_t sl = atomicAdd(spill_ctr, 1u);
-            item_spill[it - ib] = sl;
-            s_slot = sl;
+            dst = spill + (size_t)s_slot * H * n;
         }
         __syncthreads();
-        // the capacity bounds every batch (at most 2 R / K parts); an overrun
-        // would lose this part, so it is counted as an error of the call
-        if (s_slot >= spill_cap) {
-            if (threadIdx.x == 0)
-                atomicAdd(&stats[CNT_OVERFLOW], 1ull);
-            return;
-        }
-        dst = spill + (size_t)s_slot * H * n;
-    }
-    __syncthreads();
-    T2 *rows = band + S;
-    fft_lines_lm<T2, LOGN, NT, BS, true>(rows, tw, H);
-    if (whole) {
-        // rows [H b, H b + H) of column block cb are H * GCB contiguous elements
-        T2 *d0 = dst + (size_t)H * b * GCB;
-        for (int e = threadIdx.x; e < H * n; e += NT) {
-            const int cb = e / (H * GCB), w = e - cb * (H * GCB);
-            const int l = w / GCB, c = cb * GCB + (w - l * GCB);
-            d0[(size_t)cb * n * GCB + w] = rows[l * BS + c];
-        }
-    } else {
-        for (int e = threadIdx.x; e < H * n; e += NT) {
-            const int l = e / n, c = e - l * n;
-            dst[e] = rows[l * BS + c];
+        T2 *rows = band + S;
+        fft_lines_lm<T2, LOGN, NT, BS, true>(rows, tw, H);
+        if (whole) {
+            // rows [H b, H b + H) of column block cb are H * GCB contiguous elements
+            T2 *d0 = dst + (size_t)H * b * GCB;
+            for (int e = threadIdx.x; e < H * n; e += NT) {
+                const int cb = e / (H * GCB), w = e - cb * (H * GCB);
+                const int l = w / GCB, c = cb * GCB + (w - l * GCB);
+                d0[(size_t)cb * n * GCB + w] = rows[l * BS + c];
+            }
+        } else {
+            for (int e = threadIdx.x; e < H * n; e += NT) {
+                const int l = e / n, c = e - l * n;
+                dst[e] = rows[l * BS + c];
+            }
         }
     }
 }
 
 // Sums the spilled parts of every key split over several items (in item
 // order) into the task's subgrid band.  Launched with the grid of
-// k_tower_band_items; the P blocks of a key split into P parts each sum a
-// 1 / P share of the band's elements.
+// k_tower_band_items (one block per item of the batch); the P blocks of a key
+// split into P parts each sum a 1 / P share of the band's elements.
 template <typename T2, int H, int LOGN>
 __global__ __launch_bounds__(256) void k_band_combine(const Item *__restrict__ items,
                                                       const uint32_t *__restrict__ item_start,
-                                                      uint32_t key_begin, uint32_t key_end,
+                                                      BatchRef bref, int batch,
                                                       const int32_t *__restrict__ rank_of_slot,
-                                                      int t0, T2 *G, const T2 *__restrict__ spill,
+                                                      T2 *G, const T2 *__restrict__ spill,
                                                       uint32_t spill_cap,
                                                       const uint32_t *__restrict__ item_spill)
 {
     constexpr int n = 1 << LOGN;
     constexpr int NB = n / H;
-    const uint32_t ib = item_start[key_begin], ie = item_start[key_end];
+    int t0, t1;
+    if (!batch_tasks(bref, batch, t0, t1))
+        return;
+    const uint32_t ib = item_start[(uint32_t)bref.task_slot[t0] * (uint32_t)NB];
+    const uint32_t ie = item_start[(uint32_t)(bref.task_slot[t1 - 1] + 1) * (uint32_t)NB];
     const uint32_t it = ib + blockIdx.x;
     if (it >= ie)
         return;
-    const uint32_t key = items[it].key;
-    const uint32_t k0 = item_start[key], k1 = item_start[key + 1];
-    const uint32_t parts = k1 - k0;
-    if (parts < 2)
-        return;
-    const uint32_t tslot = key / (uint32_t)NB;
-    const int b = (int)(key - tslot * (uint32_t)NB);
-    const int j = rank_of_slot[tslot] - t0;
-    T2 *dst = G + (size_t)j * n * n;
-    const uint32_t share = it - k0;
-    const int e0 = (int)((uint64_t)(H * n) * share / parts);
-    const int e1 = (int)((uint64_t)(H * n) * (share + 1) / parts);
-    for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x) {
-        T2 a;
-        a.x = 0;
-        a.y = 0;
-        for (uint32_t q = k0; q < k1; ++q) {
-            const uint32_t sl = item_spill[q - ib];
-            if (sl < spill_cap) {
-                const T2 v = spill[(size_t)sl * H * n + e];
-                a.x += v.x;
-                a.y += v.y;
+    {
+        const uint32_t key = items[it].key;
+        const uint32_t k0 = item_start[key], k1 = item_start[key + 1];
+        const uint32_t parts = k1 - k0;
+        if (parts < 2)
+            return;
+        const uint32_t tslot = key / (uint32_t)NB;
+        const int b = (int)(key - tslot * (uint32_t)NB);
+        const int j = rank_of_slot[tslot] - t0;
+        T2 *dst = G + (size_t)j * n * n;
+        const uint32_t share = it - k0;
+        const int e0 = (int)((uint64_t)(H * n) * share / parts);
+        const int e1 = (int)((uint64_t)(H * n) * (share + 1) / parts);
+        for (int e = e0 + (int)threadIdx.x; e < e1; e += blockDim.x) {
+            T2 a;
+            a.x = 0;
+            a.y = 0;
+            for (uint32_t q = k0; q < k1; ++q) {
+                const uint32_t sl = item_spill[q - ib];
+                if (sl < spill_cap) {
+                    const T2 v = spill[(size_t)sl * H * n + e];
+                    a.x += v.x;
+                    a.y += v.y;
+                }
             }
+            dst[gcb_index(H * b + e / n, e % n, n)] = a;
         }
-        dst[gcb_index(H * b + e / n, e % n, n)] = a;
     }
 }
 
@@ -663,11 +880,10 @@ constexpr int SCREEN_RUN = 8;
 template <typename T, int LOGN, int CB, int H>
 __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
     const typename Vec2<T>::type *__restrict__ G, const uint32_t *__restrict__ key_start,
-    const int32_t *__restrict__ task_slot, typename Vec2<T>::type *P,
-    const int32_t *__restrict__ chunk_first,
-    const int32_t *__restrict__ chunk_count, int chunk_base, int32_t t0,
-    const int32_t *__restrict__ task_wp, const typename Vec2<T>::type *__restrict__ twg,
-    double theta, double winc)
+    BatchRef bref, int batch, const int32_t *__restrict__ task_wp,
+    const int32_t *__restrict__ chunk_first, const int32_t *__restrict__ chunk_count,
+    typename Vec2<T>::type *P, const typename Vec2<T>::type *__restrict__ twg, double theta,
+    double winc)
 {
     using T2 = typename Vec2<T>::type;
     constexpr int n = 1 << LOGN;
@@ -678,10 +894,14 @@ __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
     __shared__ T2 tw[n];
     __shared__ uint8_t s_band[NB]; // band b of the current floor has records
     static_assert(NB <= NT, "one thread per band flag");
+    int t0, t1;
+    if (!batch_tasks(bref, batch, t0, t1))
+        return;
+    const int nch = bref.totals[2];
+    if ((int)blockIdx.y >= nch)
+        return;
     for (int k = threadIdx.x; k < n; k += NT)
         tw[k] = twg[k];
-    const int c = chunk_base + blockIdx.y;
-    const int first = chunk_first[c], count = chunk_count[c];
     const int line = threadIdx.x % CB;
     const int col = blockIdx.x * CB + line;
 
@@ -693,8 +913,6 @@ __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
     for (int q = 0; q < PER; ++q) {
         base[q] = screen_base(col, fft_out_index<LOGN, CB>(q), n, theta);
         Trig<T>::rot(winc * base[q], step[q].x, step[q].y);
-        acc[q].x = T(0);
-        acc[q].y = T(0);
     }
     // acc += h E^wl
     auto close_run = [&](int wl) {
@@ -708,111 +926,138 @@ __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
         }
     };
     static_assert(CB == GCB || (1 << LOGN) < GCB, "column pass reads whole column blocks");
-    int run = 0, wp_prev = 0;
-    for (int k = first; k < first + count; ++k) {
-        const T2 *g = G + (size_t)(k - t0) * n * n + (size_t)blockIdx.x * n * CB + line;
-        // band b of the task was written iff it has work items (the previous
-        // floor's flags were read in its first stage, before its barrier)
-        if (threadIdx.x < NB) {
-            const uint32_t *f = key_start + (size_t)task_slot[k] * NB;
-            s_band[threadIdx.x] = f[threadIdx.x + 1] > f[threadIdx.x] ? 1 : 0;
-        }
-        const int wp = task_wp[k];
-        if (run > 0 && (run == SCREEN_RUN || wp != wp_prev + 1)) {
-            close_run(wp_prev);
-            run = 0;
-        }
-        const bool extend = run > 0;
-        auto load = [&](int, int row) {
-            T2 val;
-            val.x = T(0);
-            val.y = T(0);
-            if (s_band[row / H])
-                val = g[(size_t)row * CB];
-            return val;
-        };
-        auto emit = [&](int q, int, T2 val) {
-            if (extend) { // h conj(E) + val
-                const T hx = h[q].x, hy = h[q].y;
-                h[q].x = fma(hx, step[q].x, fma(-hy, step[q].y, val.x));
-                h[q].y = fma(hx, step[q].y, fma(hy, step[q].x, val.y));
-            } else {
-                h[q] = val;
-            }
-        };
-        __syncthreads(); // the previous floor's last stage has read lds; band flags set
-        fft_block_io<T2, LOGN, CB, CB, true>(lds, tw, load, emit);
-        ++run;
-        wp_prev = wp;
-    }
-    if (run > 0)
-        close_run(wp_prev);
-    T2 *pp = P + (size_t)blockIdx.y * n * n + col;
+    for (int c = blockIdx.y; c < nch; c += gridDim.y) {
+        // the chunk's tasks inside this batch
+        const int first = max(chunk_first[c], t0);
+        const int last = min(chunk_first[c] + chunk_count[c], t1);
+        if (first >= last)
+            continue;
 #pragma unroll
-    for (int q = 0; q < PER; ++q)
-        pp[(size_t)fft_out_index<LOGN, CB>(q) * n] = acc[q];
+        for (int q = 0; q < PER; ++q) {
+            acc[q].x = T(0);
+            acc[q].y = T(0);
+        }
+        int run = 0, wp_prev = 0;
+        for (int k = first; k < last; ++k) {
+            const T2 *g = G + (size_t)(k - t0) * n * n + (size_t)blockIdx.x * n * CB + line;
+            // band b of the task was written iff it has work items (the
+            // previous floor's flags were read in its first stage, before its
+            // barrier)
+            if (threadIdx.x < NB) {
+                const uint32_t *f = key_start + (size_t)bref.task_slot[k] * NB;
+                s_band[threadIdx.x] = f[threadIdx.x + 1] > f[threadIdx.x] ? 1 : 0;
+            }
+            const int wp = task_wp[k];
+            if (run > 0 && (run == SCREEN_RUN || wp != wp_prev + 1)) {
+                close_run(wp_prev);
+                run = 0;
+            }
+            const bool extend = run > 0;
+            auto load = [&](int, int row) {
+                T2 val;
+                val.x = T(0);
+                val.y = T(0);
+                if (s_band[row / H])
+                    val = g[(size_t)row * CB];
+                return val;
+            };
+            auto emit = [&](int q, int, T2 val) {
+                if (extend) { // h conj(E) + val
+                    const T hx = h[q].x, hy = h[q].y;
+                    h[q].x = fma(hx, step[q].x, fma(-hy, step[q].y, val.x));
+                    h[q].y = fma(hx, step[q].y, fma(hy, step[q].x, val.y));
+                } else {
+                    h[q] = val;
+                }
+            };
+            __syncthreads(); // the previous floor's last stage has read lds; band flags set
+            fft_block_io<T2, LOGN, CB, CB, true>(lds, tw, load, emit);
+            ++run;
+            wp_prev = wp;
+        }
+        if (run > 0)
+            close_run(wp_prev);
+        T2 *pp = P + (size_t)c * n * n + col;
+#pragma unroll
+        for (int q = 0; q < PER; ++q)
+            pp[(size_t)fft_out_index<LOGN, CB>(q) * n] = acc[q];
+    }
 }
 
-// subimage[tower of run r] += sum of the partial images of the run's chunks
+// subimage[r] += the partial images of rank r's chunks that meet the batch
 template <typename T2>
 __global__ __launch_bounds__(256) void k_tower_reduce(const T2 *__restrict__ P, T2 *subimg,
-                                                      int64_t nn, const int32_t *run_tower,
-                                                      const int32_t *run_chunk0,
-                                                      const int32_t *run_nchunk, int run_base)
+                                                      int64_t nn, BatchRef bref, int batch,
+                                                      const int32_t *__restrict__ rank_chunk0,
+                                                      const int32_t *__restrict__ rank_nchunk,
+                                                      const int32_t *__restrict__ chunk_first,
+                                                      const int32_t *__restrict__ chunk_count)
 {
-    const int r = run_base + blockIdx.y;
-    const int tower = run_tower[r], c0 = run_chunk0[r], nc = run_nchunk[r];
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        T2 a = subimg[(size_t)tower * nn + i];
-        for (int c = c0; c < c0 + nc; ++c) {
-            const T2 p = P[(size_t)c * nn + i];
-            a.x += p.x;
-            a.y += p.y;
+    int t0, t1;
+    if (!batch_tasks(bref, batch, t0, t1))
+        return;
+    const int ntow = bref.totals[1], nch = bref.totals[2];
+    for (int r = blockIdx.y; r < ntow; r += gridDim.y) {
+        const int c0 = rank_chunk0[r], c1 = min(nch, c0 + rank_nchunk[r]);
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn;
+             i += (int64_t)gridDim.x * blockDim.x) {
+            T2 a = subimg[(size_t)r * nn + i];
+            for (int c = c0; c < c1; ++c) {
+                if (chunk_first[c] >= t1 || chunk_first[c] + chunk_count[c] <= t0)
+                    continue;
+                const T2 p = P[(size_t)c * nn + i];
+                a.x += p.x;
+                a.y += p.y;
+            }
+            subimg[(size_t)r * nn + i] = a;
         }
-        subimg[(size_t)tower * nn + i] = a;
     }
 }
 
 // ---- fold ------------------------------------------------------------------------------------
 // Owner computes: each master pixel sums the towers whose window covers it.
+// Channel c's master grid is the c-th N x N grid; its towers have ids
+// (c cc + cy) cc + cx.
 template <typename T2>
-__global__ __launch_bounds__(256) void k_fold(double2 *grid, int N, const T2 *subimg,
-                                              const int32_t *tower_rank, int n, int margin,
-                                              int chunk, int cc)
+__global__ __launch_bounds__(256) void k_fold(double2 *grid, int N, int64_t total,
+                                              const T2 *subimg, const int32_t *rank_of_tower, int n,
+                                              int margin, int chunk, int cc)
 {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)N * N)
-        return;
-    const int py = (int)(idx / N), px = (int)(idx - (int64_t)py * N);
-    const int x = px - N / 2, y = py - N / 2;
-    const double norm = (double)n * (double)n;
-    double re = 0.0, im = 0.0;
-    bool any = false;
-    const int ax = x + N / 2 + margin / 2, ay = y + N / 2 + margin / 2;
-    const int cxh = min(cc - 1, ax / chunk), cyh = min(cc - 1, ay / chunk);
-    for (int cy = max(0, cyh - 2); cy <= cyh; ++cy) {
-        const int y0 = chunk * cy - N / 2 - margin / 2;
-        if (y < y0 || y >= y0 + n)
-            continue;
-        for (int cx = max(0, cxh - 2); cx <= cxh; ++cx) {
-            const int x0 = chunk * cx - N / 2 - margin / 2;
-            if (x < x0 || x >= x0 + n)
+    const int64_t NN = (int64_t)N * N;
+    for (int64_t gidx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; gidx < total;
+         gidx += (int64_t)gridDim.x * blockDim.x) {
+        const int ch = (int)(gidx / NN);
+        const int64_t idx = gidx - (int64_t)ch * NN;
+        const int py = (int)(idx / N), px = (int)(idx - (int64_t)py * N);
+        const int x = px - N / 2, y = py - N / 2;
+        const double norm = (double)n * (double)n;
+        double re = 0.0, im = 0.0;
+        bool any = false;
+        const int ax = x + N / 2 + margin / 2, ay = y + N / 2 + margin / 2;
+        const int cxh = min(cc - 1, ax / chunk), cyh = min(cc - 1, ay / chunk);
+        for (int cy = max(0, cyh - 2); cy <= cyh; ++cy) {
+            const int y0 = chunk * cy - N / 2 - margin / 2;
+            if (y < y0 || y >= y0 + n)
                 continue;
-            const int rk = tower_rank[cy * cc + cx];
-            if (rk < 0)
-                continue;
-            const T2 s = subimg[(size_t)rk * n * n + (size_t)(y - y0) * n + (x - x0)];
-            re += (double)s.x / norm;
-            im += (double)s.y / norm;
-            any = true;
+            for (int cx = max(0, cxh - 2); cx <= cxh; ++cx) {
+                const int x0 = chunk * cx - N / 2 - margin / 2;
+                if (x < x0 || x >= x0 + n)
+                    continue;
+                const int rk = rank_of_tower[(ch * cc + cy) * cc + cx];
+                if (rk < 0)
+                    continue;
+                const T2 s = subimg[(size_t)rk * n * n + (size_t)(y - y0) * n + (x - x0)];
+                re += (double)s.x / norm;
+                im += (double)s.y / norm;
+                any = true;
+            }
         }
-    }
-    if (any) {
-        double2 g = grid[idx];
-        g.x += re;
-        g.y += im;
-        grid[idx] = g;
+        if (any) {
+            double2 g = grid[gidx];
+            g.x += re;
+            g.y += im;
+            grid[gidx] = g;
+        }
     }
 }
 
@@ -896,16 +1141,9 @@ __global__ __launch_bounds__(256) void k_w0_transfer(double2 *img, int n, int la
 
 // ---- host -------------------------------------------------------------------------------------
 
-struct TaskList {
-    std::vector<int32_t> slot, tower_rank, wp;
-    std::vector<uint32_t> nvis;
-    std::vector<int32_t> tower_of_rank; // tower index of each active tower
-    std::vector<int32_t> rank_of_tower; // -1 when inactive
-};
-
 // Band height: rows of the subgrid one workgroup owns.  H rows of n complex
 // values take H n sizeof(T2) bytes of LDS; 8 rows of FP32 leave room for four
-// workgroups per CU.  FP64 supports 8 only.
+// workgroups per CU.  16 rows are FP32 only.
 static int band_height_of(const rg_grid_params *p, bool fp32)
 {
     if (p->tile == 16 && fp32)
@@ -919,16 +1157,16 @@ struct BandLaunch {
     const void *recs;
     const Item *items;
     const uint32_t *item_start;
-    uint32_t key_begin, key_end; // keys of the batch's tasks
+    BatchRef bref;
+    int batch;
     const int32_t *rank_of_slot;
-    int t0;
     const void *kern, *tw;
     uint32_t kern_bytes; // kernel table size (< 2^31)
     void *G, *spill;
     uint32_t spill_cap;
     uint32_t *spill_ctr, *item_spill;
     unsigned long long *stats;
-    uint32_t blocks; // bound on the batch's items
+    uint32_t blocks; // workgroups (each loops over the batch's items)
 };
 
 template <typename T, int S, int H, int LOGN>
@@ -937,13 +1175,12 @@ static int launch_bands_h(const BandLaunch &a, hipStream_t s)
     using T2 = typename Vec2<T>::type;
     constexpr int GRP = S * H >= 256 ? 1 : 256 / (S * H);
     hipLaunchKernelGGL((k_tower_band_items<T, S, H, LOGN, GRP>), dim3(a.blocks), dim3(S * H * GRP),
-                       0, s, (const BandRec<T> *)a.recs, a.items, a.item_start, a.key_begin,
-                       a.key_end, a.rank_of_slot, a.t0, (const T2 *)a.kern, a.kern_bytes,
-                       (const T2 *)a.tw, (T2 *)a.G, (T2 *)a.spill, a.spill_cap, a.spill_ctr,
-                       a.item_spill, a.stats);
+                       0, s, (const BandRec<T> *)a.recs, a.items, a.item_start, a.bref, a.batch,
+                       a.rank_of_slot, (const T2 *)a.kern, a.kern_bytes, (const T2 *)a.tw,
+                       (T2 *)a.G, (T2 *)a.spill, a.spill_cap, a.spill_ctr, a.item_spill, a.stats);
     RG_HIP(hipGetLastError());
     hipLaunchKernelGGL((k_band_combine<T2, H, LOGN>), dim3(a.blocks), dim3(256), 0, s, a.items,
-                       a.item_start, a.key_begin, a.key_end, a.rank_of_slot, a.t0, (T2 *)a.G,
+                       a.item_start, a.bref, a.batch, a.rank_of_slot, (T2 *)a.G,
                        (const T2 *)a.spill, a.spill_cap, (const uint32_t *)a.item_spill);
     RG_HIP(hipGetLastError());
     return RG_OK;
@@ -961,39 +1198,6 @@ static int launch_bands_t(int H, const BandLaunch &a, hipStream_t s)
     return launch_bands_h<T, S, 8, LOGN>(a, s);
 }
 
-template <typename T, int LOGN, int H>
-static int launch_cols_h(const void *G, const uint32_t *ks, const int32_t *tslot, void *P,
-                         const int32_t *cf, const int32_t *cc, int chunk_base, int nchunks,
-                         int32_t t0, const int32_t *twp, const void *tw, double theta, double winc,
-                         hipStream_t s)
-{
-    using T2 = typename Vec2<T>::type;
-    constexpr int n = 1 << LOGN;
-    constexpr int CB0 = 8; // columns per workgroup (512 threads at n = 512)
-    constexpr int CB = CB0 > n ? n : CB0;
-    hipLaunchKernelGGL((k_tower_cols<T, LOGN, CB, H>), dim3(n / CB, nchunks), dim3(CB * (n / 8)),
-                       0, s, (const T2 *)G, ks, tslot, (T2 *)P, cf, cc, chunk_base, t0, twp,
-                       (const T2 *)tw, theta, winc);
-    RG_HIP(hipGetLastError());
-    return RG_OK;
-}
-
-template <typename T, int LOGN>
-static int launch_cols_t(int H, const void *G, const uint32_t *ks, const int32_t *tslot, void *P,
-                         const int32_t *cf, const int32_t *cc, int chunk_base, int nchunks,
-                         int32_t t0, const int32_t *twp, const void *tw, double theta, double winc,
-                         hipStream_t s)
-{
-    if (H == 16)
-        return launch_cols_h<T, LOGN, 16>(G, ks, tslot, P, cf, cc, chunk_base, nchunks, t0, twp,
-                                          tw, theta, winc, s);
-    if (H == 4)
-        return launch_cols_h<T, LOGN, 4>(G, ks, tslot, P, cf, cc, chunk_base, nchunks, t0, twp,
-                                         tw, theta, winc, s);
-    return launch_cols_h<T, LOGN, 8>(G, ks, tslot, P, cf, cc, chunk_base, nchunks, t0, twp, tw,
-                                     theta, winc, s);
-}
-
 template <typename T, int S>
 static int launch_bands(int n, int H, const BandLaunch &a, hipStream_t s)
 {
@@ -1006,122 +1210,180 @@ static int launch_bands(int n, int H, const BandLaunch &a, hipStream_t s)
     }
 }
 
+struct ColsLaunch {
+    const void *G;
+    const uint32_t *key_start;
+    BatchRef bref;
+    int batch;
+    const int32_t *task_wp, *chunk_first, *chunk_count;
+    void *P;
+    const void *tw;
+    double theta, winc;
+    unsigned grid_y; // workgroup rows (each loops over the chunks)
+};
+
+template <typename T, int LOGN, int H>
+static int launch_cols_h(const ColsLaunch &a, hipStream_t s)
+{
+    using T2 = typename Vec2<T>::type;
+    constexpr int n = 1 << LOGN;
+    constexpr int CB0 = 8; // columns per workgroup (512 threads at n = 512)
+    constexpr int CB = CB0 > n ? n : CB0;
+    hipLaunchKernelGGL((k_tower_cols<T, LOGN, CB, H>), dim3(n / CB, a.grid_y), dim3(CB * (n / 8)),
+                       0, s, (const T2 *)a.G, a.key_start, a.bref, a.batch, a.task_wp,
+                       a.chunk_first, a.chunk_count, (T2 *)a.P, (const T2 *)a.tw, a.theta, a.winc);
+    RG_HIP(hipGetLastError());
+    return RG_OK;
+}
+
+template <typename T, int LOGN>
+static int launch_cols_t(int H, const ColsLaunch &a, hipStream_t s)
+{
+    if (H == 16)
+        return launch_cols_h<T, LOGN, 16>(a, s);
+    if (H == 4)
+        return launch_cols_h<T, LOGN, 4>(a, s);
+    return launch_cols_h<T, LOGN, 8>(a, s);
+}
+
 template <typename T>
-static int launch_cols(int n, int H, const void *G, const uint32_t *ks, const int32_t *tslot,
-                       void *P, const int32_t *cf, const int32_t *cc, int chunk_base, int nchunks,
-                       int32_t t0, const int32_t *twp, const void *tw, double theta, double winc,
-                       hipStream_t s)
+static int launch_cols(int n, int H, const ColsLaunch &a, hipStream_t s)
 {
     switch (n) {
-    case 64: return launch_cols_t<T, 6>(H, G, ks, tslot, P, cf, cc, chunk_base, nchunks, t0, twp, tw, theta, winc, s);
-    case 128: return launch_cols_t<T, 7>(H, G, ks, tslot, P, cf, cc, chunk_base, nchunks, t0, twp, tw, theta, winc, s);
-    case 256: return launch_cols_t<T, 8>(H, G, ks, tslot, P, cf, cc, chunk_base, nchunks, t0, twp, tw, theta, winc, s);
-    case 512: return launch_cols_t<T, 9>(H, G, ks, tslot, P, cf, cc, chunk_base, nchunks, t0, twp, tw, theta, winc, s);
+    case 64: return launch_cols_t<T, 6>(H, a, s);
+    case 128: return launch_cols_t<T, 7>(H, a, s);
+    case 256: return launch_cols_t<T, 8>(H, a, s);
+    case 512: return launch_cols_t<T, 9>(H, a, s);
     default: return fail(RG_ERR_UNSUPPORTED, "batched W-towers supports subgrid sizes 64..512 (got %d)", n);
     }
 }
 
-template <typename T>
-static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis,
-                     const rg_wkern *wk, void *grid, hipStream_t s)
+// Geometry of a W-towers call.
+struct WtGeom {
+    int N, n, M, S, chunk, cc, H, NB, mk, nchan;
+    int64_t ntw; // tower ids: nchan cc^2
+};
+
+static WtGeom wt_geom(const rg_grid_params *p, const rg_wkern *wk, int nchan, int H)
 {
-    using T2 = typename Vec2<T>::type;
-    const int N = p->grid_size, n = p->subgrid_size, M = p->subgrid_margin, S = wk->size;
-    const int chunk = n - M;
-    const int cc = N / chunk + 1; // grid_cuda.cu:537-539
-    const bool plain = p->mode == RG_MODE_PLAIN;
-    const int H = plain ? 0 : band_height_of(p, sizeof(T) == 4);
-    if (!plain && S != 8 && S != 16)
-        return fail(RG_ERR_UNSUPPORTED, "batched W-towers supports kernel sizes 8 and 16");
-    if (!plain && (n < 64 || n > 512))
-        return fail(RG_ERR_UNSUPPORTED, "batched W-towers supports subgrid sizes 64..512 (got %d)", n);
-    if (n + 2 * S >= 65536)
-        return fail(RG_ERR_UNSUPPORTED, "subgrid too large");
-    static_assert(SORT_BINS >= 512 + 16, "sort bins must cover n + S");
+    WtGeom g;
+    g.N = p->grid_size;
+    g.n = p->subgrid_size;
+    g.M = p->subgrid_margin;
+    g.S = wk->size;
+    g.chunk = g.n - g.M;
+    g.cc = g.N / g.chunk + 1; // grid_cuda.cu:537-539
+    g.H = H;
+    g.NB = H > 0 ? g.n / H : 1;
+    g.mk = H > 0 ? (g.S - 2) / H + 2 : 1; // bands an S-row footprint can reach
+    g.nchan = nchan;
+    g.ntw = (int64_t)nchan * g.cc * g.cc;
+    return g;
+}
 
-    RG_TRY(zero_fill(grid, sizeof(double2) * (size_t)N * N, s));
-    if (vis->n == 0)
-        return RG_OK;
-
-    // floor range (one small read-back)
-    RG_TRY(ctx->scratch[8].ensure(64));
-    int *d_range = ctx->scratch[8].as<int>();
-    hipLaunchKernelGGL(k_range_init, dim3(1), dim3(1), 0, s, d_range);
-    {
-        // few workgroups: each ends with one atomicMin / atomicMax on the
-        // same two words, which serialise
-        int64_t blocks = (vis->n + 255) / 256;
-        if (blocks > 1024)
-            blocks = 1024;
-        hipLaunchKernelGGL(k_floor_range, dim3((unsigned)blocks), dim3(256), 0, s, vis->u, vis->v,
-                           vis->w, vis->n, p->theta, N, chunk, cc, p->winc, d_range);
-        RG_HIP(hipGetLastError());
-    }
-    int h_range[2];
-    RG_HIP(hipMemcpyAsync(h_range, d_range, sizeof(h_range), hipMemcpyDeviceToHost, s));
-    RG_HIP(hipStreamSynchronize(s));
-    if (h_range[0] > h_range[1]) {
-        // every visibility outside the tower set
-        unsigned long long cnt[CNT__N] = {0};
-        cnt[CNT_DROPPED] = (unsigned long long)vis->n;
-        RG_HIP(hipMemcpyAsync(ctx->counters.p, cnt, sizeof(cnt), hipMemcpyHostToDevice, s));
-        RG_HIP(hipStreamSynchronize(s));
-        return RG_OK;
-    }
-    const int wp_min = h_range[0], nwp = h_range[1] - h_range[0] + 1;
-
+static TowerKeyer wt_keyer(const rg_grid_params *p, const rg_flat_vis *vis, const rg_wkern *wk,
+                           const WtGeom &g, const int *d_range, int nwp, const int64_t *d_off)
+{
     TowerKeyer kf;
     kf.u = vis->u;
     kf.v = vis->v;
     kf.w = vis->w;
     kf.vis = vis->vis;
     kf.vis_type = vis->vis_type;
-    kf.N = N;
-    kf.S = S;
+    kf.N = g.N;
+    kf.S = g.S;
     kf.R = wk->oversampling;
     kf.P = wk->plane_count;
     kf.theta = p->theta;
     kf.w_min = wk->w_min;
     kf.w_step = wk->w_step;
-    kf.n = n;
-    kf.chunk = chunk;
-    kf.cc = cc;
+    kf.n = g.n;
+    kf.chunk = g.chunk;
+    kf.cc = g.cc;
     kf.winc = p->winc;
-    kf.wp_min = wp_min;
+    kf.range = d_range;
     kf.nwp = nwp;
-    kf.band_h = H;
-    kf.nk = H > 0 ? n / H : 1;
-    kf.mk = H > 0 ? (S - 2) / H + 2 : 1; // bands an S-row footprint can reach
+    kf.band_h = g.H;
+    kf.nk = g.NB;
+    kf.mk = g.mk;
+    kf.chan_off = d_off;
+    kf.nchan = g.nchan;
+    return kf;
+}
+
+// Floor range of the call into d_range[0..1] (on the device).
+static int floor_range(const rg_grid_params *p, const rg_flat_vis *vis, const WtGeom &g,
+                       int *d_range, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_range_init, dim3(1), dim3(1), 0, s, d_range);
+    // few workgroups: each ends with one atomicMin / atomicMax on the same two
+    // words, which serialise
+    int64_t blocks = (vis->n + 255) / 256;
+    if (blocks > 1024)
+        blocks = 1024;
+    if (blocks > 0)
+        hipLaunchKernelGGL(k_floor_range, dim3((unsigned)blocks), dim3(256), 0, s, vis->u, vis->v,
+                           vis->w, vis->n, p->theta, g.N, g.chunk, g.cc, p->winc, d_range);
+    RG_HIP(hipGetLastError());
+    return RG_OK;
+}
+
+// Synchronous read of the floor range; false when no visibility falls in the
+// tower set.
+static int read_range(rg_context *ctx, const int *d_range, int &wp_min, int &nwp, bool &empty,
+                      hipStream_t s)
+{
+    int h[2];
+    RG_HIP(hipMemcpyAsync(h, d_range, sizeof(h), hipMemcpyDeviceToHost, s));
+    RG_HIP(hipStreamSynchronize(s));
+    empty = h[0] > h[1];
+    wp_min = h[0];
+    nwp = empty ? 0 : h[1] - h[0] + 1;
+    (void)ctx;
+    return RG_OK;
+}
+
+// ---- plain mode: the reference's launch structure (one task at a time) -------------------------
+
+struct TaskList {
+    std::vector<int32_t> slot, tower_rank, wp;
+    std::vector<uint32_t> nvis;
+    std::vector<int32_t> tower_of_rank; // tower index of each active tower
+    std::vector<int32_t> rank_of_tower; // -1 when inactive
+};
+
+static int wtowers_plain(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis,
+                         const rg_wkern *wk, void *grid, hipStream_t s)
+{
+    const WtGeom g = wt_geom(p, wk, 1, 0);
+    const int N = g.N, n = g.n, M = g.M, S = g.S, cc = g.cc;
+    RG_TRY(zero_fill(grid, sizeof(double2) * (size_t)N * N, s));
+    if (vis->n == 0)
+        return RG_OK;
+    RG_TRY(ctx->scratch[8].ensure(64));
+    int *d_range = ctx->scratch[8].as<int>();
+    RG_TRY(floor_range(p, vis, g, d_range, s));
+    int wp_min = 0, nwp = 0;
+    bool empty = false;
+    RG_TRY(read_range(ctx, d_range, wp_min, nwp, empty, s));
+    if (empty) {
+        unsigned long long cnt[CNT__N] = {0};
+        cnt[CNT_DROPPED] = (unsigned long long)vis->n;
+        RG_HIP(hipMemcpyAsync(ctx->counters.p, cnt, sizeof(cnt), hipMemcpyHostToDevice, s));
+        RG_HIP(hipStreamSynchronize(s));
+        return RG_OK;
+    }
+    const TowerKeyer kf = wt_keyer(p, vis, wk, g, d_range, nwp, nullptr);
     const int64_t nslots = (int64_t)cc * cc * nwp;
-    const int64_t nkeys = nslots * kf.nk;
-    // band records are addressed by 32-bit offsets: the expanded array holds
-    // up to mk records per visibility
-    if (!plain && (int64_t)kf.mk * vis->n >= ((int64_t)1 << 32))
-        return fail(RG_ERR_PARAM,
-                    "W-towers call too large: %lld visibilities x %d bands per footprint exceed "
-                    "2^32 band records; split the stream",
-                    (long long)vis->n, kf.mk);
-    // The sort makes one item per key (plain mode reads whole tasks).  Batched
-    // band work items of at most K records per band window are built after it.
-    const uint32_t K = p->vis_per_item > 0 ? (uint32_t)p->vis_per_item : 2048u;
-    const int64_t max_items_w = nkeys + (int64_t)kf.mk * vis->n / K + 1;
-    if (!plain)
-        RG_TRY(ctx->scratch[4].ensure(sizeof(Item) * std::max<int64_t>(nkeys + 1, max_items_w)));
-
     BinResult br;
-    // (task, band) keys: 4 counter replicas at the reference's workload (the
-    // scans read every replica; 16 were slower, 1 << 20 counters in all)
-    RG_TRY((bin_visibilities<TowerKeyer, T>(ctx, kf, vis->n, nkeys, 0x40000000u, s, br,
-                                            (int64_t)1 << 20)));
-
-    // visibilities per task slot (read back)
-    RG_TRY(ctx->scratch[9].ensure(sizeof(uint32_t) * (2 * nslots + nkeys + 1)));
+    RG_TRY((bin_visibilities<TowerKeyer, double>(ctx, kf, vis->n, nslots, 0x40000000u, s, br,
+                                                 (int64_t)1 << 20)));
+    RG_TRY(ctx->scratch[9].ensure(sizeof(uint32_t) * (2 * nslots + 1)));
     uint32_t *d_nvis = ctx->scratch[9].as<uint32_t>();
     uint32_t *d_item0 = d_nvis + nslots;
-    uint32_t *d_wstart = d_nvis + 2 * nslots;
     hipLaunchKernelGGL(k_task_info, dim3((unsigned)((nslots + 255) / 256)), dim3(256), 0, s,
-                       (const uint32_t *)br.vis_start, (const uint32_t *)br.item_start, nslots,
-                       kf.nk, d_nvis, d_item0);
+                       (const uint32_t *)br.vis_start, (const uint32_t *)br.item_start, nslots, 1,
+                       d_nvis, d_item0);
     RG_HIP(hipGetLastError());
     std::vector<uint32_t> h_nvis(nslots);
     RG_HIP(hipMemcpyAsync(h_nvis.data(), d_nvis, sizeof(uint32_t) * nslots, hipMemcpyDeviceToHost,
@@ -1153,277 +1415,450 @@ static int wtowers_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis
     }
     if (ntasks == 0)
         return RG_OK;
-
-    // subimages (one per active tower)
     const size_t nn = (size_t)n * n;
-    RG_TRY(ctx->scratch[10].ensure(sizeof(T2) * nn * ntowers));
-    T2 *subimg = ctx->scratch[10].as<T2>();
-    RG_TRY(zero_fill(subimg, sizeof(T2) * nn * ntowers, s));
-
-    const void *tw_inv = nullptr;
-    RG_TRY(get_twiddles(ctx, n, sizeof(T) == 4 ? RG_C64 : RG_C128, +1, &tw_inv));
-
-    // Batches of Q tasks; inside a batch each tower's run of tasks is cut into
-    // chunks of at most TPB tasks (one column-pass workgroup each).  By
-    // default one batch holds every task while its subgrids fit 8 GiB of HBM
-    // (fewer, fuller launches).
-    constexpr size_t BATCH_BYTES = (size_t)8 << 30;
-    const int q_fit = (int)std::max<size_t>(1, BATCH_BYTES / (sizeof(T2) * nn));
-    const int Q = plain ? 1 : (p->floor_batch > 0 ? p->floor_batch : std::min(ntasks, q_fit));
-    const int TPB = 32;
-    std::vector<int32_t> chunk_first, chunk_count, batch_chunk0, run_tower, run_chunk0, run_nchunk,
-        batch_run0;
-    for (int t0 = 0; t0 < ntasks; t0 += Q) {
-        const int t1 = std::min(ntasks, t0 + Q);
-        batch_chunk0.push_back((int32_t)chunk_first.size());
-        batch_run0.push_back((int32_t)run_tower.size());
-        for (int k = t0; k < t1;) {
-            int k1 = k;
-            while (k1 < t1 && tl.tower_rank[k1] == tl.tower_rank[k])
-                ++k1;
-            run_tower.push_back(tl.tower_rank[k]);
-            run_chunk0.push_back((int32_t)chunk_first.size() - batch_chunk0.back());
-            int nc = 0;
-            for (int c = k; c < k1; c += TPB, ++nc) {
-                chunk_first.push_back(c);
-                chunk_count.push_back(std::min(TPB, k1 - c));
-            }
-            run_nchunk.push_back(nc);
-            k = k1;
+    RG_TRY(ctx->scratch[10].ensure(sizeof(double2) * nn * ntowers));
+    double2 *subimg = ctx->scratch[10].as<double2>();
+    RG_TRY(zero_fill(subimg, sizeof(double2) * nn * ntowers, s));
+    RG_TRY(ctx->scratch[18].ensure(sizeof(int32_t) * cc * cc));
+    int32_t *d_rot = ctx->scratch[18].as<int32_t>();
+    RG_HIP(hipMemcpyAsync(d_rot, tl.rank_of_tower.data(), sizeof(int32_t) * cc * cc,
+                          hipMemcpyHostToDevice, s));
+    RG_TRY(ctx->scratch[7].ensure(sizeof(double2) * nn));
+    double2 *sub = ctx->scratch[7].as<double2>();
+    const Rec<double> *recs = (const Rec<double> *)br.recs;
+    std::vector<uint32_t> start_of(ntasks);
+    {
+        uint32_t run = 0;
+        int k2 = 0;
+        for (int64_t t = 0; t < nslots; ++t) {
+            if (k2 < ntasks && tl.slot[k2] == t)
+                start_of[k2++] = run;
+            run += h_nvis[t];
         }
     }
-    batch_chunk0.push_back((int32_t)chunk_first.size());
-    batch_run0.push_back((int32_t)run_tower.size());
-    int max_chunks = 0;
-    for (size_t b = 0; b + 1 < batch_chunk0.size(); ++b)
-        max_chunks = std::max(max_chunks, batch_chunk0[b + 1] - batch_chunk0[b]);
-    if (!plain) {
-        const unsigned long long nch = (unsigned long long)chunk_first.size();
-        RG_HIP(hipMemcpyAsync(ctx->counters.as<unsigned long long>() + CNT_CHUNKS, &nch,
-                              sizeof(nch), hipMemcpyHostToDevice, s));
-    }
-
-    std::vector<int32_t> tables;
-    const size_t o_rot = 0, o_slot = o_rot + (size_t)cc * cc, o_wp = o_slot + ntasks,
-                 o_cf = o_wp + ntasks, o_cc = o_cf + chunk_first.size(),
-                 o_rt = o_cc + chunk_count.size(), o_rc0 = o_rt + run_tower.size(),
-                 o_rnc = o_rc0 + run_chunk0.size(), o_ros = o_rnc + run_nchunk.size(),
-                 o_end = o_ros + (plain ? 0 : (size_t)nslots);
-    tables.resize(o_end);
-    if (!plain) {
-        std::fill(tables.begin() + o_ros, tables.end(), -1);
-        for (int k = 0; k < ntasks; ++k)
-            tables[o_ros + tl.slot[k]] = k;
-    }
-    std::copy(tl.rank_of_tower.begin(), tl.rank_of_tower.end(), tables.begin() + o_rot);
-    std::copy(tl.slot.begin(), tl.slot.end(), tables.begin() + o_slot);
-    std::copy(tl.wp.begin(), tl.wp.end(), tables.begin() + o_wp);
-    std::copy(chunk_first.begin(), chunk_first.end(), tables.begin() + o_cf);
-    std::copy(chunk_count.begin(), chunk_count.end(), tables.begin() + o_cc);
-    std::copy(run_tower.begin(), run_tower.end(), tables.begin() + o_rt);
-    std::copy(run_chunk0.begin(), run_chunk0.end(), tables.begin() + o_rc0);
-    std::copy(run_nchunk.begin(), run_nchunk.end(), tables.begin() + o_rnc);
-    RG_TRY(ctx->scratch[8].ensure(64 + sizeof(int32_t) * o_end));
-    int32_t *d_tab = reinterpret_cast<int32_t *>(ctx->scratch[8].as<char>() + 64);
-    RG_HIP(hipMemcpyAsync(d_tab, tables.data(), sizeof(int32_t) * o_end, hipMemcpyHostToDevice, s));
-
-    if (plain) {
-        // reference structure: one task at a time.  Keys are task slots here,
-        // so a task's records start at the running sum of the slot counts.
-        RG_TRY(ctx->scratch[7].ensure(sizeof(double2) * nn));
-        double2 *sub = ctx->scratch[7].as<double2>();
-        const Rec<double> *recs = (const Rec<double> *)br.recs;
-        std::vector<uint32_t> start_of(ntasks);
-        {
-            uint32_t run = 0;
-            int k2 = 0;
-            for (int64_t t = 0; t < nslots; ++t) {
-                if (k2 < ntasks && tl.slot[k2] == t)
-                    start_of[k2++] = run;
-                run += h_nvis[t];
-            }
-        }
-        const int blocks_px = (int)((nn + 255) / 256);
-        int k = 0;
-        for (int r = 0; r < ntowers; ++r) {
-            double2 *img = (double2 *)subimg + (size_t)r * nn;
-            int last_wp = 0;
-            bool first = true;
-            for (; k < ntasks && tl.tower_rank[k] == r; ++k) {
-                RG_TRY(zero_fill(sub, sizeof(double2) * nn, s));
-                hipLaunchKernelGGL(k_tower_plain_scatter, dim3(128), dim3(128), 0, s, recs,
-                                   start_of[k], tl.nvis[k], 128, (const double2 *)wk->data, S, sub,
-                                   n);
-                RG_HIP(hipGetLastError());
-                RG_TRY(fft2d(ctx, sub, RG_C128, n, 1, +1, s));
-                if (first) {
-                    last_wp = tl.wp[k];
-                    first = false;
-                }
-                hipLaunchKernelGGL(k_fresnel_mac, dim3(blocks_px), dim3(256), 0, s,
-                                   (const double2 *)sub, img, n, tl.wp[k] - last_wp, p->theta,
-                                   p->winc);
-                RG_HIP(hipGetLastError());
+    const int blocks_px = (int)((nn + 255) / 256);
+    int k = 0;
+    for (int r = 0; r < ntowers; ++r) {
+        double2 *img = subimg + (size_t)r * nn;
+        int last_wp = 0;
+        bool first = true;
+        for (; k < ntasks && tl.tower_rank[k] == r; ++k) {
+            RG_TRY(zero_fill(sub, sizeof(double2) * nn, s));
+            hipLaunchKernelGGL(k_tower_plain_scatter, dim3(128), dim3(128), 0, s, recs, start_of[k],
+                               tl.nvis[k], 128, (const double2 *)wk->data, S, sub, n);
+            RG_HIP(hipGetLastError());
+            RG_TRY(fft2d(ctx, sub, RG_C128, n, 1, +1, s));
+            if (first) {
                 last_wp = tl.wp[k];
+                first = false;
             }
-            if (last_wp != 0) {
-                hipLaunchKernelGGL(k_w0_transfer, dim3(blocks_px), dim3(256), 0, s, img, n, last_wp,
-                                   p->theta, p->winc);
-                RG_HIP(hipGetLastError());
-            }
-            RG_TRY(fft2d(ctx, img, RG_C128, n, 1, -1, s));
-        }
-    } else {
-        const int NB = n / H;
-        const int nbatch = (ntasks + Q - 1) / Q;
-        // per batch: bound on its work items (grid of the band launch) and
-        // spill slots (parts of split keys: at most 2 R / K for R records)
-        std::vector<uint32_t> blocks(nbatch);
-        uint64_t max_blocks = 1, max_spill = 1;
-        for (int bi = 0; bi < nbatch; ++bi) {
-            uint64_t nb = 0, rec = 0;
-            for (int k = bi * Q; k < std::min(ntasks, (bi + 1) * Q); ++k) {
-                const uint64_t wrec = (uint64_t)kf.mk * tl.nvis[k]; // window records
-                nb += std::min<uint64_t>((uint64_t)NB, wrec) + wrec / K;
-                rec += wrec;
-            }
-            blocks[bi] = (uint32_t)std::min<uint64_t>(nb + 1, 0x7fffffffu);
-            max_blocks = std::max<uint64_t>(max_blocks, nb + 1);
-            max_spill = std::max<uint64_t>(max_spill, 2 * rec / K + 1);
-        }
-        const size_t band_bytes = sizeof(T2) * (size_t)H * n;
-        const size_t g_bytes = sizeof(T2) * nn * Q;
-        const size_t p_bytes = sizeof(T2) * nn * (size_t)max_chunks;
-        const size_t sp_bytes = band_bytes * max_spill;
-        const size_t is_bytes = (sizeof(uint32_t) * max_blocks + 255) / 256 * 256;
-        const size_t ctr_bytes = (sizeof(uint32_t) * nbatch + 255) / 256 * 256;
-        RG_TRY(ctx->scratch[7].ensure(g_bytes + p_bytes + sp_bytes + is_bytes + ctr_bytes));
-        char *base7 = ctx->scratch[7].as<char>();
-        T2 *G = reinterpret_cast<T2 *>(base7);
-        T2 *Pp = reinterpret_cast<T2 *>(base7 + g_bytes);
-        T2 *spill = reinterpret_cast<T2 *>(base7 + g_bytes + p_bytes);
-        uint32_t *item_spill = reinterpret_cast<uint32_t *>(base7 + g_bytes + p_bytes + sp_bytes);
-        uint32_t *spill_ctr =
-            reinterpret_cast<uint32_t *>(base7 + g_bytes + p_bytes + sp_bytes + is_bytes);
-        RG_TRY(zero_fill(spill_ctr, sizeof(uint32_t) * nbatch, s));
-        // band window work items (item_start_w in the binning's item_start
-        // slot, window lengths and prefix in the count / cursor slots), then
-        // every item's records sorted by footprint column into the expanded
-        // array (window records laid out item after item)
-        const int64_t nexp = (int64_t)kf.mk * vis->n;
-        RG_TRY(ctx->scratch[11].ensure(sizeof(BandRec<T>) * (size_t)std::max<int64_t>(nexp, 1)));
-        BandRec<T> *sorted = ctx->scratch[11].as<BandRec<T>>();
-        uint32_t *item_start_w = br.item_start;
-        Item *items_w = br.items;
-        {
-            KernelTimer kt(ctx, RG_TIMER_BINNING, s);
-            uint32_t *len = ctx->scratch[0].as<uint32_t>();
-            uint32_t *prefix = ctx->scratch[3].as<uint32_t>();
-            hipLaunchKernelGGL(k_window_len, dim3((unsigned)((nkeys + 255) / 256)), dim3(256), 0, s,
-                               (const uint32_t *)br.vis_start, nkeys, NB, kf.mk, len, d_wstart,
-                               ctx->counters.as<unsigned long long>());
+            hipLaunchKernelGGL(k_fresnel_mac, dim3(blocks_px), dim3(256), 0, s,
+                               (const double2 *)sub, img, n, tl.wp[k] - last_wp, p->theta,
+                               p->winc);
             RG_HIP(hipGetLastError());
-            RG_TRY(items_from_lengths(ctx, len, nkeys, K, prefix, item_start_w, items_w, s));
-            hipLaunchKernelGGL((k_sort_items<T>),
-                               dim3((unsigned)std::min<int64_t>(max_items_w, 8192)), dim3(256), 0, s,
-                               (const Rec<T> *)br.recs, sorted, (const Item *)items_w,
-                               (const unsigned long long *)ctx->counters.p, (const uint32_t *)prefix,
-                               (const uint32_t *)d_wstart, n + S, S, H, NB);
+            last_wp = tl.wp[k];
+        }
+        if (last_wp != 0) {
+            hipLaunchKernelGGL(k_w0_transfer, dim3(blocks_px), dim3(256), 0, s, img, n, last_wp,
+                               p->theta, p->winc);
             RG_HIP(hipGetLastError());
         }
-        BandLaunch bl;
-        bl.recs = sorted;
-        bl.items = items_w;
-        bl.item_start = item_start_w;
-        bl.rank_of_slot = d_tab + o_ros;
-        {
-            // the FP32 band pass reads the table with zero rows after every
-            // sub-kernel; taps are addressed by 32-bit byte offsets
-            const int64_t nsub = (int64_t)wk->plane_count * wk->oversampling * wk->oversampling;
-            const int64_t rows = padded_taps<T>() ? (int64_t)H + nsub * padded_rows(S, H) : nsub * S;
-            const int64_t kb = rows * S * (int64_t)sizeof(T2);
-            if (kb >= ((int64_t)1 << 31))
-                return fail(RG_ERR_UNSUPPORTED, "batched W-towers kernel table too large (%lld bytes)",
-                            (long long)kb);
-            bl.kern = wk->data;
-            if (padded_taps<T>()) {
-                RG_TRY(ctx->scratch[12].ensure((size_t)kb));
-                T2 *padded = ctx->scratch[12].as<T2>();
-                hipLaunchKernelGGL(k_pad_kernels<T2>,
-                                   dim3((unsigned)std::min<int64_t>((rows * S + 255) / 256, 65536)),
-                                   dim3(256), 0, s, (const T2 *)wk->data, padded, nsub, S, H);
-                RG_HIP(hipGetLastError());
-                bl.kern = padded;
-            }
-            bl.kern_bytes = (uint32_t)kb;
-        }
-        bl.tw = tw_inv;
-        bl.G = G;
-        bl.spill = spill;
-        bl.spill_cap = (uint32_t)max_spill;
-        bl.item_spill = item_spill;
-        bl.stats = ctx->counters.as<unsigned long long>();
-        int b = 0;
-        for (int t0 = 0; t0 < ntasks; t0 += Q, ++b) {
-            const int t1 = std::min(ntasks, t0 + Q);
-            bl.key_begin = (uint32_t)((int64_t)tl.slot[t0] * NB);
-            bl.key_end = (uint32_t)((int64_t)(tl.slot[t1 - 1] + 1) * NB);
-            bl.t0 = t0;
-            bl.spill_ctr = spill_ctr + b;
-            bl.blocks = blocks[b];
-            {
-                KernelTimer kt(ctx, RG_TIMER_TOWER_TILES, s);
-                if (S == 16)
-                    RG_TRY((launch_bands<T, 16>(n, H, bl, s)));
-                else
-                    RG_TRY((launch_bands<T, 8>(n, H, bl, s)));
-            }
-            const int cb0 = batch_chunk0[b], ncb = batch_chunk0[b + 1] - batch_chunk0[b];
-            {
-                KernelTimer kt(ctx, RG_TIMER_TOWER_COLS, s);
-                RG_TRY(launch_cols<T>(n, H, G, item_start_w, d_tab + o_slot, Pp, d_tab + o_cf,
-                                      d_tab + o_cc, cb0, ncb, t0, d_tab + o_wp, tw_inv, p->theta,
-                                      p->winc, s));
-            }
-            const int rb = batch_run0[b], nr = batch_run0[b + 1] - batch_run0[b];
-            {
-                KernelTimer kt(ctx, RG_TIMER_TOWER_REDUCE, s);
-                int64_t gx = (int64_t)((nn + 255) / 256);
-                if (gx > 1024)
-                    gx = 1024;
-                hipLaunchKernelGGL((k_tower_reduce<T2>), dim3((unsigned)gx, nr), dim3(256), 0, s,
-                                   (const T2 *)Pp, subimg, (int64_t)nn,
-                                   (const int32_t *)(d_tab + o_rt), (const int32_t *)(d_tab + o_rc0),
-                                   (const int32_t *)(d_tab + o_rnc), rb);
-                RG_HIP(hipGetLastError());
-            }
-        }
+        RG_TRY(fft2d(ctx, img, RG_C128, n, 1, -1, s));
     }
-
     KernelTimer kt_finish(ctx, RG_TIMER_TOWER_FINISH, s);
-    if (!plain)
-        RG_TRY(fft2d(ctx, subimg, sizeof(T) == 4 ? RG_C64 : RG_C128, n, ntowers, -1, s));
     const int64_t npx = (int64_t)N * N;
-    hipLaunchKernelGGL((k_fold<T2>), dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, s,
-                       (double2 *)grid, N, (const T2 *)subimg, (const int32_t *)(d_tab + o_rot), n,
-                       M, chunk, cc);
+    hipLaunchKernelGGL((k_fold<double2>), dim3((unsigned)std::min<int64_t>((npx + 255) / 256, 65536)),
+                       dim3(256), 0, s, (double2 *)grid, N, npx, (const double2 *)subimg,
+                       (const int32_t *)d_rot, n, M, g.chunk, cc);
     RG_HIP(hipGetLastError());
     return RG_OK;
 }
 
+// ---- batched mode ----------------------------------------------------------------------------
+// Every table is built on the device.  The key space and the buffers are sized
+// by a plan kept in the context: the first call of a shape (or one that
+// outgrows its plan) reads the floor range and the task counts back and makes
+// the plan, with some headroom; later calls of that shape issue no read-back
+// until their end, where an eager call checks the overrun counter and redoes
+// itself with a fresh plan if a capacity was exceeded.  A call on a capturing
+// stream never reads back: it needs a plan from an eager call of its shape,
+// and an overrun is reported by rg_context_stats.
+
+static bool plan_fits(const WtPlan &pl, const rg_grid_params *p, const rg_flat_vis *vis,
+                      const rg_wkern *wk, int nchan, int H, int type)
+{
+    return pl.valid && pl.N == p->grid_size && pl.n == p->subgrid_size &&
+           pl.M == p->subgrid_margin && pl.S == wk->size && pl.R == wk->oversampling &&
+           pl.P == wk->plane_count && pl.H == H && pl.type == type && pl.nchan == nchan &&
+           pl.K == (p->vis_per_item > 0 ? p->vis_per_item : 2048) &&
+           pl.floor_batch == p->floor_batch && pl.theta == p->theta && pl.winc == p->winc &&
+           pl.w_min == wk->w_min && pl.w_step == wk->w_step && vis->n <= pl.nvis;
+}
+
+template <typename T>
+static int wtowers_batched_t(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis,
+                             const int64_t *d_off, int nchan, const rg_wkern *wk, void *grid,
+                             hipStream_t s, bool capturing, bool measure)
+{
+    using T2 = typename Vec2<T>::type;
+    const int H = band_height_of(p, sizeof(T) == 4);
+    const WtGeom g = wt_geom(p, wk, nchan, H);
+    const int n = g.n, S = g.S, NB = g.NB;
+    const size_t nn = (size_t)n * n;
+    const int type = sizeof(T) == 4 ? RG_C64 : RG_C128;
+    const uint32_t K = p->vis_per_item > 0 ? (uint32_t)p->vis_per_item : 2048u;
+    WtPlan &pl = ctx->wt_plan;
+    auto *stats = ctx->counters.as<unsigned long long>();
+
+    RG_TRY(zero_fill(grid, sizeof(double2) * (size_t)g.N * g.N * nchan, s));
+    if (vis->n == 0)
+        return RG_OK;
+
+    // floor range (device); the plan fixes the floors per tower slot
+    RG_TRY(ctx->scratch[8].ensure(256));
+    int *d_range = ctx->scratch[8].as<int>();
+    RG_TRY(floor_range(p, vis, g, d_range, s));
+    int nwp_cap;
+    if (measure) {
+        int wp_min = 0, nwp = 0;
+        bool empty = false;
+        RG_TRY(read_range(ctx, d_range, wp_min, nwp, empty, s));
+        if (empty) {
+            unsigned long long cnt[CNT__N] = {0};
+            cnt[CNT_DROPPED] = (unsigned long long)vis->n;
+            RG_HIP(hipMemcpyAsync(ctx->counters.p, cnt, sizeof(cnt), hipMemcpyHostToDevice, s));
+            RG_HIP(hipStreamSynchronize(s));
+            return RG_OK;
+        }
+        nwp_cap = nwp + nwp / 8 + 4;
+    } else {
+        nwp_cap = pl.nwp_cap;
+    }
+    const int64_t nslots = g.ntw * nwp_cap;
+    const int64_t nkeys = nslots * NB;
+    // band records are addressed by 32-bit offsets: the expanded array holds
+    // up to mk records per visibility
+    if ((int64_t)g.mk * vis->n >= ((int64_t)1 << 32))
+        return fail(RG_ERR_PARAM,
+                    "W-towers call too large: %lld visibilities x %d bands per footprint exceed "
+                    "2^32 band records; split the stream",
+                    (long long)vis->n, g.mk);
+    if (nkeys >= ((int64_t)1 << 31))
+        return fail(RG_ERR_PARAM, "W-towers key space too large (%lld keys: %lld towers x %d "
+                    "floors x %d bands)", (long long)nkeys, (long long)g.ntw, nwp_cap, NB);
+    // the sort makes one item per key; band work items of at most K records
+    // per band window are built after it
+    const int64_t max_items_w = nkeys + (int64_t)g.mk * vis->n / K + 1;
+    RG_TRY(ctx->scratch[4].ensure(sizeof(Item) * std::max<int64_t>(nkeys + 1, max_items_w)));
+    const TowerKeyer kf = wt_keyer(p, vis, wk, g, d_range, nwp_cap, d_off);
+    BinResult br;
+    // (task, band) keys: 4 counter replicas at the reference's workload (the
+    // scans read every replica; 16 were slower, 1 << 20 counters in all)
+    RG_TRY((bin_visibilities<TowerKeyer, T>(ctx, kf, vis->n, nkeys, 0x40000000u, s, br,
+                                            (int64_t)1 << 20)));
+
+    // capacities: tables by the slot count, buffers by the plan (or, measuring,
+    // by the counts read back below)
+    const int ntw = (int)g.ntw;
+    const int64_t nchunks_bound = nslots / TASKS_PER_CHUNK + g.ntw;
+    int ntasks_cap, nchunks_cap, ntowers_cap, nitems_cap;
+    if (measure) {
+        ntasks_cap = (int)std::min<int64_t>(nslots, 0x7fffffff);
+        nchunks_cap = (int)std::min<int64_t>(nchunks_bound, 0x7fffffff);
+        ntowers_cap = ntw;
+        nitems_cap = (int)std::min<int64_t>(max_items_w, 0x7fffffff);
+    } else {
+        ntasks_cap = pl.ntasks_cap;
+        nchunks_cap = pl.nchunks_cap;
+        ntowers_cap = pl.ntowers_cap;
+        nitems_cap = pl.nitems_cap;
+    }
+    // table layout (int32): 7 per-tower arrays, 3 per-task, 2 per-chunk, one
+    // per slot, totals
+    const size_t o_tn = 0, o_to = o_tn + ntw, o_co = o_to + ntw, o_rot = o_co + ntw,
+                 o_tor = o_rot + ntw, o_rc0 = o_tor + ntw, o_rnc = o_rc0 + ntw,
+                 o_ts = o_rnc + ntw, o_twp = o_ts + ntasks_cap, o_trk = o_twp + ntasks_cap,
+                 o_cf = o_trk + ntasks_cap, o_cn = o_cf + nchunks_cap, o_ros = o_cn + nchunks_cap,
+                 o_tot = o_ros + nslots, o_end = o_tot + 16;
+    RG_TRY(ctx->scratch[18].ensure(sizeof(int32_t) * o_end));
+    int32_t *tab = ctx->scratch[18].as<int32_t>();
+    TaskTables tt;
+    tt.tower_ntasks = tab + o_tn;
+    tt.tower_task_off = tab + o_to;
+    tt.tower_chunk_off = tab + o_co;
+    tt.rank_of_tower = tab + o_rot;
+    tt.tower_of_rank = tab + o_tor;
+    tt.rank_chunk0 = tab + o_rc0;
+    tt.rank_nchunk = tab + o_rnc;
+    tt.task_slot = tab + o_ts;
+    tt.task_wp = tab + o_twp;
+    tt.task_rank = tab + o_trk;
+    tt.chunk_first = tab + o_cf;
+    tt.chunk_count = tab + o_cn;
+    tt.rank_of_slot = tab + o_ros;
+    tt.totals = tab + o_tot;
+    tt.ntw = ntw;
+    tt.nwp = nwp_cap;
+    tt.nk = NB;
+    tt.ntasks_cap = ntasks_cap;
+    tt.nchunks_cap = nchunks_cap;
+    tt.ntowers_cap = ntowers_cap;
+    hipLaunchKernelGGL(k_tower_count, dim3(ntw), dim3(256), 0, s, (const uint32_t *)br.vis_start,
+                       tt);
+    hipLaunchKernelGGL(k_tower_scan, dim3(1), dim3(1024), 0, s, tt, stats);
+    hipLaunchKernelGGL(k_tower_tasks, dim3(ntw), dim3(256), 0, s, (const uint32_t *)br.vis_start,
+                       tt, (const int *)d_range);
+    RG_HIP(hipGetLastError());
+
+    // band window work items (in the binning's item_start slot; window lengths
+    // and prefix in the count / cursor slots)
+    RG_TRY(ctx->scratch[9].ensure(sizeof(uint32_t) * (nkeys + 1)));
+    uint32_t *d_wstart = ctx->scratch[9].as<uint32_t>();
+    uint32_t *item_start_w = br.item_start;
+    Item *items_w = br.items;
+    uint32_t *prefix = ctx->scratch[3].as<uint32_t>();
+    {
+        KernelTimer kt(ctx, RG_TIMER_BINNING, s);
+        uint32_t *len = ctx->scratch[0].as<uint32_t>();
+        hipLaunchKernelGGL(k_window_len, dim3((unsigned)((nkeys + 255) / 256)), dim3(256), 0, s,
+                           (const uint32_t *)br.vis_start, nkeys, NB, g.mk, len, d_wstart, stats);
+        RG_HIP(hipGetLastError());
+        RG_TRY(items_from_lengths(ctx, len, nkeys, K, prefix, item_start_w, items_w, s));
+    }
+
+    if (measure) {
+        // the plan: counts of this call plus headroom for calls of the same shape
+        unsigned long long c[CNT__N];
+        RG_HIP(hipMemcpyAsync(c, ctx->counters.p, sizeof(c), hipMemcpyDeviceToHost, s));
+        RG_HIP(hipStreamSynchronize(s));
+        const int64_t nt = (int64_t)c[CNT_TASKS], ntow = (int64_t)c[CNT_TOWERS],
+                      nch = (int64_t)c[CNT_CHUNKS], nit = (int64_t)c[CNT_ITEMS];
+        pl.valid = false;
+        pl.N = p->grid_size;
+        pl.n = n;
+        pl.M = g.M;
+        pl.S = S;
+        pl.R = wk->oversampling;
+        pl.P = wk->plane_count;
+        pl.H = H;
+        pl.type = type;
+        pl.nchan = nchan;
+        pl.K = K;
+        pl.floor_batch = p->floor_batch;
+        pl.theta = p->theta;
+        pl.winc = p->winc;
+        pl.w_min = wk->w_min;
+        pl.w_step = wk->w_step;
+        pl.nvis = vis->n;
+        pl.nwp_cap = nwp_cap;
+        pl.ntasks_cap = (int)std::min<int64_t>(ntasks_cap, nt + nt / 8 + 16);
+        pl.nchunks_cap = (int)std::min<int64_t>(nchunks_cap, nch + nch / 8 + 16);
+        pl.ntowers_cap = (int)std::min<int64_t>(ntw, ntow + ntow / 8 + 4);
+        pl.nitems_cap = (int)std::min<int64_t>(max_items_w, nit + nit / 8 + 64);
+        if (nt == 0)
+            return RG_OK;
+        // this call's buffers use the plan's capacities (the tables above were
+        // built with the slot-count bounds, which hold every index)
+        ntasks_cap = pl.ntasks_cap;
+        nchunks_cap = pl.nchunks_cap;
+        ntowers_cap = pl.ntowers_cap;
+        nitems_cap = pl.nitems_cap;
+        pl.valid = true;
+    }
+
+    // subimages (one per active tower, by rank)
+    RG_TRY(ctx->scratch[10].ensure(sizeof(T2) * nn * ntowers_cap));
+    T2 *subimg = ctx->scratch[10].as<T2>();
+    RG_TRY(zero_fill(subimg, sizeof(T2) * nn * ntowers_cap, s));
+    const void *tw_inv = nullptr;
+    RG_TRY(get_twiddles(ctx, n, type, +1, &tw_inv));
+
+    // batches of Q tasks (by default every task while the subgrids fit 16 GiB)
+    constexpr size_t BATCH_BYTES = (size_t)16 << 30;
+    const int q_fit = (int)std::max<size_t>(1, BATCH_BYTES / (sizeof(T2) * nn));
+    const int Q = std::max(1, std::min(ntasks_cap, p->floor_batch > 0 ? p->floor_batch : q_fit));
+    const int nbatch = (ntasks_cap + Q - 1) / Q;
+    // spill slots (parts of split keys: at most 2 R / K for R records), per batch
+    const uint64_t max_spill = 2 * ((uint64_t)g.mk * vis->n) / K + 1;
+    const size_t band_bytes = sizeof(T2) * (size_t)H * n;
+    const size_t g_bytes = sizeof(T2) * nn * Q;
+    const size_t p_bytes = sizeof(T2) * nn * (size_t)nchunks_cap;
+    const size_t sp_bytes = band_bytes * max_spill;
+    const size_t is_bytes = (sizeof(uint32_t) * (size_t)max_items_w + 255) / 256 * 256;
+    const size_t ctr_bytes = (sizeof(uint32_t) * nbatch + 255) / 256 * 256;
+    RG_TRY(ctx->scratch[7].ensure(g_bytes + p_bytes + sp_bytes + is_bytes + ctr_bytes));
+    char *base7 = ctx->scratch[7].as<char>();
+    T2 *G = reinterpret_cast<T2 *>(base7);
+    T2 *Pp = reinterpret_cast<T2 *>(base7 + g_bytes);
+    T2 *spill = reinterpret_cast<T2 *>(base7 + g_bytes + p_bytes);
+    uint32_t *item_spill = reinterpret_cast<uint32_t *>(base7 + g_bytes + p_bytes + sp_bytes);
+    uint32_t *spill_ctr =
+        reinterpret_cast<uint32_t *>(base7 + g_bytes + p_bytes + sp_bytes + is_bytes);
+    RG_TRY(zero_fill(spill_ctr, sizeof(uint32_t) * nbatch, s));
+
+    // every work item's records sorted by footprint column into the expanded
+    // array (window records laid out item after item)
+    const int64_t nexp = (int64_t)g.mk * vis->n;
+    RG_TRY(ctx->scratch[11].ensure(sizeof(BandRec<T>) * (size_t)std::max<int64_t>(nexp, 1)));
+    BandRec<T> *sorted = ctx->scratch[11].as<BandRec<T>>();
+    {
+        KernelTimer kt(ctx, RG_TIMER_BINNING, s);
+        hipLaunchKernelGGL((k_sort_items<T>),
+                           dim3((unsigned)std::min<int64_t>(max_items_w, 8192)), dim3(256), 0, s,
+                           (const Rec<T> *)br.recs, sorted, (const Item *)items_w, stats,
+                           (const uint32_t *)prefix, (const uint32_t *)d_wstart, n + S, S, H, NB);
+        RG_HIP(hipGetLastError());
+    }
+
+    BandLaunch bl;
+    bl.recs = sorted;
+    bl.items = items_w;
+    bl.item_start = item_start_w;
+    bl.bref.task_slot = tt.task_slot;
+    bl.bref.totals = tt.totals;
+    bl.bref.Q = Q;
+    bl.rank_of_slot = tt.rank_of_slot;
+    {
+        // the FP32 band pass reads the table with zero rows after every
+        // sub-kernel; taps are addressed by 32-bit byte offsets
+        const int64_t nsub = (int64_t)wk->plane_count * wk->oversampling * wk->oversampling;
+        const int64_t rows = padded_taps<T>() ? (int64_t)H + nsub * padded_rows(S, H) : nsub * S;
+        const int64_t kb = rows * S * (int64_t)sizeof(T2);
+        if (kb >= ((int64_t)1 << 31))
+            return fail(RG_ERR_UNSUPPORTED, "batched W-towers kernel table too large (%lld bytes)",
+                        (long long)kb);
+        bl.kern = wk->data;
+        if (padded_taps<T>()) {
+            RG_TRY(ctx->scratch[12].ensure((size_t)kb));
+            T2 *padded = ctx->scratch[12].as<T2>();
+            hipLaunchKernelGGL(k_pad_kernels<T2>,
+                               dim3((unsigned)std::min<int64_t>((rows * S + 255) / 256, 65536)),
+                               dim3(256), 0, s, (const T2 *)wk->data, padded, nsub, S, H);
+            RG_HIP(hipGetLastError());
+            bl.kern = padded;
+        }
+        bl.kern_bytes = (uint32_t)kb;
+    }
+    bl.tw = tw_inv;
+    bl.G = G;
+    bl.spill = spill;
+    bl.spill_cap = (uint32_t)std::min<uint64_t>(max_spill, 0xffffffffu);
+    bl.item_spill = item_spill;
+    bl.stats = stats;
+    // one band-pass workgroup per work item (the plan's item capacity; a
+    // batch with more items reports an overrun)
+    bl.blocks = (uint32_t)std::max(1, nitems_cap);
+    ColsLaunch cl;
+    cl.G = G;
+    cl.key_start = item_start_w;
+    cl.bref = bl.bref;
+    cl.task_wp = tt.task_wp;
+    cl.chunk_first = tt.chunk_first;
+    cl.chunk_count = tt.chunk_count;
+    cl.P = Pp;
+    cl.tw = tw_inv;
+    cl.theta = p->theta;
+    cl.winc = p->winc;
+    cl.grid_y = (unsigned)std::min<int64_t>(std::max(nchunks_cap, 1), 65535);
+    int64_t rgx = (int64_t)((nn + 255) / 256);
+    if (rgx > 1024)
+        rgx = 1024;
+    const unsigned rgy = (unsigned)std::min<int64_t>(std::max(ntowers_cap, 1), 65535);
+    for (int b = 0; b < nbatch; ++b) {
+        bl.batch = b;
+        bl.spill_ctr = spill_ctr + b;
+        cl.batch = b;
+        {
+            KernelTimer kt(ctx, RG_TIMER_TOWER_TILES, s);
+            if (S == 16)
+                RG_TRY((launch_bands<T, 16>(n, H, bl, s)));
+            else
+                RG_TRY((launch_bands<T, 8>(n, H, bl, s)));
+        }
+        {
+            KernelTimer kt(ctx, RG_TIMER_TOWER_COLS, s);
+            RG_TRY(launch_cols<T>(n, H, cl, s));
+        }
+        {
+            KernelTimer kt(ctx, RG_TIMER_TOWER_REDUCE, s);
+            hipLaunchKernelGGL((k_tower_reduce<T2>), dim3((unsigned)rgx, rgy), dim3(256), 0, s,
+                               (const T2 *)Pp, subimg, (int64_t)nn, bl.bref, b,
+                               (const int32_t *)tt.rank_chunk0, (const int32_t *)tt.rank_nchunk,
+                               (const int32_t *)tt.chunk_first, (const int32_t *)tt.chunk_count);
+            RG_HIP(hipGetLastError());
+        }
+    }
+
+    KernelTimer kt_finish(ctx, RG_TIMER_TOWER_FINISH, s);
+    RG_TRY(fft2d(ctx, subimg, type, n, ntowers_cap, -1, s, nullptr, tt.totals + 1));
+    const int64_t npx = (int64_t)g.N * g.N * nchan;
+    hipLaunchKernelGGL((k_fold<T2>), dim3((unsigned)std::min<int64_t>((npx + 255) / 256, 1 << 20)),
+                       dim3(256), 0, s, (double2 *)grid, g.N, npx, (const T2 *)subimg,
+                       (const int32_t *)tt.rank_of_tower, n, g.M, g.chunk, g.cc);
+    RG_HIP(hipGetLastError());
+    (void)capturing;
+    return RG_OK;
+}
+
+template <typename T>
+static int wtowers_batched(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis,
+                           const int64_t *d_off, int nchan, const rg_wkern *wk, void *grid,
+                           hipStream_t s, bool capturing)
+{
+    const int H = band_height_of(p, sizeof(T) == 4);
+    const int type = sizeof(T) == 4 ? RG_C64 : RG_C128;
+    const bool fits = plan_fits(ctx->wt_plan, p, vis, wk, nchan, H, type);
+    if (capturing) {
+        if (!fits)
+            return fail(RG_ERR_PARAM,
+                        "a W-towers call on a capturing stream needs an earlier eager call of the "
+                        "same shape (grid, subgrid, kernels, parameters, no more visibilities) on "
+                        "this context");
+        return wtowers_batched_t<T>(ctx, p, vis, d_off, nchan, wk, grid, s, true, false);
+    }
+    if (fits) {
+        RG_TRY((wtowers_batched_t<T>(ctx, p, vis, d_off, nchan, wk, grid, s, false, false)));
+        unsigned long long over = 0;
+        RG_HIP(hipMemcpyAsync(&over, ctx->counters.as<unsigned long long>() + CNT_OVERFLOW,
+                              sizeof(over), hipMemcpyDeviceToHost, s));
+        RG_HIP(hipStreamSynchronize(s));
+        if (over == 0)
+            return RG_OK;
+        // a capacity of the plan was too small for this call: measure again
+        RG_TRY(zero_fill(ctx->counters.p, sizeof(unsigned long long) * CNT__N, s));
+    }
+    ctx->wt_plan.valid = false;
+    return wtowers_batched_t<T>(ctx, p, vis, d_off, nchan, wk, grid, s, false, true);
+}
+
 int wtowers_run(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis,
-                const rg_wkern *wk, void *grid, int grid_type, hipStream_t s)
+                const int64_t *d_off, int nchan, const rg_wkern *wk, void *grid, int grid_type,
+                hipStream_t s, bool capturing)
 {
     if (grid_type != RG_C128)
         return fail(RG_ERR_PARAM, "W-towers folds into an FP64 (RG_C128) master grid");
+    const int n = p->subgrid_size, S = wk->size;
+    if (n + 2 * S >= 65536)
+        return fail(RG_ERR_UNSUPPORTED, "subgrid too large");
     if (p->mode == RG_MODE_PLAIN) {
         if (vis->vis_type != RG_C128 || wk->type != RG_C128)
             return fail(RG_ERR_PARAM, "plain mode is FP64: visibilities and kernels must be RG_C128");
-        return wtowers_t<double>(ctx, p, vis, wk, grid, s);
+        if (nchan != 1)
+            return fail(RG_ERR_UNSUPPORTED, "plain mode grids one channel per call");
+        if (capturing)
+            return fail(RG_ERR_UNSUPPORTED,
+                        "plain W-towers (the reference's per-task launches with host read-backs) "
+                        "cannot be captured");
+        return wtowers_plain(ctx, p, vis, wk, grid, s);
     }
+    if (S != 8 && S != 16)
+        return fail(RG_ERR_UNSUPPORTED, "batched W-towers supports kernel sizes 8 and 16");
+    if (n < 64 || n > 512)
+        return fail(RG_ERR_UNSUPPORTED, "batched W-towers supports subgrid sizes 64..512 (got %d)", n);
+    static_assert(SORT_BINS >= 512 + 16, "sort bins must cover n + S");
     if (wk->type == RG_C64)
-        return wtowers_t<float>(ctx, p, vis, wk, grid, s);
-    return wtowers_t<double>(ctx, p, vis, wk, grid, s);
+        return wtowers_batched<float>(ctx, p, vis, d_off, nchan, wk, grid, s, capturing);
+    return wtowers_batched<double>(ctx, p, vis, d_off, nchan, wk, grid, s, capturing);
 }
 
 } // namespace rg

@@ -89,6 +89,7 @@ SIGNATURES = [
     ("rg_context_create", C.c_int, [C.POINTER(_P), C.c_int]),
     ("rg_context_destroy", C.c_int, [_P]),
     ("rg_context_stats", C.c_int, [_P, C.POINTER(GridStats)]),
+    ("rg_context_thaw", C.c_int, [_P]),
     ("rg_context_set_timing", C.c_int, [_P, _I]),
     ("rg_context_timer", C.c_int, [_P, _I, C.POINTER(C.c_double), C.POINTER(C.c_int64)]),
     ("rg_grid", C.c_int, [_P, C.POINTER(GridParams), C.POINTER(FlatVis), C.POINTER(WKern), _P, _I, _P]),
@@ -251,6 +252,10 @@ class Context:
         check(lib().rg_grid_channels(self.h, C.byref(p), C.byref(fv),
                                      off.ctypes.data_as(C.POINTER(C.c_int64)), len(off) - 1,
                                      C.byref(wk), _ptr(grids), grid_type_of(grids), _stream(stream)))
+
+    def thaw(self):
+        """rg_context_thaw: no graph captured from this context is replayed again."""
+        check(lib().rg_context_thaw(self.h))
 
     def set_timing(self, enable: bool):
         check(lib().rg_context_set_timing(self.h, 1 if enable else 0))

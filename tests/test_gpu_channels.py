@@ -95,6 +95,7 @@ def test_graph_replay_matches_eager(ctx, case):
         assert rel_max(g.cpu().numpy(), ref) < 1e-12, rep
         assert rel_max(g.cpu().numpy()[0], refs[0]) < 1e-12, rep
     del graph
+    ctx.thaw()  # the shared context may grow its scratch memory again
 
 
 def test_autotune_picks_a_candidate(ctx, case):
