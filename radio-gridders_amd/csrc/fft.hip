@@ -188,6 +188,51 @@ int get_twiddles(rg_context *ctx, int n, int type, int sign, const void **out)
     return RG_OK;
 }
 
+// Stage table of the Stockham sequence used by fft_band_rows (radix 8
+// stages, then one radix-4 or radix-2 stage): entry Ns - 1 + (r - 1) Ns + k =
+// tw[r k n / (Ns R)], the same values the plain table gives.
+int get_stage_twiddles(rg_context *ctx, int n, int type, int sign, const void **out)
+{
+    TwKey key(n, type, 2 * sign);
+    auto it = ctx->twiddles.find(key);
+    if (it != ctx->twiddles.end()) {
+        *out = it->second.p;
+        return RG_OK;
+    }
+    int logn = 0;
+    while ((1 << logn) < n)
+        ++logn;
+    std::vector<double> hd(2 * (size_t)n, 0.0);
+    for (int logns = 0; logns < logn;) {
+        const int rem = logn - logns;
+        const int R = rem >= 3 ? 8 : (rem == 2 ? 4 : 2);
+        const int Ns = 1 << logns;
+        for (int r = 1; r < R; ++r)
+            for (int k = 0; k < Ns; ++k) {
+                const int m = (int)(((int64_t)r * k * (n / (Ns * R))) % n);
+                const double a = 2.0 * M_PI * (double)m / (double)n;
+                const size_t e = (size_t)(Ns - 1 + (r - 1) * Ns + k);
+                hd[2 * e] = cos(a);
+                hd[2 * e + 1] = sign * sin(a);
+            }
+        logns += R == 8 ? 3 : (R == 4 ? 2 : 1);
+    }
+    DevBuf buf;
+    const size_t esz = type == RG_C128 ? 16 : 8;
+    RG_TRY(buf.ensure(esz * n));
+    if (type == RG_C128) {
+        RG_HIP(hipMemcpy(buf.p, hd.data(), esz * n, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> hf(2 * (size_t)n);
+        for (size_t i = 0; i < hf.size(); ++i)
+            hf[i] = (float)hd[i];
+        RG_HIP(hipMemcpy(buf.p, hf.data(), esz * n, hipMemcpyHostToDevice));
+    }
+    *out = buf.p;
+    ctx->twiddles.emplace(key, buf);
+    return RG_OK;
+}
+
 // ---- dispatch ----------------------------------------------------------------------
 
 template <typename T2, int LOGN, bool INV>

@@ -389,4 +389,94 @@ __device__ inline void fft_block(T2 *lds, const T2 *tw)
     }
 }
 
+// ---- padded band rows ---------------------------------------------------------------
+// Rows of a band stored with one pad element after every 16: logical index x
+// at x + (x >> 4).  Row l starts at the (16-aligned) logical index rowbase_l,
+// so its element e lies at pad(rowbase_l) + e + (e >> 4).  With the pads the
+// strided stores of the first Stockham stages (lanes 8 or 64 elements apart)
+// spread over all LDS banks.  Twiddles come from a stage table: stage Ns,
+// radix R holds w^(r k n / (Ns R)) for r = 1..R-1, k < Ns at
+// tws[Ns - 1 + (r - 1) Ns + k], so the lanes of a wavefront read consecutive
+// words (a plain table read at r k conflicts for even r).
+__host__ __device__ constexpr int pad16(int x) { return x + (x >> 4); }
+
+template <typename T2, int LOGN, int R, bool INV>
+__device__ inline void fft_stage_band(T2 *row, const T2 *tws, int logns, int lt, bool active)
+{
+    constexpr int n = 1 << LOGN;
+    constexpr int TPL = n / 8;
+    constexpr int NBUT = n / R;
+    constexpr int PER = NBUT / TPL;
+    const int Ns = 1 << logns;
+    T2 v[8];
+    if (active) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int j = lt + q * TPL;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                v[q * R + r] = row[pad16(j + r * NBUT)];
+        }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int q = 0; q < PER; ++q) {
+            const int j = lt + q * TPL;
+            const int k = j & (Ns - 1);
+            if (logns > 0) { // the first stage's twiddles are all 1
+#pragma unroll
+                for (int r = 1; r < R; ++r)
+                    v[q * R + r] = cmul(v[q * R + r], tws[Ns - 1 + (r - 1) * Ns + k]);
+            }
+            if (R == 8)
+                dft8<INV>(&v[q * R]);
+            else if (R == 4)
+                dft4<INV>(v[q * R], v[q * R + 1], v[q * R + 2], v[q * R + 3]);
+            else
+                dft2<INV>(v[q * R], v[q * R + 1]);
+            const int d = (j >> logns) * (Ns * R) + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                row[pad16(d + r * Ns)] = v[q * R + r];
+        }
+    }
+    __syncthreads();
+}
+
+// Transforms rows 0..nl-1 of a padded band with NT threads; row l begins at
+// band + pad16(LEAD + l LS) (LEAD and LS multiples of 16).
+template <typename T2, int LOGN, int NT, int LEAD, int LS, bool INV>
+__device__ inline void fft_band_rows(T2 *band, const T2 *tws, int nl)
+{
+    static_assert(LEAD % 16 == 0 && LS % 16 == 0, "rows start on pad boundaries");
+    constexpr int n = 1 << LOGN;
+    constexpr int TPL = n / 8;
+    static_assert(TPL <= NT, "line longer than 8 x threads");
+    constexpr int B = NT / TPL;
+    const int lt = threadIdx.x % TPL;
+    for (int l0 = 0; l0 < nl; l0 += B) {
+        const int l = l0 + (int)threadIdx.x / TPL;
+        const bool active = (int)threadIdx.x < B * TPL && l < nl;
+        T2 *row = band + pad16(LEAD + (active ? l : 0) * LS);
+        int logns = 0;
+#pragma unroll
+        for (int st = 0; st < 8; ++st) {
+            if (logns >= LOGN)
+                break;
+            const int rem = LOGN - logns;
+            if (rem >= 3) {
+                fft_stage_band<T2, LOGN, 8, INV>(row, tws, logns, lt, active);
+                logns += 3;
+            } else if (rem == 2) {
+                fft_stage_band<T2, LOGN, 4, INV>(row, tws, logns, lt, active);
+                logns += 2;
+            } else {
+                fft_stage_band<T2, LOGN, 2, INV>(row, tws, logns, lt, active);
+                logns += 1;
+            }
+        }
+    }
+}
+
 } // namespace rg

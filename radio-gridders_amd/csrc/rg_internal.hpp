@@ -186,6 +186,8 @@ struct KernelTimer {
 };
 // twiddle table exp(sign * 2 pi i k / n), k < n, of `type`
 int get_twiddles(rg_context *ctx, int n, int type, int sign, const void **out);
+// the same twiddles in the stage layout of fft_band_rows (fft_core.hpp)
+int get_stage_twiddles(rg_context *ctx, int n, int type, int sign, const void **out);
 
 // internal entry points shared between translation units
 int wproj_plain(rg_context *ctx, const rg_grid_params *p, const rg_flat_vis *vis,

@@ -385,9 +385,9 @@ __device__ inline size_t gcb_index(int row, int col, int n)
 // ---- band records ------------------------------------------------------------------------
 // A visibility as band b of its task sees it, with everything a band lane
 // needs precomputed (z = sizeof(T2), the element size):
-//   xy = gz | cy0 << 16, gz = (gx + S) z the footprint's first column in the
-//        band's row layout (S elements of left pad), cy0 = H b - gy (signed)
-//        the footprint row that band row 0 meets;
+//   xy = gz | cy0 << 16, gz = (gx + BAND_LEAD) z the footprint's first column
+//        in the band's logical row layout, cy0 = H b - gy (signed) the
+//        footprint row that band row 0 meets;
 //   kb, byte offset of that footprint row's first tap:
 //   - FP32 (padded table: H zero rows, then every S x S sub-kernel followed
 //     by H zero rows, PS = S + H): kb = ((H + kidx PS + min(cy0, S)) S) z;
@@ -411,13 +411,19 @@ static_assert(sizeof(BandRec<float>) == sizeof(Rec<float>) &&
 
 __host__ __device__ constexpr int padded_rows(int S, int H) { return S + H; }
 
+// Band rows in LDS: logical index BAND_LEAD + l LS + c for row l, column c
+// (LS = n + BAND_LEAD), so the BAND_LEAD >= S columns left of a row and
+// right of it fall in gaps between rows; stored with one pad element per 16
+// (pad16, fft_core.hpp).
+constexpr int BAND_LEAD = 16;
+
 template <typename T>
 __device__ inline BandRec<T> band_record(const Rec<T> &r, int S, int H, int Hb, int z)
 {
     const int gx = (int)(r.pos & 0xffffu) - S, gy = (int)(r.pos >> 16) - S;
     const int cy0 = Hb - gy; // >= 1 - H for every record of band Hb / H's window
     BandRec<T> o;
-    o.xy = (uint32_t)((gx + S) * z) | ((uint32_t)cy0 << 16);
+    o.xy = (uint32_t)((gx + BAND_LEAD) * z) | ((uint32_t)cy0 << 16);
     if constexpr (padded_taps<T>())
         o.kb = ((H + (int32_t)r.kidx * padded_rows(S, H) + min(cy0, S)) * S) * z;
     else
@@ -592,8 +598,7 @@ __global__ __launch_bounds__(256) void k_window_len(const uint32_t *__restrict__
 // (myU, myV) owns band row myV and the columns congruent to myU modulo S: a
 // footprint covering that row has exactly one cell there.  The lane keeps the
 // register sum of its current cell and adds it into the LDS band when the
-// cell changes (LDS float atomics when GRP > 1, since lanes of different
-// groups own the same cells).  Band rows lie BS = n + S + 8 elements apart
+// cell changes (lanes of different groups own the same cells: see flush).  Band rows lie BS = n + S + 8 elements apart
 // after a lead of S elements, so a footprint column outside [0, n) lands in
 // the gap between rows and is dropped without a test.  The band's rows are
 // then inverse FFT'd in LDS.  A key of one item stores the band straight into
@@ -620,14 +625,17 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
     constexpr int RS = 128;
     constexpr int RG = RS / GRP;
     static_assert(RS % GRP == 0 && RS <= NT, "record staging");
-    // row l of the band at band[S + l BS .. S + l BS + n); the S - 1 columns
-    // either side of a row fall in the gaps.  The extra 8 elements spread the
-    // column-block copy-out (8 rows at once) over the LDS banks.
-    constexpr int BS = n + S + 8;
-    constexpr int BANDE = S + H * BS;
+    // row l of the band at logical [BAND_LEAD + l LS, + n) (pad16 in LDS); the
+    // S - 1 columns either side of a row fall in the gaps
+    static_assert(BAND_LEAD >= S, "gaps hold the footprint columns outside a row");
+    constexpr int LS = n + BAND_LEAD;
+    constexpr int BANDE = pad16(BAND_LEAD + H * LS);
     __shared__ T2 band[BANDE];
-    __shared__ T2 tw[n];
-    __shared__ BandRec<T> s_rec[RS];
+    // staged records during the scatter, the FFT's stage twiddles afterwards
+    constexpr int REC_BYTES = RS * (int)sizeof(BandRec<T>), TW_BYTES = n * (int)sizeof(T2);
+    __shared__ __align__(16) char s_buf[REC_BYTES > TW_BYTES ? REC_BYTES : TW_BYTES];
+    BandRec<T> *s_rec = reinterpret_cast<BandRec<T> *>(s_buf);
+    T2 *tw = reinterpret_cast<T2 *>(s_buf);
     __shared__ uint32_t s_slot;
     // the batch's tasks and the work items of their keys
     int t0, t1;
@@ -641,8 +649,13 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
     const uint32_t it = ib + blockIdx.x;
     if (it >= ie)
         return;
-    for (int e = threadIdx.x; e < n; e += NT)
-        tw[e] = twg[e];
+    // this thread's share of the stage twiddles, stored once the scatter is done
+    constexpr int TWPER = (n + NT - 1) / NT;
+    T2 twr[TWPER];
+#pragma unroll
+    for (int q = 0; q < TWPER; ++q)
+        if (threadIdx.x + q * NT < n)
+            twr[q] = twg[threadIdx.x + q * NT];
     {
         const Item item = items[it];
         const uint32_t tslot = item.key / (uint32_t)NB;
@@ -659,7 +672,7 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
                 // padding, no visibility: padded table: column n, after every
                 // footprint column (its cells lie in the row gaps); otherwise a
                 // row out of reach of every band row (no tap, cell kept)
-                r.xy = padded_taps<T>() ? (uint32_t)((n + S) * z) : 0x40000000u;
+                r.xy = padded_taps<T>() ? (uint32_t)((n + BAND_LEAD) * z) : 0x40000000u;
                 r.kb = 0;
                 r.vis.x = T(0);
                 r.vis.y = T(0);
@@ -678,18 +691,32 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
         const int lane = threadIdx.x % NL, grp = threadIdx.x / NL;
         const int myU = lane % S, myV = lane / S;
         // per lane, in bytes: column residue, tap row, cell row
-        const int uz = myU * z, tap_row = myV * S * z, cell_row = myV * BS * z;
+        const int uz = myU * z, tap_row = myV * S * z, cell_row = myV * LS * z;
         char *band_b = reinterpret_cast<char *>(band);
+        // cells are logical byte offsets; pad16 places them in LDS
         int cur = -1, carry = -1; // carry: the kept cell of the unpadded form
         T sr = T(0), si = T(0);
+        constexpr int LOGZ = sizeof(T2) == 8 ? 3 : 4;
+        // The groups take contiguous, column-sorted runs of each record stage,
+        // so the cells of two groups can meet only within one footprint width
+        // of the column where their runs meet.  A group's flush is a plain
+        // read-modify-write in its own column range [safe_a, safe_b) (no other
+        // lane writes those cells during the stage) and an LDS atomic add
+        // outside it; every lane flushes at the end of a stage.  (LDS float
+        // atomics cost about as much as the rest of the scatter.)
+        int safe_a = INT_MIN, safe_b = INT_MAX;
         auto flush = [&]() {
-            T *c = reinterpret_cast<T *>(band_b + cur);
-            if (GRP > 1) {
+            T *c = reinterpret_cast<T *>(band_b + cur + ((cur >> (LOGZ + 4)) << LOGZ));
+            const int col = cur - cell_row;
+            if (GRP > 1 && (col < safe_a || col >= safe_b)) {
                 atomicAdd(c, sr);
                 atomicAdd(c + 1, si);
             } else {
-                c[0] += sr;
-                c[1] += si;
+                T2 *c2 = reinterpret_cast<T2 *>(c);
+                T2 o = *c2;
+                o.x += sr;
+                o.y += si;
+                *c2 = o;
             }
         };
         // U records at a time: U tap loads in flight, then the U updates in order.
@@ -706,6 +733,12 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
             __syncthreads();
             const int ng = min(RG, max(0, (int)nloc - grp * RG));
             const BandRec<T> *gr = s_rec + grp * RG;
+            if (GRP > 1) {
+                // columns reached by the previous group's last record, and the
+                // next group's first footprint column
+                safe_a = grp > 0 ? (int)(s_rec[grp * RG - 1].xy & 0xffffu) + S * z : INT_MIN;
+                safe_b = grp < GRP - 1 ? (int)(s_rec[(grp + 1) * RG].xy & 0xffffu) : INT_MAX;
+            }
             using Bt = RecBatch<T, U>;
             // A lane whose row the footprint misses reads a zero tap: from the
             // padding of the record's sub-kernel (its cell then follows the
@@ -764,6 +797,15 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
                 }
             };
             pipelined_records<U, Bt>((uint32_t)((ng + U - 1) / U * U), fetch, update);
+            if (GRP > 1) {
+                // the next stage's runs start elsewhere: nothing stays in flight
+                if (cur >= 0 && (sr != T(0) || si != T(0)))
+                    flush();
+                cur = -1;
+                carry = -1;
+                sr = T(0);
+                si = T(0);
+            }
         }
         if (cur >= 0 && (sr != T(0) || si != T(0)))
             flush();
@@ -791,21 +833,27 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
             }
             dst = spill + (size_t)s_slot * H * n;
         }
+        __syncthreads(); // records no longer read: their space takes the twiddles
+#pragma unroll
+        for (int q = 0; q < TWPER; ++q)
+            if (threadIdx.x + q * NT < n)
+                tw[threadIdx.x + q * NT] = twr[q];
         __syncthreads();
-        T2 *rows = band + S;
-        fft_lines_lm<T2, LOGN, NT, BS, true>(rows, tw, H);
+        fft_band_rows<T2, LOGN, NT, BAND_LEAD, LS, true>(band, tw, H);
+        // element (l, c) of the band
+        auto at = [&](int l, int c) { return band[pad16(BAND_LEAD + l * LS) + pad16(c)]; };
         if (whole) {
             // rows [H b, H b + H) of column block cb are H * GCB contiguous elements
             T2 *d0 = dst + (size_t)H * b * GCB;
             for (int e = threadIdx.x; e < H * n; e += NT) {
                 const int cb = e / (H * GCB), w = e - cb * (H * GCB);
                 const int l = w / GCB, c = cb * GCB + (w - l * GCB);
-                d0[(size_t)cb * n * GCB + w] = rows[l * BS + c];
+                d0[(size_t)cb * n * GCB + w] = at(l, c);
             }
         } else {
             for (int e = threadIdx.x; e < H * n; e += NT) {
                 const int l = e / n, c = e - l * n;
-                dst[e] = rows[l * BS + c];
+                dst[e] = at(l, c);
             }
         }
     }
@@ -1670,8 +1718,9 @@ static int wtowers_batched_t(rg_context *ctx, const rg_grid_params *p, const rg_
     RG_TRY(ctx->scratch[10].ensure(sizeof(T2) * nn * ntowers_cap));
     T2 *subimg = ctx->scratch[10].as<T2>();
     RG_TRY(zero_fill(subimg, sizeof(T2) * nn * ntowers_cap, s));
-    const void *tw_inv = nullptr;
+    const void *tw_inv = nullptr, *tw_stage = nullptr;
     RG_TRY(get_twiddles(ctx, n, type, +1, &tw_inv));
+    RG_TRY(get_stage_twiddles(ctx, n, type, +1, &tw_stage));
 
     // batches of Q tasks (by default every task while the subgrids fit 16 GiB)
     constexpr size_t BATCH_BYTES = (size_t)16 << 30;
@@ -1739,7 +1788,7 @@ static int wtowers_batched_t(rg_context *ctx, const rg_grid_params *p, const rg_
         }
         bl.kern_bytes = (uint32_t)kb;
     }
-    bl.tw = tw_inv;
+    bl.tw = tw_stage;
     bl.G = G;
     bl.spill = spill;
     bl.spill_cap = (uint32_t)std::min<uint64_t>(max_spill, 0xffffffffu);
