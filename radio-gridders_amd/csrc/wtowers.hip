@@ -620,9 +620,9 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
     constexpr int NT = NL * GRP;
     constexpr int NB = n / H;
     constexpr int z = (int)sizeof(T2);
-    // records staged RS at a time (RS / GRP per group): 128 keeps the FP32
-    // 8-row band at four workgroups per CU
-    constexpr int RS = 128;
+    // records staged RS at a time (RS / GRP per group); they share LDS with
+    // the twiddles, which keeps the FP32 8-row band at four workgroups per CU
+    constexpr int RS = 256;
     constexpr int RG = RS / GRP;
     static_assert(RS % GRP == 0 && RS <= NT, "record staging");
     // row l of the band at logical [BAND_LEAD + l LS, + n) (pad16 in LDS); the
@@ -699,16 +699,22 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
         constexpr int LOGZ = sizeof(T2) == 8 ? 3 : 4;
         // The groups take contiguous, column-sorted runs of each record stage,
         // so the cells of two groups can meet only within one footprint width
-        // of the column where their runs meet.  A group's flush is a plain
-        // read-modify-write in its own column range [safe_a, safe_b) (no other
-        // lane writes those cells during the stage) and an LDS atomic add
-        // outside it; every lane flushes at the end of a stage.  (LDS float
-        // atomics cost about as much as the rest of the scatter.)
+        // of the column where their runs meet.  With two groups the second
+        // one walks its run backwards: both then reach that column last, so
+        // during the stage their cells never meet and every flush is a plain
+        // LDS read-modify-write; at the end of the stage the first group
+        // flushes its last cells, and after a barrier the second.  (LDS float
+        // atomics cost about as much as the rest of the scatter.)  With more
+        // groups a flush is plain in the group's own column range
+        // [safe_a, safe_b) and an LDS atomic add outside it.
+        constexpr bool REV = GRP == 2;
         int safe_a = INT_MIN, safe_b = INT_MAX;
+        // a cell of the row's gap: flushes into it are discarded
+        const int gap_cell = cell_row + (n + BAND_LEAD) * z;
         auto flush = [&]() {
             T *c = reinterpret_cast<T *>(band_b + cur + ((cur >> (LOGZ + 4)) << LOGZ));
             const int col = cur - cell_row;
-            if (GRP > 1 && (col < safe_a || col >= safe_b)) {
+            if (!REV && GRP > 1 && (col < safe_a || col >= safe_b)) {
                 atomicAdd(c, sr);
                 atomicAdd(c + 1, si);
             } else {
@@ -719,6 +725,10 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
                 *c2 = o;
             }
         };
+        if (REV) {
+            cur = gap_cell;
+            carry = gap_cell;
+        }
         // U records at a time: U tap loads in flight, then the U updates in order.
         constexpr int U = 8;
         static_assert(RG % U == 0, "record batch must be a multiple of the unroll");
@@ -726,14 +736,19 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
             const uint32_t nloc = min((uint32_t)RS, item.count - base);
             __syncthreads();
             if (threadIdx.x < RS) {
-                s_rec[threadIdx.x] = next;
+                // the second group's half backwards (its walk descends)
+                const int t = (int)threadIdx.x;
+                s_rec[REV && t >= RG ? 3 * RG - 1 - t : t] = next;
                 if (base + RS < item.count)
                     next = stage_rec(base + RS);
             }
             __syncthreads();
             const int ng = min(RG, max(0, (int)nloc - grp * RG));
+            const int nbu = (ng + U - 1) / U * U; // records walked, padding included
             const BandRec<T> *gr = s_rec + grp * RG;
-            if (GRP > 1) {
+            if (REV && grp == 1)
+                gr += RG - nbu; // this group's half is stored in reverse
+            if (GRP > 2) {
                 // columns reached by the previous group's last record, and the
                 // next group's first footprint column
                 safe_a = grp > 0 ? (int)(s_rec[grp * RG - 1].xy & 0xffffu) + S * z : INT_MIN;
@@ -783,7 +798,9 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
     #pragma unroll
                 for (int q = 0; q < U; ++q) {
                     if (bt.cell[q] != cur) {
-                        if (cur >= 0 && (sr != T(0) || si != T(0)))
+                        // (a zero test before the flush costs more than the
+                        // flush of a zero sum)
+                        if (REV || cur >= 0)
                             flush();
                         cur = bt.cell[q];
                         sr = T(0);
@@ -796,8 +813,20 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
                     si = fma(-bt.k[q].y, bt.v[q].x, si);
                 }
             };
-            pipelined_records<U, Bt>((uint32_t)((ng + U - 1) / U * U), fetch, update);
-            if (GRP > 1) {
+            pipelined_records<U, Bt>((uint32_t)nbu, fetch, update);
+            if (REV) {
+                // last cells: the first group's, then (after a barrier) the
+                // second's, which may be the same cells
+                if (grp == 0)
+                    flush();
+                __syncthreads();
+                if (grp == 1)
+                    flush();
+                cur = gap_cell;
+                carry = gap_cell;
+                sr = T(0);
+                si = T(0);
+            } else if (GRP > 1) {
                 // the next stage's runs start elsewhere: nothing stays in flight
                 if (cur >= 0 && (sr != T(0) || si != T(0)))
                     flush();
@@ -807,7 +836,7 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
                 si = T(0);
             }
         }
-        if (cur >= 0 && (sr != T(0) || si != T(0)))
+        if (GRP == 1 && cur >= 0 && (sr != T(0) || si != T(0)))
             flush();
 
         // destination: the task's subgrid band (column blocks), or a spill slot
