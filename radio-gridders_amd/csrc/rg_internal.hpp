@@ -137,8 +137,8 @@ struct rg_context {
     //            transposes
     //   13       channel offsets, 14 DFT, 16 uniform-weighting counts
     //   17       the W-stack, kept between calls
-    //   8        W-towers floor range; 18 W-towers task tables
-    rg::DevBuf scratch[19];
+    //   8        W-towers floor range; 18 W-towers task tables, 19 band masks
+    rg::DevBuf scratch[20];
     // channel offsets resident in scratch[13] (uploaded only when they change,
     // so a repeated multi-channel call issues no copy and can be captured)
     std::vector<int64_t> chan_off;

@@ -329,6 +329,26 @@ __global__ __launch_bounds__(256) void k_tower_tasks(const uint32_t *__restrict_
     }
 }
 
+// Per task: bit b of word b / 32 set when band b has work items (the band was
+// written); MW = NB / 32 words per task (rounded up).
+__global__ __launch_bounds__(256) void k_band_masks(const uint32_t *__restrict__ item_start,
+                                                    const int32_t *__restrict__ task_slot,
+                                                    const int32_t *__restrict__ totals, int nb,
+                                                    int mw, uint32_t *masks)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; // (task, word)
+    const int k = (int)(e / mw), wd = (int)(e - (int64_t)k * mw);
+    if (k >= totals[0])
+        return;
+    const uint32_t *f = item_start + (size_t)task_slot[k] * nb;
+    uint32_t m = 0;
+    for (int q = 0; q < 32 && wd * 32 + q < nb; ++q) {
+        const int bnd = wd * 32 + q;
+        m |= (f[bnd + 1] > f[bnd] ? 1u : 0u) << q;
+    }
+    masks[e] = m;
+}
+
 // Tasks [t0, t1) of batch b (Q tasks per batch); false when the batch is empty.
 struct BatchRef {
     const int32_t *task_slot;
@@ -630,7 +650,7 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
     static_assert(BAND_LEAD >= S, "gaps hold the footprint columns outside a row");
     constexpr int LS = n + BAND_LEAD;
     constexpr int BANDE = pad16(BAND_LEAD + H * LS);
-    __shared__ T2 band[BANDE];
+    __shared__ __align__(16) T2 band[BANDE];
     // staged records during the scatter, the FFT's stage twiddles afterwards
     constexpr int REC_BYTES = RS * (int)sizeof(BandRec<T>), TW_BYTES = n * (int)sizeof(T2);
     __shared__ __align__(16) char s_buf[REC_BYTES > TW_BYTES ? REC_BYTES : TW_BYTES];
@@ -682,9 +702,16 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
         BandRec<T> next;
         if (threadIdx.x < RS)
             next = stage_rec(0);
-        for (int e = threadIdx.x; e < BANDE; e += NT) {
-            band[e].x = T(0);
-            band[e].y = T(0);
+        {
+            // zero the band, 16 bytes per store
+            float4 *b4 = reinterpret_cast<float4 *>(band);
+            constexpr int N4 = (BANDE * (int)sizeof(T2)) / 16;
+            for (int e = threadIdx.x; e < N4; e += NT)
+                b4[e] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if ((BANDE * (int)sizeof(T2)) % 16 != 0 && threadIdx.x == 0) {
+                band[BANDE - 1].x = T(0);
+                band[BANDE - 1].y = T(0);
+            }
         }
         const __amdgpu_buffer_rsrc_t ktab =
             __builtin_amdgcn_make_buffer_rsrc((void *)kern, 0, (int)kern_bytes, BUF_RSRC_W3);
@@ -724,6 +751,15 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
                 o.y += si;
                 *c2 = o;
             }
+        };
+        // with two groups a lane reads a cell's LDS value when it starts
+        // summing into it (no other lane writes that cell before the lane's
+        // flush), so the flush is a store
+        T2 pre;
+        pre.x = T(0);
+        pre.y = T(0);
+        auto cellp = [&](int cl) {
+            return reinterpret_cast<T2 *>(band_b + cl + ((cl >> (LOGZ + 4)) << LOGZ));
         };
         if (REV) {
             cur = gap_cell;
@@ -798,11 +834,18 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
     #pragma unroll
                 for (int q = 0; q < U; ++q) {
                     if (bt.cell[q] != cur) {
-                        // (a zero test before the flush costs more than the
-                        // flush of a zero sum)
-                        if (REV || cur >= 0)
-                            flush();
-                        cur = bt.cell[q];
+                        if (REV) {
+                            T2 o = pre;
+                            o.x += sr;
+                            o.y += si;
+                            *cellp(cur) = o;
+                            cur = bt.cell[q];
+                            pre = *cellp(cur);
+                        } else {
+                            if (cur >= 0)
+                                flush();
+                            cur = bt.cell[q];
+                        }
                         sr = T(0);
                         si = T(0);
                     }
@@ -871,18 +914,29 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
         fft_band_rows<T2, LOGN, NT, BAND_LEAD, LS, true>(band, tw, H);
         // element (l, c) of the band
         auto at = [&](int l, int c) { return band[pad16(BAND_LEAD + l * LS) + pad16(c)]; };
+        // two elements per thread and store (columns c, c + 1 with c even are
+        // adjacent in LDS and in both destination layouts)
+        auto put2 = [&](T2 *d, int l, int c) {
+            if constexpr (sizeof(T2) == 8) {
+                const T2 a0 = at(l, c), a1 = at(l, c + 1);
+                *reinterpret_cast<float4 *>(d) = make_float4(a0.x, a0.y, a1.x, a1.y);
+            } else {
+                d[0] = at(l, c);
+                d[1] = at(l, c + 1);
+            }
+        };
         if (whole) {
             // rows [H b, H b + H) of column block cb are H * GCB contiguous elements
             T2 *d0 = dst + (size_t)H * b * GCB;
-            for (int e = threadIdx.x; e < H * n; e += NT) {
+            for (int e = 2 * threadIdx.x; e < H * n; e += 2 * NT) {
                 const int cb = e / (H * GCB), w = e - cb * (H * GCB);
                 const int l = w / GCB, c = cb * GCB + (w - l * GCB);
-                d0[(size_t)cb * n * GCB + w] = at(l, c);
+                put2(d0 + (size_t)cb * n * GCB + w, l, c);
             }
         } else {
-            for (int e = threadIdx.x; e < H * n; e += NT) {
+            for (int e = 2 * threadIdx.x; e < H * n; e += 2 * NT) {
                 const int l = e / n, c = e - l * n;
-                dst[e] = at(l, c);
+                put2(dst + e, l, c);
             }
         }
     }
@@ -956,7 +1010,7 @@ constexpr int SCREEN_RUN = 8;
 
 template <typename T, int LOGN, int CB, int H>
 __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
-    const typename Vec2<T>::type *__restrict__ G, const uint32_t *__restrict__ key_start,
+    const typename Vec2<T>::type *__restrict__ G, const uint32_t *__restrict__ band_masks,
     BatchRef bref, int batch, const int32_t *__restrict__ task_wp,
     const int32_t *__restrict__ chunk_first, const int32_t *__restrict__ chunk_count,
     typename Vec2<T>::type *P, const typename Vec2<T>::type *__restrict__ twg, double theta,
@@ -969,8 +1023,8 @@ __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
     constexpr int NB = n / H;
     __shared__ T2 lds[n * CB];
     __shared__ T2 tw[n];
-    __shared__ uint8_t s_band[NB]; // band b of the current floor has records
-    static_assert(NB <= NT, "one thread per band flag");
+    constexpr int MW = (NB + 31) / 32; // band-mask words per task
+    static_assert(MW <= 4, "at most 128 bands");
     int t0, t1;
     if (!batch_tasks(bref, batch, t0, t1))
         return;
@@ -1017,13 +1071,12 @@ __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
         int run = 0, wp_prev = 0;
         for (int k = first; k < last; ++k) {
             const T2 *g = G + (size_t)(k - t0) * n * n + (size_t)blockIdx.x * n * CB + line;
-            // band b of the task was written iff it has work items (the
-            // previous floor's flags were read in its first stage, before its
-            // barrier)
-            if (threadIdx.x < NB) {
-                const uint32_t *f = key_start + (size_t)bref.task_slot[k] * NB;
-                s_band[threadIdx.x] = f[threadIdx.x + 1] > f[threadIdx.x] ? 1 : 0;
-            }
+            // band b of the task was written iff it has work items (bit b of
+            // the task's mask; uniform loads)
+            uint32_t bm[MW];
+#pragma unroll
+            for (int q = 0; q < MW; ++q)
+                bm[q] = band_masks[(size_t)k * MW + q];
             const int wp = task_wp[k];
             if (run > 0 && (run == SCREEN_RUN || wp != wp_prev + 1)) {
                 close_run(wp_prev);
@@ -1034,7 +1087,8 @@ __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
                 T2 val;
                 val.x = T(0);
                 val.y = T(0);
-                if (s_band[row / H])
+                const int bnd = row / H;
+                if ((bm[bnd >> 5] >> (bnd & 31)) & 1u)
                     val = g[(size_t)row * CB];
                 return val;
             };
@@ -1047,7 +1101,7 @@ __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
                     h[q] = val;
                 }
             };
-            __syncthreads(); // the previous floor's last stage has read lds; band flags set
+            __syncthreads(); // the previous floor's last stage has read lds
             fft_block_io<T2, LOGN, CB, CB, true>(lds, tw, load, emit);
             ++run;
             wp_prev = wp;
@@ -1289,7 +1343,7 @@ static int launch_bands(int n, int H, const BandLaunch &a, hipStream_t s)
 
 struct ColsLaunch {
     const void *G;
-    const uint32_t *key_start;
+    const uint32_t *band_masks; // per task, (n / H + 31) / 32 words
     BatchRef bref;
     int batch;
     const int32_t *task_wp, *chunk_first, *chunk_count;
@@ -1307,7 +1361,7 @@ static int launch_cols_h(const ColsLaunch &a, hipStream_t s)
     constexpr int CB0 = 8; // columns per workgroup (512 threads at n = 512)
     constexpr int CB = CB0 > n ? n : CB0;
     hipLaunchKernelGGL((k_tower_cols<T, LOGN, CB, H>), dim3(n / CB, a.grid_y), dim3(CB * (n / 8)),
-                       0, s, (const T2 *)a.G, a.key_start, a.bref, a.batch, a.task_wp,
+                       0, s, (const T2 *)a.G, a.band_masks, a.bref, a.batch, a.task_wp,
                        a.chunk_first, a.chunk_count, (T2 *)a.P, (const T2 *)a.tw, a.theta, a.winc);
     RG_HIP(hipGetLastError());
     return RG_OK;
@@ -1826,9 +1880,20 @@ static int wtowers_batched_t(rg_context *ctx, const rg_grid_params *p, const rg_
     // one band-pass workgroup per work item (the plan's item capacity; a
     // batch with more items reports an overrun)
     bl.blocks = (uint32_t)std::max(1, nitems_cap);
+    // per task: which bands were written
+    const int mw = (NB + 31) / 32;
+    RG_TRY(ctx->scratch[19].ensure(sizeof(uint32_t) * (size_t)std::max(ntasks_cap, 1) * mw));
+    uint32_t *band_masks = ctx->scratch[19].as<uint32_t>();
+    {
+        const int64_t ne = (int64_t)ntasks_cap * mw;
+        hipLaunchKernelGGL(k_band_masks, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s,
+                           (const uint32_t *)item_start_w, (const int32_t *)tt.task_slot,
+                           (const int32_t *)tt.totals, NB, mw, band_masks);
+        RG_HIP(hipGetLastError());
+    }
     ColsLaunch cl;
     cl.G = G;
-    cl.key_start = item_start_w;
+    cl.band_masks = band_masks;
     cl.bref = bl.bref;
     cl.task_wp = tt.task_wp;
     cl.chunk_first = tt.chunk_first;
