@@ -119,7 +119,6 @@ def main():
     SUB, MARGIN, WINC = 512, 16, 10.0
 
     ctx = rgamd.Context(0)
-    ctx.set_timing(True)
     stream = torch.cuda.current_stream()
     if args.calibrate:
         # a plain 1 GiB device copy: the reference dispatch for scaling the
@@ -130,29 +129,57 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    def timed(label, p, fv, wk, grid, steps, warmup):
-        for _ in range(warmup):
+    def timed(label, p, fv, wk, grid, steps, warmup, graph=False):
+        """ms per step of `steps` calls (kernel timers off), then the per-kernel
+        breakdown from a second pass with the timers on.  graph=True: the call
+        is captured once in a HIP graph (after an eager call of its shape) and
+        the graph is replayed `steps` times instead (no host work per step)."""
+        ctx.set_timing(False)
+        for _ in range(max(warmup, 1 if graph else 0)):
             ctx.grid(p, fv, wk, grid)
         torch.cuda.synchronize()
-        for name in rgamd.TIMERS:
-            ctx.timer(name)  # reset
+        run = lambda: ctx.grid(p, fv, wk, grid)  # noqa: E731
+        g = None
+        if graph:
+            g = torch.cuda.CUDAGraph()
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):
+                with torch.cuda.graph(g, stream=side):
+                    ctx.grid(p, fv, wk, grid, stream=side.cuda_stream)
+            torch.cuda.current_stream().wait_stream(side)
+            g.replay()
+            torch.cuda.synchronize()
+            run = g.replay
         t_start = time.perf_counter()
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         ev0.record(stream)
         for _ in range(steps):
-            ctx.grid(p, fv, wk, grid)
+            run()
         ev1.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t_start
         ms = ev0.elapsed_time(ev1) / steps
-        st = ctx.stats()
+        st = ctx.stats()  # raises when a replay outgrew the capacities it was captured with
         timers = {}
-        for name in rgamd.TIMERS:
-            tm, cnt = ctx.timer(name)
-            if cnt:
-                timers[name] = {"ms_per_step": tm / steps, "launches_per_step": cnt / steps}
-        log(f"[bench] {label}: {ms:.3f} ms/step ({obs.n_vis / ms / 1e3:.1f} Mvis/s), "
-            f"wall {wall / steps * 1e3:.1f} ms/step, timers {timers}")
+        if g is not None:
+            del g
+            ctx.thaw()
+        else:
+            ctx.set_timing(True)
+            for name in rgamd.TIMERS:
+                ctx.timer(name)  # reset
+            for _ in range(steps):
+                ctx.grid(p, fv, wk, grid)
+            torch.cuda.synchronize()
+            for name in rgamd.TIMERS:
+                tm, cnt = ctx.timer(name)
+                if cnt:
+                    timers[name] = {"ms_per_step": tm / steps, "launches_per_step": cnt / steps}
+            ctx.set_timing(False)
+        log(f"[bench] {label}{' (graph replay)' if graph else ''}: {ms:.3f} ms/step "
+            f"({obs.n_vis / ms / 1e3:.1f} Mvis/s), wall {wall / steps * 1e3:.1f} ms/step, "
+            f"timers {timers}")
         return {"ms_per_step": ms, "mvis_s": obs.n_vis / ms / 1e3, "timers": timers,
                 "gridded": st.n_gridded, "dropped": st.n_dropped, "bad_plane": st.n_bad_plane,
                 "items": st.n_items, "tasks": st.n_tasks, "towers": st.n_towers,
@@ -185,6 +212,8 @@ def main():
                                   g64 if args.skip_plain else torch.empty_like(g64), K, W)
         r["fp32_batched"] = timed("W-proj FP32 batched", rgamd.params(N, theta), fv64, k64_p, g32,
                                   K, W)
+        r["fp32_batched_graph"] = timed("W-proj FP32 batched", rgamd.params(N, theta), fv64,
+                                        k64_p, g32, K, W, graph=True)
         if not args.skip_plain:
             ctx.grid(rgamd.params(N, theta, rgamd.RG_MODE_PLAIN), fv128, k128_p, g64)
         i32, i64 = image(g32), image(g64)
@@ -208,6 +237,8 @@ def main():
         else:
             g64.copy_(gb)
         r["fp32_batched"] = timed("W-towers FP32 batched", pb, fv64, k64_t, g32, K, W)
+        r["fp32_batched_graph"] = timed("W-towers FP32 batched", pb, fv64, k64_t, g32, K, W,
+                                        graph=True)
         i32, i64, ib = image(g32), image(g64), image(gb)
         r["error_fp32_batched_vs_fp64"] = errs(i32, i64)
         r["error_fp64_batched_vs_fp64"] = errs(ib, i64)
@@ -262,7 +293,11 @@ def main():
         print(json.dumps({"metric": METRIC, "value": None, "results": res}), flush=True)
         return
     head_kind = "wtowers" if "wtowers" in res else "wproj"
-    head = res[head_kind]["fp32_batched"]
+    # the step replayed from a captured graph (the batched path's serving mode:
+    # no host work per step); kernel times come from the eager run's timers
+    eager = res[head_kind]["fp32_batched"]
+    head = dict(res[head_kind].get("fp32_batched_graph", eager))
+    head["timers"] = eager["timers"]
     plain = res[head_kind].get("fp64_plain")
     nv = obs.n_vis
     # the counter summaries were taken on the default 512-station, 64-dump workload
@@ -318,7 +353,11 @@ def main():
                                 "(theta 0.1, lambda 20480), W-kernels S16 x R8 "
                                 + ("W-projection (w-step 10)" if head_kind == "wproj"
                                    else "W-towers subgrid 512, margin 16, winc 10")),
-                   "n_vis": nv, "grid": N, "path": head_kind},
+                   "n_vis": nv, "grid": N, "path": head_kind,
+                   "mode": ("one batched call per step, replayed from a captured HIP graph"
+                            if "fp32_batched_graph" in res[head_kind] else
+                            "one batched call per step")},
+        "eager_mvis_s": eager["mvis_s"],
         "speedup_vs_fp64_plain": (head["mvis_s"] / plain["mvis_s"]) if plain else None,
         "error_vs_fp64": res[head_kind]["error_fp32_batched_vs_fp64"],
         "gate": ("max and RMS relative image error of FP32 batched against FP64 plain on "
@@ -668,6 +707,7 @@ def bench_channels(ctx, steps, warmup):
                          "rms_rel": float(torch.sqrt((d ** 2).mean()) /
                                           torch.sqrt((b.abs() ** 2).mean()))})
     del graph, grids
+    ctx.thaw()  # the capture froze the context's scratch memory
     out = {"n_vis": nv, "channels": nc, "grid": N, "w_planes": P,
            "counts": {"min": int(np.diff(off).min()), "median": float(np.median(np.diff(off))),
                       "max": int(np.diff(off).max())},
