@@ -32,18 +32,26 @@ FP32_PEAK_TFLOPS = 157.3  # MI355X FP32 vector (published)
 PROFILES = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
 
 
-def profiled_traffic(name):
+def profiled_traffic(name, launch_params):
     """HBM bytes per launch of a kernel, from the FETCH_SIZE / WRITE_SIZE
-    summary under profiles/ (counters scaled by a 1 GiB copy), or None."""
+    summary under profiles/ (counters scaled by a 1 GiB copy), or None when
+    the summary is missing or was taken with other launch parameters."""
     path = os.path.join(PROFILES, name)
+    traffic, prof_params = None, None
     try:
         with open(path) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_dispatch:"):
-                    return float(line.split(":", 1)[1])
-    except OSError:
-        pass
-    return None
+                    traffic = float(line.split(":", 1)[1])
+                elif line.startswith("launch params:"):
+                    prof_params = json.loads(line.split(":", 1)[1])
+    except (OSError, ValueError):
+        return None
+    return traffic if prof_params == launch_params else None
+
+
+def launch_params(p):
+    return {k: int(getattr(p, k)) for k in ("tile", "vis_per_item", "floor_batch", "fft_radix")}
 
 
 def log(*a):
@@ -78,6 +86,12 @@ def main():
                     help="seeded pixels of the FP64 DFT oracle comparison (0 = skip)")
     ap.add_argument("--tower-band", type=int, default=0, help="W-towers band height (8 or 16)")
     ap.add_argument("--floor-batch", type=int, default=0, help="W-towers tasks per batch")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="FP32 batched calls with the default (or given) launch parameters "
+                         "instead of the tuned ones")
+    ap.add_argument("--tune-cache", default=None,
+                    help="tuning cache file (default $RGAMD_TUNE_CACHE or "
+                         "~/.cache/rgamd/tune.json)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the gridding path is "
@@ -87,6 +101,7 @@ def main():
     import torch
 
     import rgamd
+    import rgtune
     import synth
 
     t0 = time.time()
@@ -196,6 +211,19 @@ def main():
         return {"max_rel": float(d.max() / b.abs().max()),
                 "rms_rel": float(torch.sqrt((d ** 2).mean()) / torch.sqrt((b.abs() ** 2).mean()))}
 
+    def tune(label, r, base, fv, wk, grid):
+        """Launch parameters of the FP32 batched call from the tuning cache, or
+        searched on this data (outside the timed regions) and cached."""
+        if args.no_tune:
+            return base
+        t0 = time.time()
+        p, info = rgtune.tuned_params(ctx, base, fv, wk, grid, cache=args.tune_cache)
+        r["tuning"] = {"source": info["source"], "key": info["key"], "chosen": info["chosen"],
+                       "ms": info["ms"], "candidates": info["table"]}
+        log(f"[bench] {label} FP32 launch parameters ({info['source']}, "
+            f"{time.time() - t0:.1f}s): {info['chosen']}")
+        return p
+
     res = {}
     dft_images = {}
     K, W = args.steps, args.warmup
@@ -210,10 +238,11 @@ def main():
             ctx.grid(rgamd.params(N, theta, rgamd.RG_MODE_BATCHED), fv128, k128_p, g64)
         r["fp64_batched"] = timed("W-proj FP64 batched", rgamd.params(N, theta), fv128, k128_p,
                                   g64 if args.skip_plain else torch.empty_like(g64), K, W)
-        r["fp32_batched"] = timed("W-proj FP32 batched", rgamd.params(N, theta), fv64, k64_p, g32,
-                                  K, W)
-        r["fp32_batched_graph"] = timed("W-proj FP32 batched", rgamd.params(N, theta), fv64,
-                                        k64_p, g32, K, W, graph=True)
+        p32 = tune("W-proj", r, rgamd.params(N, theta), fv64, k64_p, g32)
+        r["launch_params"] = launch_params(p32)
+        r["fp32_batched"] = timed("W-proj FP32 batched", p32, fv64, k64_p, g32, K, W)
+        r["fp32_batched_graph"] = timed("W-proj FP32 batched", p32, fv64, k64_p, g32, K, W,
+                                        graph=True)
         if not args.skip_plain:
             ctx.grid(rgamd.params(N, theta, rgamd.RG_MODE_PLAIN), fv128, k128_p, g64)
         i32, i64 = image(g32), image(g64)
@@ -236,8 +265,10 @@ def main():
             r["fp64_plain"] = timed("W-towers FP64 plain", pp, fv128, k128_t, g64, 1, 0)
         else:
             g64.copy_(gb)
-        r["fp32_batched"] = timed("W-towers FP32 batched", pb, fv64, k64_t, g32, K, W)
-        r["fp32_batched_graph"] = timed("W-towers FP32 batched", pb, fv64, k64_t, g32, K, W,
+        p32 = tune("W-towers", r, pb, fv64, k64_t, g32)
+        r["launch_params"] = launch_params(p32)
+        r["fp32_batched"] = timed("W-towers FP32 batched", p32, fv64, k64_t, g32, K, W)
+        r["fp32_batched_graph"] = timed("W-towers FP32 batched", p32, fv64, k64_t, g32, K, W,
                                         graph=True)
         i32, i64, ib = image(g32), image(g64), image(gb)
         r["error_fp32_batched_vs_fp64"] = errs(i32, i64)
@@ -300,16 +331,17 @@ def main():
     head["timers"] = eager["timers"]
     plain = res[head_kind].get("fp64_plain")
     nv = obs.n_vis
-    # the counter summaries were taken on the default 512-station, 64-dump workload
-    default_workload = (args.stations, args.dumps) == (512, 64) and not args.tower_band \
-        and not args.floor_batch
-    H = args.tower_band or 8
+    # the counter summaries were taken on the default 512-station, 64-dump
+    # workload (and are used only with the launch parameters they name)
+    default_workload = (args.stations, args.dumps) == (512, 64)
+    lp = res[head_kind]["launch_params"]
+    H = lp["tile"] if lp["tile"] in (4, 16) else 8
     # FP32 tables: W-projection as given; W-towers band pass: the padded copy
     # (H zero rows after every S x S sub-kernel and before the first)
     tab_bytes = (wp_planes * 64 * S * S * 8 if head_kind == "wproj"
                  else (H + 11 * 64 * (S + H)) * S * 8)
-    roof, job = roofline(head_kind, head, head["_stats"], nv, S, N, SUB, H, default_workload,
-                         tab_bytes)
+    roof, job = roofline(head_kind, head, head["_stats"], nv, S, N, SUB, H,
+                         lp if default_workload else None, tab_bytes)
 
     cpu = None
     if not args.skip_cpu:
@@ -353,7 +385,7 @@ def main():
                                 "(theta 0.1, lambda 20480), W-kernels S16 x R8 "
                                 + ("W-projection (w-step 10)" if head_kind == "wproj"
                                    else "W-towers subgrid 512, margin 16, winc 10")),
-                   "n_vis": nv, "grid": N, "path": head_kind,
+                   "n_vis": nv, "grid": N, "path": head_kind, "launch_params": lp,
                    "mode": ("one batched call per step, replayed from a captured HIP graph"
                             if "fp32_batched_graph" in res[head_kind] else
                             "one batched call per step")},
@@ -844,7 +876,7 @@ def wproj_work(st, nv, N, S, table_bytes, s_v=8, s_grid=8):
     return b, f
 
 
-def roofline(kind, head, st, nv, S, N, n, H, default_workload, table_bytes):
+def roofline(kind, head, st, nv, S, N, n, H, launch, table_bytes):
     """Dominant-kernel and whole-step rooflines.  `frac` (HBM) and `valu.frac`
     use the compulsory bytes and useful flop of tower_work / wproj_work over
     the kernel's average launch time; `effective` keeps the per-visibility
@@ -869,7 +901,7 @@ def roofline(kind, head, st, nv, S, N, n, H, default_workload, table_bytes):
         avg_ms = t["ms_per_step"] / launches
         byt = b[part] / launches
         ach = byt / (avg_ms * 1e-3) / 1e9
-        traffic = profiled_traffic(prof) if default_workload else None
+        traffic = profiled_traffic(prof, launch) if launch else None
         tf = f[part] / launches / (avg_ms * 1e-3) / 1e12
         roof = {"kernel": name, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic,

@@ -96,6 +96,8 @@ typedef struct rg_grid_params {
                                compensated tile sums (worth it only into an RG_C128 grid) */
     int32_t floor_batch;    /* batched W-towers: (tower, floor) tasks per batch (0 = as many as
                                fit 16 GiB of subgrids) */
+    int32_t fft_radix;      /* batched W-towers: radix of the column pass's FFT stages (8 or
+                               4, 0 = 8; the last stage takes what is left of log2 n) */
 } rg_grid_params;
 
 /* Counters of the last rg_grid call (read back synchronously). */

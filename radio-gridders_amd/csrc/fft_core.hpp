@@ -187,16 +187,19 @@ __device__ inline void fft_stage(T2 *lds, const T2 *tw, int logns)
 // stage (compile-time after unrolling), idx is the position in the line
 // (line = threadIdx.x % B).  The caller synchronises before the next use of
 // lds (the last stage reads it).  Needs at least two stages (n >= 16).
-template <int LOGN> __host__ __device__ constexpr int last_radix()
+// RMAX (8 or 4) is the radix of every stage but the last, which takes what
+// is left of LOGN.
+template <int LOGN, int RMAX = 8> __host__ __device__ constexpr int last_radix()
 {
-    return (LOGN % 3) == 0 ? 8 : ((LOGN % 3) == 2 ? 4 : 2);
+    return RMAX == 8 ? ((LOGN % 3) == 0 ? 8 : ((LOGN % 3) == 2 ? 4 : 2))
+                     : ((LOGN % 2) == 0 ? 4 : 2);
 }
 
 // position in the line of the last stage's output `slot` for this thread
-template <int LOGN, int B> __device__ inline int fft_out_index(int slot)
+template <int LOGN, int B, int RMAX = 8> __device__ inline int fft_out_index(int slot)
 {
     constexpr int n = 1 << LOGN;
-    constexpr int R = last_radix<LOGN>();
+    constexpr int R = last_radix<LOGN, RMAX>();
     const int lt = threadIdx.x / B;
     return lt + (slot / R) * (n / 8) + (slot % R) * (n / R);
 }
@@ -251,19 +254,22 @@ __device__ inline void fft_stage_io(T2 *lds, const T2 *tw, int logns, Load &load
         __syncthreads();
 }
 
-template <typename T2, int LOGN, int B, int LS, bool INV, typename Load, typename Emit>
+template <typename T2, int LOGN, int B, int LS, bool INV, int RMAX = 8, typename Load,
+          typename Emit>
 __device__ inline void fft_block_io(T2 *lds, const T2 *tw, Load &&load, Emit &&emit)
 {
-    static_assert(LOGN >= 4, "two or more stages");
-    constexpr int NST = LOGN / 3 + (LOGN % 3 ? 1 : 0);
-    fft_stage_io<T2, LOGN, B, LS, 8, INV, 1>(lds, tw, 0, load, emit);
-    int logns = 3;
+    static_assert(RMAX == 8 || RMAX == 4, "stage radix 8 or 4");
+    constexpr int LR = RMAX == 8 ? 3 : 2;
+    static_assert(LOGN > LR, "two or more stages");
+    constexpr int NST = LOGN / LR + (LOGN % LR ? 1 : 0);
+    fft_stage_io<T2, LOGN, B, LS, RMAX, INV, 1>(lds, tw, 0, load, emit);
+    int logns = LR;
 #pragma unroll
     for (int st = 1; st < NST - 1; ++st) {
-        fft_stage<T2, LOGN, B, LS, 8, INV>(lds, tw, logns);
-        logns += 3;
+        fft_stage<T2, LOGN, B, LS, RMAX, INV>(lds, tw, logns);
+        logns += LR;
     }
-    fft_stage_io<T2, LOGN, B, LS, last_radix<LOGN>(), INV, 2>(lds, tw, logns, load, emit);
+    fft_stage_io<T2, LOGN, B, LS, last_radix<LOGN, RMAX>(), INV, 2>(lds, tw, logns, load, emit);
 }
 
 // ---- line-major variant ---------------------------------------------------------

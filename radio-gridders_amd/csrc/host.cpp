@@ -193,6 +193,10 @@ static int check_params(const rg_grid_params *p, const rg_wkern *wk)
         if (!(p->winc > 0))
             return fail(RG_ERR_PARAM, "winc must be positive");
     }
+    if (p->fft_radix != 0 && p->fft_radix != 4 && p->fft_radix != 8)
+        return fail(RG_ERR_PARAM, "fft_radix must be 0, 4 or 8 (got %d)", p->fft_radix);
+    if (p->vis_per_item < 0 || p->floor_batch < 0)
+        return fail(RG_ERR_PARAM, "vis_per_item and floor_batch must not be negative");
     return RG_OK;
 }
 

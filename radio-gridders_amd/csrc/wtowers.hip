@@ -1008,7 +1008,7 @@ __global__ __launch_bounds__(256) void k_band_combine(const Item *__restrict__ i
 // the rounding of E compounds over at most SCREEN_RUN - 1 floors.
 constexpr int SCREEN_RUN = 8;
 
-template <typename T, int LOGN, int CB, int H>
+template <typename T, int LOGN, int CB, int H, int RX>
 __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
     const typename Vec2<T>::type *__restrict__ G, const uint32_t *__restrict__ band_masks,
     BatchRef bref, int batch, const int32_t *__restrict__ task_wp,
@@ -1042,7 +1042,7 @@ __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
     T2 step[PER], acc[PER], h[PER];
 #pragma unroll
     for (int q = 0; q < PER; ++q) {
-        base[q] = screen_base(col, fft_out_index<LOGN, CB>(q), n, theta);
+        base[q] = screen_base(col, fft_out_index<LOGN, CB, RX>(q), n, theta);
         Trig<T>::rot(winc * base[q], step[q].x, step[q].y);
     }
     // acc += h E^wl
@@ -1102,7 +1102,7 @@ __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
                 }
             };
             __syncthreads(); // the previous floor's last stage has read lds
-            fft_block_io<T2, LOGN, CB, CB, true>(lds, tw, load, emit);
+            fft_block_io<T2, LOGN, CB, CB, true, RX>(lds, tw, load, emit);
             ++run;
             wp_prev = wp;
         }
@@ -1111,7 +1111,7 @@ __global__ __launch_bounds__(CB *((1 << LOGN) / 8)) void k_tower_cols(
         T2 *pp = P + (size_t)c * n * n + col;
 #pragma unroll
         for (int q = 0; q < PER; ++q)
-            pp[(size_t)fft_out_index<LOGN, CB>(q) * n] = acc[q];
+            pp[(size_t)fft_out_index<LOGN, CB, RX>(q) * n] = acc[q];
     }
 }
 
@@ -1351,20 +1351,28 @@ struct ColsLaunch {
     const void *tw;
     double theta, winc;
     unsigned grid_y; // workgroup rows (each loops over the chunks)
+    int radix;       // FFT stage radix (8 or 4)
 };
 
-template <typename T, int LOGN, int H>
-static int launch_cols_h(const ColsLaunch &a, hipStream_t s)
+template <typename T, int LOGN, int H, int RX>
+static int launch_cols_r(const ColsLaunch &a, hipStream_t s)
 {
     using T2 = typename Vec2<T>::type;
     constexpr int n = 1 << LOGN;
     constexpr int CB0 = 8; // columns per workgroup (512 threads at n = 512)
     constexpr int CB = CB0 > n ? n : CB0;
-    hipLaunchKernelGGL((k_tower_cols<T, LOGN, CB, H>), dim3(n / CB, a.grid_y), dim3(CB * (n / 8)),
-                       0, s, (const T2 *)a.G, a.band_masks, a.bref, a.batch, a.task_wp,
-                       a.chunk_first, a.chunk_count, (T2 *)a.P, (const T2 *)a.tw, a.theta, a.winc);
+    hipLaunchKernelGGL((k_tower_cols<T, LOGN, CB, H, RX>), dim3(n / CB, a.grid_y),
+                       dim3(CB * (n / 8)), 0, s, (const T2 *)a.G, a.band_masks, a.bref, a.batch,
+                       a.task_wp, a.chunk_first, a.chunk_count, (T2 *)a.P, (const T2 *)a.tw,
+                       a.theta, a.winc);
     RG_HIP(hipGetLastError());
     return RG_OK;
+}
+
+template <typename T, int LOGN, int H>
+static int launch_cols_h(const ColsLaunch &a, hipStream_t s)
+{
+    return a.radix == 4 ? launch_cols_r<T, LOGN, H, 4>(a, s) : launch_cols_r<T, LOGN, H, 8>(a, s);
 }
 
 template <typename T, int LOGN>
@@ -1903,6 +1911,7 @@ static int wtowers_batched_t(rg_context *ctx, const rg_grid_params *p, const rg_
     cl.theta = p->theta;
     cl.winc = p->winc;
     cl.grid_y = (unsigned)std::min<int64_t>(std::max(nchunks_cap, 1), 65535);
+    cl.radix = p->fft_radix == 4 ? 4 : 8;
     int64_t rgx = (int64_t)((nn + 255) / 256);
     if (rgx > 1024)
         rgx = 1024;

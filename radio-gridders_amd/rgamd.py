@@ -58,7 +58,7 @@ class GridParams(C.Structure):
     _fields_ = [("grid_size", C.c_int32), ("mode", C.c_int32), ("theta", C.c_double),
                 ("subgrid_size", C.c_int32), ("subgrid_margin", C.c_int32), ("winc", C.c_double),
                 ("tile", C.c_int32), ("vis_per_item", C.c_int32), ("compensated", C.c_int32),
-                ("floor_batch", C.c_int32)]
+                ("floor_batch", C.c_int32), ("fft_radix", C.c_int32)]
 
 
 class SepKernDesc(C.Structure):
@@ -215,9 +215,10 @@ def wkern_desc(table, w_min: float, w_step: float) -> WKern:
 
 
 def params(grid_size, theta, mode=RG_MODE_BATCHED, subgrid=0, margin=0, winc=0.0, tile=0,
-           vis_per_item=0, compensated=0, floor_batch=0) -> GridParams:
+           vis_per_item=0, compensated=0, floor_batch=0, fft_radix=0) -> GridParams:
     return GridParams(int(grid_size), int(mode), float(theta), int(subgrid), int(margin),
-                      float(winc), int(tile), int(vis_per_item), int(compensated), int(floor_batch))
+                      float(winc), int(tile), int(vis_per_item), int(compensated), int(floor_batch),
+                      int(fft_radix))
 
 
 def grid_type_of(t) -> int:
@@ -229,6 +230,7 @@ class Context:
     """An rg_context on one device."""
 
     def __init__(self, device: int = 0):
+        self.device = int(device)
         self.h = C.c_void_p()
         check(lib().rg_context_create(C.byref(self.h), int(device)))
 

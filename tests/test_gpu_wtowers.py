@@ -43,11 +43,13 @@ def test_plain_fp64_matches_oracle(ctx, case):
     assert st.n_tasks > 1 and st.n_towers >= 1
 
 
-@pytest.mark.parametrize("floor_batch,band", [(1, 8), (5, 8), (128, 8), (5, 4)])
-def test_batched_fp64_matches_oracle(ctx, case, floor_batch, band):
+@pytest.mark.parametrize("floor_batch,band,radix", [(1, 8, 0), (5, 8, 0), (128, 8, 0), (5, 4, 0),
+                                                   (5, 8, 4), (0, 4, 4)])
+def test_batched_fp64_matches_oracle(ctx, case, floor_batch, band, radix):
+    """radix 4: the column pass's FFT in radix-4 stages (4, 4, 4 at n = 64)."""
     obs, vis, table, ref = case
     g, st = run(ctx, obs, vis, table, rgamd.RG_MODE_BATCHED, floor_batch=floor_batch,
-                vis_per_item=50, tile=band)
+                vis_per_item=50, tile=band, fft_radix=radix)
     assert rel_max(g, ref) < 1e-11
     assert st.n_gridded == obs.n_vis and st.n_dropped == 0
 
@@ -109,8 +111,11 @@ def test_stress_shape_matches_oracle(ctx):
 
 
 @pytest.mark.parametrize("sub,margin,S", [(128, 16, 8), (128, 8, 8), (256, 16, 16)])
-def test_other_shapes_match_oracle(ctx, sub, margin, S):
-    """Kernel support 8 and subgrids 128 / 256 through the batched path."""
+@pytest.mark.parametrize("radix", [8, 4])
+def test_other_shapes_match_oracle(ctx, sub, margin, S, radix):
+    """Kernel support 8 and subgrids 128 / 256 through the batched path, the
+    column pass's FFT in radix-8 stages (8, 8, 2 / 8, 8, 4) or radix-4 stages
+    (4, 4, 4, 2 / 4, 4, 4, 4)."""
     obs, vis = small_obs(n_st=14, n_dumps=24, arm_m=900.0, lam=5120.0)  # N = 512
     table = rgamd.make_wkernels(11, 4, S, obs.theta, -5.0, 1.0)
     ref, _, bad = orc.grid_wtowers(obs.grid_size, obs.theta, obs.u, obs.v, obs.w, vis, table,
@@ -122,7 +127,8 @@ def test_other_shapes_match_oracle(ctx, sub, margin, S):
         kt, wk = kern(table, -5.0, 1.0, dt)
         g = torch.empty((N, N), dtype=torch.complex128, device="cuda")
         ctx.grid(rgamd.params(N, obs.theta, mode=rgamd.RG_MODE_BATCHED, subgrid=sub,
-                              margin=margin, winc=WINC, vis_per_item=97), fv, wk, g)
+                              margin=margin, winc=WINC, vis_per_item=97, fft_radix=radix),
+                 fv, wk, g)
         st = ctx.stats()
         torch.cuda.synchronize()
         assert rel_max(g.cpu().numpy(), ref) < tol, (dt, sub, S)

@@ -69,27 +69,31 @@ def test_plain_fp64(ctx, c2):
     assert st.n_tasks == len(tasks) and st.n_gridded == obs.n_vis
 
 
-@pytest.mark.parametrize("band,per_item,floor_batch", [(8, 0, 0), (4, 0, 0), (8, 37, 97),
-                                                       (4, 61, 500)])
-def test_batched_fp64(ctx, c2, band, per_item, floor_batch):
+@pytest.mark.parametrize("band,per_item,floor_batch,radix", [(8, 0, 0, 0), (4, 0, 0, 0),
+                                                             (8, 37, 97, 0), (4, 61, 500, 0),
+                                                             (8, 0, 0, 4)])
+def test_batched_fp64(ctx, c2, band, per_item, floor_batch, radix):
     """FP64 bands of 8 and 4 rows (n = 512 takes 81 KB of LDS per workgroup at
     8 rows); per_item 37 / 61 split most band keys into spilled parts, and
-    floor_batch < tasks runs several batches."""
+    floor_batch < tasks runs several batches; radix 4 runs the column pass's
+    FFT in stages 4, 4, 4, 4, 2."""
     obs, vis, table, ref, _, tasks = c2
     g, st = grid(ctx, obs, vis, table, rgamd.RG_MODE_BATCHED, torch.complex128, tile=band,
-                 vis_per_item=per_item, floor_batch=floor_batch)
+                 vis_per_item=per_item, floor_batch=floor_batch, fft_radix=radix)
     assert rel_max(g, ref) < 1e-11, (band, per_item, floor_batch)
     assert st.n_tasks == len(tasks) and st.n_gridded == obs.n_vis and st.n_dropped == 0
 
 
-@pytest.mark.parametrize("band,per_item,floor_batch", [(8, 0, 0), (16, 0, 0), (4, 0, 0),
-                                                       (8, 37, 0), (16, 37, 250), (4, 53, 97)])
-def test_batched_fp32(ctx, c2, band, per_item, floor_batch):
+@pytest.mark.parametrize("band,per_item,floor_batch,radix", [
+    (8, 0, 0, 0), (16, 0, 0, 0), (4, 0, 0, 0), (8, 37, 0, 0), (16, 37, 250, 0), (4, 53, 97, 0),
+    (16, 0, 0, 4)])
+def test_batched_fp32(ctx, c2, band, per_item, floor_batch, radix):
     """FP32 bands of 8 (default), 16 and 4 rows: the image within 1e-5 of the
-    FP64 oracle's (max and RMS relative), also with split keys and batches."""
+    FP64 oracle's (max and RMS relative), also with split keys, batches and
+    radix-4 column FFTs."""
     obs, vis, table, _, img_ref, tasks = c2
     g, st = grid(ctx, obs, vis, table, rgamd.RG_MODE_BATCHED, torch.complex64, tile=band,
-                 vis_per_item=per_item, floor_batch=floor_batch)
+                 vis_per_item=per_item, floor_batch=floor_batch, fft_radix=radix)
     img = orc.grid_to_image(g)
     assert rel_max(img, img_ref) < 1e-5, (band, per_item, floor_batch)
     assert rel_rms(img, img_ref) < 1e-5, (band, per_item, floor_batch)

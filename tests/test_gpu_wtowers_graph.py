@@ -40,7 +40,11 @@ def params(obs, **kw):
                         margin=MARGIN, winc=WINC, **kw)
 
 
-@pytest.mark.parametrize("dt,tol", [(torch.complex128, 1e-12), (torch.complex64, 1e-6)])
+# replay against eager: the counting sort and the band pass's column sort
+# place records with atomics, so the order of the sums inside a cell (and the
+# FP32 rounding) changes from call to call; FP64 agrees to ~1e-15, FP32 to a
+# few 1e-7 of the grid's peak
+@pytest.mark.parametrize("dt,tol", [(torch.complex128, 1e-12), (torch.complex64, 5e-6)])
 def test_graph_replay_matches_eager_and_oracle(own_ctx, case, dt, tol):
     obs, vis, table, ref = case
     _, fv = stage(obs.u, obs.v, obs.w, vis, dt)

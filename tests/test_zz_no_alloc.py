@@ -43,8 +43,10 @@ def both(ctx, obs, vis, table, w_min, lo, hi, dt, gdt, **kw):
     return g_ref.cpu().numpy().astype(np.complex128), g_na.cpu().numpy().astype(np.complex128)
 
 
+# the counting sort places records with atomics, so the order of the sums in a
+# tile (and the FP32 rounding) can change from call to call
 @pytest.mark.parametrize("dt,gdt,tol", [(torch.complex128, torch.complex128, 1e-12),
-                                        (torch.complex64, torch.complex64, 1e-6)])
+                                        (torch.complex64, torch.complex64, 5e-6)])
 def test_two_calls_match_rg_grid(ctx, c1, dt, gdt, tol):
     obs, vis, table, w_min = c1
     n = obs.n_vis
