@@ -51,7 +51,12 @@ def profiled_traffic(name, launch_params):
 
 
 def launch_params(p):
-    return {k: int(getattr(p, k)) for k in ("tile", "vis_per_item", "floor_batch", "fft_radix")}
+    """The launch parameters a batched call runs with (defaults filled in)."""
+    towers = p.subgrid_size > 0
+    fp32_band = p.tile if p.tile in (4, 16) else 8
+    return {"tile": fp32_band if towers else (p.tile or 32),
+            "vis_per_item": p.vis_per_item or 2048, "floor_batch": int(p.floor_batch),
+            "fft_radix": (p.fft_radix or 8) if towers else 0}
 
 
 def log(*a):
@@ -84,14 +89,21 @@ def main():
     ap.add_argument("--calibrate", action="store_true", help="run a 1 GiB copy for counter scaling")
     ap.add_argument("--dft-pixels", type=int, default=65536,
                     help="seeded pixels of the FP64 DFT oracle comparison (0 = skip)")
-    ap.add_argument("--tower-band", type=int, default=0, help="W-towers band height (8 or 16)")
+    ap.add_argument("--tower-band", type=int, default=0, help="W-towers band height (4, 8 or 16)")
     ap.add_argument("--floor-batch", type=int, default=0, help="W-towers tasks per batch")
+    ap.add_argument("--vis-per-item", type=int, default=0,
+                    help="visibilities per work item (0 = default)")
+    ap.add_argument("--fft-radix", type=int, default=0,
+                    help="W-towers column-pass FFT stage radix (8 or 4, 0 = 8)")
     ap.add_argument("--no-tune", action="store_true",
                     help="FP32 batched calls with the default (or given) launch parameters "
                          "instead of the tuned ones")
+    ap.add_argument("--retune", action="store_true",
+                    help="search the launch parameters even when a cache holds them")
     ap.add_argument("--tune-cache", default=None,
                     help="tuning cache file (default $RGAMD_TUNE_CACHE or "
-                         "~/.cache/rgamd/tune.json)")
+                         "~/.cache/rgamd/tune.json; radio-gridders_amd/tuned/mi355x.json is "
+                         "read after it)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the gridding path is "
@@ -217,7 +229,8 @@ def main():
         if args.no_tune:
             return base
         t0 = time.time()
-        p, info = rgtune.tuned_params(ctx, base, fv, wk, grid, cache=args.tune_cache)
+        p, info = rgtune.tuned_params(ctx, base, fv, wk, grid, cache=args.tune_cache,
+                                      retune=args.retune, reps=5)
         r["tuning"] = {"source": info["source"], "key": info["key"], "chosen": info["chosen"],
                        "ms": info["ms"], "candidates": info["table"]}
         log(f"[bench] {label} FP32 launch parameters ({info['source']}, "
@@ -238,7 +251,8 @@ def main():
             ctx.grid(rgamd.params(N, theta, rgamd.RG_MODE_BATCHED), fv128, k128_p, g64)
         r["fp64_batched"] = timed("W-proj FP64 batched", rgamd.params(N, theta), fv128, k128_p,
                                   g64 if args.skip_plain else torch.empty_like(g64), K, W)
-        p32 = tune("W-proj", r, rgamd.params(N, theta), fv64, k64_p, g32)
+        p32 = tune("W-proj", r, rgamd.params(N, theta, vis_per_item=args.vis_per_item), fv64,
+                   k64_p, g32)
         r["launch_params"] = launch_params(p32)
         r["fp32_batched"] = timed("W-proj FP32 batched", p32, fv64, k64_p, g32, K, W)
         r["fp32_batched_graph"] = timed("W-proj FP32 batched", p32, fv64, k64_p, g32, K, W,
@@ -258,7 +272,8 @@ def main():
         r = {}
         pp = rgamd.params(N, theta, rgamd.RG_MODE_PLAIN, subgrid=SUB, margin=MARGIN, winc=WINC)
         pb = rgamd.params(N, theta, rgamd.RG_MODE_BATCHED, subgrid=SUB, margin=MARGIN, winc=WINC,
-                          tile=args.tower_band, floor_batch=args.floor_batch)
+                          tile=args.tower_band, floor_batch=args.floor_batch,
+                          vis_per_item=args.vis_per_item, fft_radix=args.fft_radix)
         gb = torch.empty_like(g64)
         r["fp64_batched"] = timed("W-towers FP64 batched", pb, fv128, k128_t, gb, K, W)
         if not args.skip_plain:
@@ -289,7 +304,7 @@ def main():
 
     if args.only in ("all", "channels") and not args.skip_channels:
         try:
-            res["channels"] = bench_channels(ctx, K, W)
+            res["channels"] = bench_channels(ctx, K, W, args.tune_cache, args.retune)
         except Exception as e:  # reported, never hides the headline
             log(f"[bench] channels workload failed: {e!r}")
             res["channels"] = {"error": repr(e)}
@@ -656,14 +671,16 @@ def bench_epic(ctx, steps, warmup, skip_cpu):
     return r
 
 
-def bench_channels(ctx, steps, warmup):
+def bench_channels(ctx, steps, warmup, tune_cache=None, retune=False):
     """256 channels (100-200 MHz) with lognormal counts (median 40 000, sigma 1)
     from 256 stations, theta 0.05 (1024^2 FP32 grid per channel): one batched
     FP32 call over all channels (eager and replayed from a captured graph)
-    against 256 plain FP64 per-channel calls; parameters from the autotuner."""
+    against 256 plain FP64 per-channel calls; launch parameters from the
+    tuner."""
     import torch
 
     import rgamd
+    import rgtune
     import synth
 
     t0 = time.time()
@@ -685,8 +702,8 @@ def bench_channels(ctx, steps, warmup):
         f"({time.time() - t0:.1f}s)")
     grids = torch.empty((nc, N, N), dtype=torch.complex64, device="cuda")
     base = rgamd.params(N, obs.theta)
-    cands = [{"tile": t, "vis_per_item": k} for t in (32, 48) for k in (1024, 2048, 4096)]
-    best, table = rgamd.autotune(ctx, base, fv64, k64, grids, cands, offsets=off, reps=2)
+    best, info = rgtune.tuned_params(ctx, base, fv64, k64, grids, offsets=off, cache=tune_cache,
+                                     retune=retune, reps=5)
     for _ in range(warmup):
         ctx.grid_channels(best, fv64, off, k64, grids)
     torch.cuda.synchronize()
@@ -743,9 +760,8 @@ def bench_channels(ctx, steps, warmup):
     out = {"n_vis": nv, "channels": nc, "grid": N, "w_planes": P,
            "counts": {"min": int(np.diff(off).min()), "median": float(np.median(np.diff(off))),
                       "max": int(np.diff(off).max())},
-           "autotune": {"chosen": {"tile": best.tile, "vis_per_item": best.vis_per_item},
-                        "ms": [{"tile": c["tile"], "vis_per_item": c["vis_per_item"], "ms": t}
-                               for c, t in table]},
+           "tuning": {"source": info["source"], "key": info["key"], "chosen": info["chosen"],
+                      "ms": info["ms"], "candidates": info["table"]},
            "fp32_batched": {"ms_per_step": ms, "mvis_s": nv / ms / 1e3},
            "fp32_batched_graph": {"ms_per_step": ms_graph, "mvis_s": nv / ms_graph / 1e3},
            "fp64_plain_per_channel": {"ms_per_step": t_plain, "mvis_s": nv / t_plain / 1e3,
@@ -753,7 +769,8 @@ def bench_channels(ctx, steps, warmup):
            "speedup_vs_fp64_plain": t_plain / ms,
            "error_fp32_batched_vs_fp64_plain": errs}
     log(f"[bench] channels: batched {ms:.3f} ms ({nv / ms / 1e3:.1f} Mvis/s), graph replay "
-        f"{ms_graph:.3f} ms, plain per channel {t_plain:.2f} ms; autotune {out['autotune']}")
+        f"{ms_graph:.3f} ms, plain per channel {t_plain:.2f} ms; launch parameters "
+        f"({info['source']}) {info['chosen']}")
     return out
 
 

@@ -13,10 +13,13 @@ of the same shape -- in this process or another -- skip the search.
 
 Key: grid size N, subgrid size n (0 for W-projection), kernel support S and
 oversampling R, visibility precision (fp32 / fp64), grid type, channel count
-and the device name.  Cache file: $RGAMD_TUNE_CACHE, else
-~/.cache/rgamd/tune.json.  Entries that no longer parse are ignored (and
-replaced on the next write); writes go through a temporary file and an
-atomic rename, merged with what another process may have written meanwhile.
+and the device (name, ISA, compute units).  Lookup order: the user's cache
+file ($RGAMD_TUNE_CACHE, else ~/.cache/rgamd/tune.json), then the table
+shipped beside this module (tuned/mi355x.json: searches made on an MI355X for
+the shapes of the benchmark workloads), then a search whose result goes to
+the user's file.  Entries that no longer parse are ignored (and replaced on
+the next write); writes go through a temporary file and an atomic rename,
+merged with what another process may have written meanwhile.
 """
 from __future__ import annotations
 
@@ -29,6 +32,7 @@ import warnings
 import rgamd
 
 KNOBS = ("tile", "vis_per_item", "floor_batch", "fft_radix")
+PACKAGED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned", "mi355x.json")
 
 
 def cache_path() -> str:
@@ -134,25 +138,28 @@ def coordinate_search(space: dict, measure, sweeps: int = 2):
 def tuned_params(ctx: rgamd.Context, base: rgamd.GridParams, fv: rgamd.FlatVis,
                  wk: rgamd.WKern, grid, offsets=None, cache: str | None = None,
                  space: dict | None = None, reps: int = 3, retune: bool = False,
-                 device: str | None = None, measure=None):
+                 device: str | None = None, measure=None, packaged: str | None = PACKAGED):
     """GridParams for `base`'s call on this data: the cached choice for the
-    call's key, else the result of a search (which overwrites `grid`) that is
-    then cached.  Returns (params, info) with info = {"key", "source"
-    ("cache" | "measured"), "chosen", "ms", "table", "path"}.  `measure`
-    replaces the device timing (a function of a candidate dict returning ms);
-    `space` replaces search_space()."""
+    call's key (user cache, then the shipped table `packaged`), else the
+    result of a search (which overwrites `grid`) that is then cached.
+    retune=True searches whatever the caches hold.  Returns (params, info)
+    with info = {"key", "source" ("cache" | "packaged" | "measured"),
+    "chosen", "ms", "table", "path"}.  `measure` replaces the device timing
+    (a function of a candidate dict returning ms); `space` replaces
+    search_space()."""
     path = cache or cache_path()
     n_chan = 1 if offsets is None else len(offsets) - 1
     if device is None:
         device = device_name(getattr(ctx, "device", 0))
     key = tune_key(base, fv, wk, rgamd.grid_type_of(grid), n_chan, device)
     if not retune:
-        hit = _load(path).get(key)
-        if isinstance(hit, dict) and isinstance(hit.get("params"), dict):
-            chosen = {k: int(v) for k, v in hit["params"].items() if k in KNOBS}
-            return _with(base, chosen), {"key": key, "source": "cache", "chosen": chosen,
-                                         "ms": hit.get("ms"), "table": hit.get("table", []),
-                                         "path": path}
+        for source, f in (("cache", path), ("packaged", packaged)):
+            hit = _load(f).get(key) if f else None
+            if isinstance(hit, dict) and isinstance(hit.get("params"), dict):
+                chosen = {k: int(v) for k, v in hit["params"].items() if k in KNOBS}
+                return _with(base, chosen), {"key": key, "source": source, "chosen": chosen,
+                                             "ms": hit.get("ms"),
+                                             "table": hit.get("table", []), "path": f}
 
     def run(p):
         if offsets is None:

@@ -32,12 +32,12 @@ def test_wtowers_tuned_cached_and_correct(ctx, case, tmp_path):
     N = obs.grid_size
     g = torch.empty((N, N), dtype=torch.complex128, device="cuda")
     base = rgamd.params(N, obs.theta, subgrid=64, margin=16, winc=10.0)
-    p, info = rgtune.tuned_params(ctx, base, fv, wk, g, cache=path, reps=1)
+    p, info = rgtune.tuned_params(ctx, base, fv, wk, g, cache=path, reps=1, packaged=None)
     assert info["source"] == "measured"
     tried = {(e["tile"], e["fft_radix"]) for e in info["table"]}
     assert {(8, 8), (4, 8), (16, 8)} <= tried and any(r == 4 for _, r in tried)
     assert json.load(open(path))[info["key"]]["params"] == info["chosen"]
-    p2, info2 = rgtune.tuned_params(ctx, base, fv, wk, g, cache=path)
+    p2, info2 = rgtune.tuned_params(ctx, base, fv, wk, g, cache=path, packaged=None)
     assert info2["source"] == "cache" and info2["chosen"] == info["chosen"]
     ctx.grid(p2, fv, wk, g)
     torch.cuda.synchronize()
@@ -53,8 +53,8 @@ def test_wproj_tuned_and_cached(ctx, case, tmp_path):
     N = obs.grid_size
     g = torch.empty((N, N), dtype=torch.complex64, device="cuda")
     base = rgamd.params(N, obs.theta)
-    p, info = rgtune.tuned_params(ctx, base, fv, wk, g, cache=path, reps=1)
+    p, info = rgtune.tuned_params(ctx, base, fv, wk, g, cache=path, reps=1, packaged=None)
     assert info["source"] == "measured" and "n=0" in info["key"] and "grid=c64" in info["key"]
     assert p.tile in (32, 48) and p.vis_per_item in (1024, 2048, 4096)
-    _, info2 = rgtune.tuned_params(ctx, base, fv, wk, g, cache=path)
+    _, info2 = rgtune.tuned_params(ctx, base, fv, wk, g, cache=path, packaged=None)
     assert info2["source"] == "cache"

@@ -110,3 +110,29 @@ def test_unreadable_cache_is_replaced(tmp_path, monkeypatch):
     assert info["source"] == "measured" and info["path"] == str(path)
     assert list(json.load(open(path))) == [info["key"]]
     assert not [f for f in os.listdir(tmp_path) if f.startswith(".tune-")]
+
+
+def test_shipped_table_after_user_cache(tmp_path):
+    fv, wk = descs()
+    grid = torch.empty((2, 2), dtype=torch.complex64)
+    key = rgtune.tune_key(base(), fv, wk, rgamd.RG_C64, 1, "devA")
+    shipped = tmp_path / "shipped.json"
+    shipped.write_text(json.dumps({key: {"params": {"tile": 4, "vis_per_item": 1024,
+                                                    "floor_batch": 0, "fft_radix": 8},
+                                         "ms": 5.0}}))
+    user = str(tmp_path / "user.json")
+
+    def never(c):
+        raise AssertionError("no search while a table holds the key")
+
+    p, info = rgtune.tuned_params(None, base(), fv, wk, grid, cache=user, device="devA",
+                                  measure=never, packaged=str(shipped))
+    assert info["source"] == "packaged" and (p.tile, p.vis_per_item) == (4, 1024)
+    assert not os.path.exists(user)  # a table hit writes nothing
+    # a search (retune) goes to the user's file, which then wins over the table
+    _, info2 = rgtune.tuned_params(None, base(), fv, wk, grid, cache=user, device="devA",
+                                   measure=cost, packaged=str(shipped), retune=True)
+    assert info2["source"] == "measured" and info2["chosen"]["tile"] == 16
+    p3, info3 = rgtune.tuned_params(None, base(), fv, wk, grid, cache=user, device="devA",
+                                    measure=never, packaged=str(shipped))
+    assert info3["source"] == "cache" and p3.tile == 16
