@@ -86,7 +86,7 @@ def main():
                     help="also run the stress workload (~260 M visibilities, 4096^2, W-towers "
                          "subgrid 256); adds about a minute")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--calibrate", action="store_true", help="run a 1 GiB copy for counter scaling")
+    ap.add_argument("--calibrate", action="store_true", help="run a 1 GiB fill and a 1 GiB sum for counter scaling")
     ap.add_argument("--dft-pixels", type=int, default=65536,
                     help="seeded pixels of the FP64 DFT oracle comparison (0 = skip)")
     ap.add_argument("--tower-band", type=int, default=0, help="W-towers band height (4, 8 or 16)")
@@ -98,6 +98,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true",
                     help="FP32 batched calls with the default (or given) launch parameters "
                          "instead of the tuned ones")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the graph-replay runs (headline from the eager calls)")
     ap.add_argument("--retune", action="store_true",
                     help="search the launch parameters even when a cache holds them")
     ap.add_argument("--tune-cache", default=None,
@@ -148,13 +150,13 @@ def main():
     ctx = rgamd.Context(0)
     stream = torch.cuda.current_stream()
     if args.calibrate:
-        # a plain 1 GiB device copy: the reference dispatch for scaling the
-        # FETCH_SIZE / WRITE_SIZE counters of the gridding kernels
+        # dispatches of known traffic for scaling the FETCH_SIZE / WRITE_SIZE
+        # counters of the gridding kernels: a fill that writes 1 GiB and a
+        # sum that reads it
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        total = float(src.sum())
+        assert total == float(1 << 28)
+        del src
 
     def timed(label, p, fv, wk, grid, steps, warmup, graph=False):
         """ms per step of `steps` calls (kernel timers off), then the per-kernel
@@ -255,8 +257,9 @@ def main():
                    k64_p, g32)
         r["launch_params"] = launch_params(p32)
         r["fp32_batched"] = timed("W-proj FP32 batched", p32, fv64, k64_p, g32, K, W)
-        r["fp32_batched_graph"] = timed("W-proj FP32 batched", p32, fv64, k64_p, g32, K, W,
-                                        graph=True)
+        if not args.no_graph:
+            r["fp32_batched_graph"] = timed("W-proj FP32 batched", p32, fv64, k64_p, g32, K, W,
+                                            graph=True)
         if not args.skip_plain:
             ctx.grid(rgamd.params(N, theta, rgamd.RG_MODE_PLAIN), fv128, k128_p, g64)
         i32, i64 = image(g32), image(g64)
@@ -283,8 +286,9 @@ def main():
         p32 = tune("W-towers", r, pb, fv64, k64_t, g32)
         r["launch_params"] = launch_params(p32)
         r["fp32_batched"] = timed("W-towers FP32 batched", p32, fv64, k64_t, g32, K, W)
-        r["fp32_batched_graph"] = timed("W-towers FP32 batched", p32, fv64, k64_t, g32, K, W,
-                                        graph=True)
+        if not args.no_graph:
+            r["fp32_batched_graph"] = timed("W-towers FP32 batched", p32, fv64, k64_t, g32, K, W,
+                                            graph=True)
         i32, i64, ib = image(g32), image(g64), image(gb)
         r["error_fp32_batched_vs_fp64"] = errs(i32, i64)
         r["error_fp64_batched_vs_fp64"] = errs(ib, i64)

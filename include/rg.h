@@ -146,11 +146,12 @@ int rg_context_thaw(rg_context *ctx);
 #define RG_TIMER_WPROJ_TILES 0  /* batched W-projection tile kernel */
 #define RG_TIMER_WPROJ_PLAIN 1  /* plain W-projection scatter */
 #define RG_TIMER_BINNING 2      /* count + scan + scatter of the sort */
-#define RG_TIMER_TOWER_TILES 3  /* batched W-towers band pass: subgrid scatter + row IFFT */
+#define RG_TIMER_TOWER_TILES 3  /* batched W-towers band pass (k_tower_band_items): scatter + row IFFT */
 #define RG_TIMER_TOWER_REDUCE 4 /* batched W-towers sum of partial images into subimages */
 #define RG_TIMER_TOWER_COLS 5   /* batched W-towers column IFFT + screen pass */
 #define RG_TIMER_TOWER_FINISH 6 /* W-towers forward FFTs + fold */
-#define RG_TIMER__COUNT 7
+#define RG_TIMER_TOWER_COMBINE 7 /* batched W-towers sum of bands split over work items */
+#define RG_TIMER__COUNT 8
 int rg_context_set_timing(rg_context *ctx, int32_t enable);
 int rg_context_timer(rg_context *ctx, int32_t which, double *total_ms, int64_t *launches);
 

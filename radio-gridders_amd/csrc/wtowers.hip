@@ -1298,6 +1298,7 @@ struct BandLaunch {
     uint32_t *spill_ctr, *item_spill;
     unsigned long long *stats;
     uint32_t blocks; // workgroups (each loops over the batch's items)
+    rg_context *timer_ctx;
 };
 
 template <typename T, int S, int H, int LOGN>
@@ -1305,11 +1306,16 @@ static int launch_bands_h(const BandLaunch &a, hipStream_t s)
 {
     using T2 = typename Vec2<T>::type;
     constexpr int GRP = S * H >= 256 ? 1 : 256 / (S * H);
-    hipLaunchKernelGGL((k_tower_band_items<T, S, H, LOGN, GRP>), dim3(a.blocks), dim3(S * H * GRP),
-                       0, s, (const BandRec<T> *)a.recs, a.items, a.item_start, a.bref, a.batch,
-                       a.rank_of_slot, (const T2 *)a.kern, a.kern_bytes, (const T2 *)a.tw,
-                       (T2 *)a.G, (T2 *)a.spill, a.spill_cap, a.spill_ctr, a.item_spill, a.stats);
-    RG_HIP(hipGetLastError());
+    {
+        KernelTimer kt(a.timer_ctx, RG_TIMER_TOWER_TILES, s);
+        hipLaunchKernelGGL((k_tower_band_items<T, S, H, LOGN, GRP>), dim3(a.blocks),
+                           dim3(S * H * GRP), 0, s, (const BandRec<T> *)a.recs, a.items,
+                           a.item_start, a.bref, a.batch, a.rank_of_slot, (const T2 *)a.kern,
+                           a.kern_bytes, (const T2 *)a.tw, (T2 *)a.G, (T2 *)a.spill, a.spill_cap,
+                           a.spill_ctr, a.item_spill, a.stats);
+        RG_HIP(hipGetLastError());
+    }
+    KernelTimer kt(a.timer_ctx, RG_TIMER_TOWER_COMBINE, s);
     hipLaunchKernelGGL((k_band_combine<T2, H, LOGN>), dim3(a.blocks), dim3(256), 0, s, a.items,
                        a.item_start, a.bref, a.batch, a.rank_of_slot, (T2 *)a.G,
                        (const T2 *)a.spill, a.spill_cap, (const uint32_t *)a.item_spill);
@@ -1851,6 +1857,7 @@ static int wtowers_batched_t(rg_context *ctx, const rg_grid_params *p, const rg_
     }
 
     BandLaunch bl;
+    bl.timer_ctx = ctx;
     bl.recs = sorted;
     bl.items = items_w;
     bl.item_start = item_start_w;
@@ -1920,13 +1927,10 @@ static int wtowers_batched_t(rg_context *ctx, const rg_grid_params *p, const rg_
         bl.batch = b;
         bl.spill_ctr = spill_ctr + b;
         cl.batch = b;
-        {
-            KernelTimer kt(ctx, RG_TIMER_TOWER_TILES, s);
-            if (S == 16)
-                RG_TRY((launch_bands<T, 16>(n, H, bl, s)));
-            else
-                RG_TRY((launch_bands<T, 8>(n, H, bl, s)));
-        }
+        if (S == 16)
+            RG_TRY((launch_bands<T, 16>(n, H, bl, s)));
+        else
+            RG_TRY((launch_bands<T, 8>(n, H, bl, s)));
         {
             KernelTimer kt(ctx, RG_TIMER_TOWER_COLS, s);
             RG_TRY(launch_cols<T>(n, H, cl, s));

@@ -34,7 +34,7 @@ RG_IMAGE_GPU = 0
 RG_IMAGE_CPU = 1
 
 TIMERS = {"wproj_tiles": 0, "wproj_plain": 1, "binning": 2, "tower_tiles": 3, "tower_reduce": 4,
-          "tower_cols": 5, "tower_finish": 6}
+          "tower_cols": 5, "tower_finish": 6, "tower_combine": 7}
 
 
 class RGError(RuntimeError):

@@ -13,7 +13,7 @@ of the same shape -- in this process or another -- skip the search.
 
 Key: grid size N, subgrid size n (0 for W-projection), kernel support S and
 oversampling R, visibility precision (fp32 / fp64), grid type, channel count
-and the device (name, ISA, compute units).  Lookup order: the user's cache
+and the device (ISA, compute units).  Lookup order: the user's cache
 file ($RGAMD_TUNE_CACHE, else ~/.cache/rgamd/tune.json), then the table
 shipped beside this module (tuned/mi355x.json: searches made on an MI355X for
 the shapes of the benchmark workloads), then a search whose result goes to
@@ -41,12 +41,12 @@ def cache_path() -> str:
 
 
 def device_name(device: int = 0) -> str:
-    """Marketing name, ISA and compute-unit count (ROCm's names alone can be
-    generic)."""
+    """ISA and compute-unit count (the marketing name that ROCm reports can be
+    generic or empty, depending on the environment)."""
     import torch
     pr = torch.cuda.get_device_properties(device)
-    arch = getattr(pr, "gcnArchName", "")
-    return f"{pr.name} {arch} {pr.multi_processor_count}CU".replace("  ", " ")
+    arch = getattr(pr, "gcnArchName", "") or pr.name
+    return f"{arch} {pr.multi_processor_count}CU"
 
 
 def tune_key(p: rgamd.GridParams, fv: rgamd.FlatVis, wk: rgamd.WKern, grid_type: int,
