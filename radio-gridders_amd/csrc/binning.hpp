@@ -10,11 +10,11 @@
 //             every replica's share of every key
 //   items   : one work item per K records of a key
 //   scatter : record i written at start[rep][key] + rank (no atomics)
-// Block b uses counter replica rep = b mod NREP.  Device-scope atomics on one
-// address serialise, and a few keys (the short-baseline tiles at the uv
-// centre) take thousands of wavefront updates, so the counters of a key are
-// spread over NREP separate arrays; replica rep's records of a key follow
-// those of replicas 0..rep-1.
+// Chunk c of BIN_CHUNK visibilities uses counter replica rep = c mod NREP.
+// Device-scope atomics on one address serialise, and a few keys (the
+// short-baseline tiles at the uv centre) take thousands of wavefront updates,
+// so the counters of a key are spread over NREP separate arrays; replica
+// rep's records of a key follow those of replicas 0..rep-1.
 // Consecutive visibilities of a wavefront that share a key keep their stream
 // order, so baseline-track runs stay contiguous inside a key (the locality
 // the W-projection Romein accumulation relies on).
@@ -48,15 +48,18 @@ template <typename T> struct Rec {
 // and as an overrun of the call)
 enum VisStatus { VIS_OK = 0, VIS_DROPPED = 1, VIS_BAD_PLANE = 2, VIS_OVERFLOW = 3 };
 
-// Returns, for the calling lane, old counter value + rank inside its run (when
-// `want` is set).  A run is a maximal stretch of consecutive active lanes with
-// the same key; its first lane adds the run length to the key's counter with
-// one atomic.  All runs of the wavefront issue their atomics in one
-// instruction and the results are gathered once, so a wavefront costs one
-// atomic round trip whatever its keys (visibility streams are baseline-track
-// ordered, so neighbouring lanes mostly share keys).  Must be called by the
-// whole wavefront in uniform control flow.
-__device__ inline uint32_t wave_agg_add(uint32_t *counters, uint32_t key, bool active, bool want)
+// A run is a maximal stretch of consecutive active lanes with the same key;
+// its first lane (the head) adds the run length to the key's counter with one
+// atomic.  All runs of the wavefront issue their atomics in one instruction
+// and the results are gathered once, so a wavefront costs one atomic round
+// trip whatever its keys (visibility streams are baseline-track ordered, so
+// neighbouring lanes often share keys).  wave_agg_issue returns the atomic's
+// old value (meaningful on head lanes) and the lane's head; wave_agg_rank
+// turns them into the lane's rank.  Splitting the two lets a thread keep
+// several atomics in flight.  Must be called by the whole wavefront in
+// uniform control flow.
+__device__ inline uint32_t wave_agg_issue(uint32_t *counters, uint32_t key, bool active,
+                                          int &my_head)
 {
     const int lane = __lane_id();
     const unsigned long long act = __ballot(active);
@@ -70,12 +73,23 @@ __device__ inline uint32_t wave_agg_add(uint32_t *counters, uint32_t key, bool a
         const uint32_t len = nxt ? (uint32_t)__ffsll((long long)nxt) : (uint32_t)(64 - lane);
         base = atomicAdd(&counters[key], len);
     }
-    if (!want)
-        return 0;
     const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
     const unsigned long long hm = heads & upto;
-    const int my_head = hm ? 63 - __clzll((long long)hm) : 0;
-    return (uint32_t)__shfl((int)base, my_head) + (uint32_t)(lane - my_head);
+    my_head = hm ? 63 - __clzll((long long)hm) : 0;
+    return base;
+}
+
+__device__ inline uint32_t wave_agg_rank(uint32_t base, int my_head)
+{
+    return (uint32_t)__shfl((int)base, my_head) + (uint32_t)(__lane_id() - my_head);
+}
+
+// Both steps; the rank only when `want` (uniform) is set.
+__device__ inline uint32_t wave_agg_add(uint32_t *counters, uint32_t key, bool active, bool want)
+{
+    int my_head;
+    const uint32_t base = wave_agg_issue(counters, key, active, my_head);
+    return want ? wave_agg_rank(base, my_head) : 0u;
 }
 
 // Counter replicas: NREP arrays of `stride` entries (stride >= nkeys + 1).
@@ -88,53 +102,86 @@ struct Replicas {
 // packed position, tap index.
 constexpr uint32_t KEY_NONE = 0xffffffffu;
 
+// Count and scatter workgroups take BIN_CHUNK consecutive visibilities,
+// BIN_VPT per thread (visibility chunk + q BIN_TB + thread, so the lanes of a
+// wavefront hold consecutive visibilities); chunk c uses counter replica
+// c mod NREP.
+constexpr int BIN_TB = 256;
+constexpr int BIN_VPT = 4;
+constexpr int BIN_CHUNK = BIN_TB * BIN_VPT;
+
 template <typename Keyer>
-__global__ __launch_bounds__(256) void k_bin_count(Keyer kf, int64_t n, uint32_t *count,
-                                                   Replicas rp, uint4 *krp,
-                                                   unsigned long long *stats)
+__global__ __launch_bounds__(BIN_TB) void k_bin_count(Keyer kf, int64_t n, uint32_t *count,
+                                                      Replicas rp, uint4 *krp,
+                                                      unsigned long long *stats)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t key = 0, pos = 0, kidx = 0;
-    int st = VIS_DROPPED;
-    if (i < n)
-        st = kf.classify(i, key, pos, kidx);
-    const bool ok = i < n && st == VIS_OK;
-    const uint32_t rank =
-        wave_agg_add(count + (blockIdx.x & rp.mask) * rp.stride, key, ok, true);
-    if (i < n)
-        krp[i] = make_uint4(ok ? key : KEY_NONE, rank, pos, kidx);
-    // per-wave status counts
-    const unsigned long long dropped =
-        __ballot(i < n && (st == VIS_DROPPED || st == VIS_OVERFLOW));
-    const unsigned long long badp = __ballot(i < n && st == VIS_BAD_PLANE);
-    const unsigned long long over = __ballot(i < n && st == VIS_OVERFLOW);
+    const int64_t i0 = (int64_t)blockIdx.x * BIN_CHUNK + threadIdx.x;
+    uint32_t *cnt = count + (blockIdx.x & rp.mask) * rp.stride;
+    uint32_t key[BIN_VPT], pos[BIN_VPT], kidx[BIN_VPT], base[BIN_VPT];
+    int st[BIN_VPT], head[BIN_VPT];
+#pragma unroll
+    for (int q = 0; q < BIN_VPT; ++q) {
+        const int64_t i = i0 + q * BIN_TB;
+        key[q] = pos[q] = kidx[q] = 0;
+        st[q] = i < n ? kf.classify(i, key[q], pos[q], kidx[q]) : VIS_DROPPED;
+    }
+    // every atomic in flight before the first result is used
+#pragma unroll
+    for (int q = 0; q < BIN_VPT; ++q)
+        base[q] = wave_agg_issue(cnt, key[q], i0 + q * BIN_TB < n && st[q] == VIS_OK, head[q]);
+    uint32_t dropped = 0, badp = 0, over = 0;
+#pragma unroll
+    for (int q = 0; q < BIN_VPT; ++q) {
+        const int64_t i = i0 + q * BIN_TB;
+        const bool in = i < n, ok = in && st[q] == VIS_OK;
+        const uint32_t rank = wave_agg_rank(base[q], head[q]);
+        if (in)
+            krp[i] = make_uint4(ok ? key[q] : KEY_NONE, rank, pos[q], kidx[q]);
+        // per-wave status counts
+        dropped += (uint32_t)__popcll(__ballot(in && (st[q] == VIS_DROPPED || st[q] == VIS_OVERFLOW)));
+        badp += (uint32_t)__popcll(__ballot(in && st[q] == VIS_BAD_PLANE));
+        over += (uint32_t)__popcll(__ballot(in && st[q] == VIS_OVERFLOW));
+    }
     if (__lane_id() == 0) {
         if (dropped)
-            atomicAdd(&stats[CNT_DROPPED], (unsigned long long)__popcll(dropped));
+            atomicAdd(&stats[CNT_DROPPED], (unsigned long long)dropped);
         if (badp)
-            atomicAdd(&stats[CNT_BAD_PLANE], (unsigned long long)__popcll(badp));
+            atomicAdd(&stats[CNT_BAD_PLANE], (unsigned long long)badp);
         if (over)
-            atomicAdd(&stats[CNT_OVERFLOW], (unsigned long long)__popcll(over));
+            atomicAdd(&stats[CNT_OVERFLOW], (unsigned long long)over);
     }
 }
 
-// Same grid as k_bin_count (block b, replica b mod NREP).
+// Same grid and chunks as k_bin_count.
 template <typename Keyer, typename T>
-__global__ __launch_bounds__(256) void k_bin_scatter(Keyer kf, int64_t n, const uint32_t *start,
-                                                     Replicas rp, const uint4 *__restrict__ krp,
-                                                     Rec<T> *recs)
+__global__ __launch_bounds__(BIN_TB) void k_bin_scatter(Keyer kf, int64_t n,
+                                                        const uint32_t *start, Replicas rp,
+                                                        const uint4 *__restrict__ krp,
+                                                        Rec<T> *recs)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    const uint4 e = krp[i];
-    if (e.x == KEY_NONE)
-        return;
-    Rec<T> r;
-    r.pos = e.z;
-    r.kidx = e.w;
-    r.vis = kf.template vis_value<T>(i);
-    recs[start[(blockIdx.x & rp.mask) * rp.stride + e.x] + e.y] = r;
+    const int64_t i0 = (int64_t)blockIdx.x * BIN_CHUNK + threadIdx.x;
+    const uint32_t *st = start + (blockIdx.x & rp.mask) * rp.stride;
+    uint4 e[BIN_VPT];
+    typename Vec2<T>::type v[BIN_VPT];
+#pragma unroll
+    for (int q = 0; q < BIN_VPT; ++q) {
+        const int64_t i = i0 + q * BIN_TB;
+        e[q] = make_uint4(KEY_NONE, 0, 0, 0);
+        if (i < n) {
+            e[q] = krp[i];
+            v[q] = kf.template vis_value<T>(i);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < BIN_VPT; ++q) {
+        if (e[q].x == KEY_NONE)
+            continue;
+        Rec<T> r;
+        r.pos = e[q].z;
+        r.kidx = e[q].w;
+        r.vis = v[q];
+        recs[st[e[q].x] + e[q].y] = r;
+    }
 }
 
 // Phase 1: per-block totals of (count, items).

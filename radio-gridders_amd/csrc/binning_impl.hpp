@@ -14,11 +14,10 @@ int bin_run(const Keyer &kf, int64_t n, int64_t nkeys, uint32_t per_item, const 
     KernelTimer kt(timer_ctx, RG_TIMER_BINNING, s);
     RG_TRY(zero_fill(b.count, pl.bytes[BIN_COUNT], s));
 
-    const int tb = 256;
-    const int64_t nb = (n + tb - 1) / tb;
+    const int64_t nb = (n + BIN_CHUNK - 1) / BIN_CHUNK;
     if (nb > 0)
-        hipLaunchKernelGGL((k_bin_count<Keyer>), dim3((unsigned)nb), dim3(tb), 0, s, kf, n, b.count,
-                           rp, b.krp, b.stats);
+        hipLaunchKernelGGL((k_bin_count<Keyer>), dim3((unsigned)nb), dim3(BIN_TB), 0, s, kf, n,
+                           b.count, rp, b.krp, b.stats);
     RG_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)pl.nblocks_scan), dim3(SCAN_BLOCK), 0, s,
                        (const uint32_t *)b.count, rp, nkeys, per_item, b.bsums);
@@ -29,7 +28,7 @@ int bin_run(const Keyer &kf, int64_t n, int64_t nkeys, uint32_t per_item, const 
                        b.vis_start, b.item_start, b.cursor, b.items);
     RG_HIP(hipGetLastError());
     if (nb > 0)
-        hipLaunchKernelGGL((k_bin_scatter<Keyer, T>), dim3((unsigned)nb), dim3(tb), 0, s, kf, n,
+        hipLaunchKernelGGL((k_bin_scatter<Keyer, T>), dim3((unsigned)nb), dim3(BIN_TB), 0, s, kf, n,
                            (const uint32_t *)b.cursor, rp, (const uint4 *)b.krp,
                            (Rec<T> *)b.recs);
     RG_HIP(hipGetLastError());

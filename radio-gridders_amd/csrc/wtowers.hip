@@ -113,39 +113,59 @@ struct TowerKeyer {
     }
 };
 
-// min / max floor index over visibilities that fall in the tower set
+// min / max floor index over visibilities that fall in the tower set; four
+// visibilities per thread and pass (their loads issued together), wavefront
+// then workgroup reduction, one atomic pair per workgroup
+constexpr int RANGE_VPT = 4;
+
 __global__ __launch_bounds__(256) void k_floor_range(const double *u, const double *v,
                                                      const double *w, int64_t n, double theta,
                                                      int N, int chunk, int cc, double winc,
                                                      int *range)
 {
-    __shared__ int smin[256], smax[256];
+    __shared__ int smin[4], smax[4];
     int lo = INT_MAX, hi = INT_MIN;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int cx = tower_index(u[i], theta, N, chunk);
-        const int cy = tower_index(v[i], theta, N, chunk);
-        if (cx < 0 || cx >= cc || cy < 0 || cy >= cc)
-            continue;
-        const int wp = floor_index(w[i], winc);
-        lo = min(lo, wp);
-        hi = max(hi, wp);
-    }
-    smin[threadIdx.x] = lo;
-    smax[threadIdx.x] = hi;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            smin[threadIdx.x] = min(smin[threadIdx.x], smin[threadIdx.x + s]);
-            smax[threadIdx.x] = max(smax[threadIdx.x], smax[threadIdx.x + s]);
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i0 < n;
+         i0 += RANGE_VPT * stride) {
+        double uu[RANGE_VPT], vv[RANGE_VPT], ww[RANGE_VPT];
+#pragma unroll
+        for (int q = 0; q < RANGE_VPT; ++q) {
+            const int64_t i = i0 + q * stride < n ? i0 + q * stride : i0;
+            uu[q] = u[i];
+            vv[q] = v[i];
+            ww[q] = w[i];
         }
-        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < RANGE_VPT; ++q) {
+            const int cx = tower_index(uu[q], theta, N, chunk);
+            const int cy = tower_index(vv[q], theta, N, chunk);
+            if (cx < 0 || cx >= cc || cy < 0 || cy >= cc)
+                continue;
+            const int wp = floor_index(ww[q], winc);
+            lo = min(lo, wp);
+            hi = max(hi, wp);
+        }
     }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, __shfl_xor(lo, o));
+        hi = max(hi, __shfl_xor(hi, o));
+    }
+    const int wv = threadIdx.x / 64;
+    if (__lane_id() == 0) {
+        smin[wv] = lo;
+        smax[wv] = hi;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        if (smin[0] != INT_MAX)
-            atomicMin(&range[0], smin[0]);
-        if (smax[0] != INT_MIN)
-            atomicMax(&range[1], smax[0]);
+        for (int k = 1; k < (int)blockDim.x / 64; ++k) {
+            lo = min(lo, smin[k]);
+            hi = max(hi, smax[k]);
+        }
+        if (lo != INT_MAX)
+            atomicMin(&range[0], lo);
+        if (hi != INT_MIN)
+            atomicMax(&range[1], hi);
     }
 }
 
@@ -1463,9 +1483,9 @@ static int floor_range(const rg_grid_params *p, const rg_flat_vis *vis, const Wt
     hipLaunchKernelGGL(k_range_init, dim3(1), dim3(1), 0, s, d_range);
     // few workgroups: each ends with one atomicMin / atomicMax on the same two
     // words, which serialise
-    int64_t blocks = (vis->n + 255) / 256;
-    if (blocks > 1024)
-        blocks = 1024;
+    int64_t blocks = (vis->n + 256 * RANGE_VPT - 1) / (256 * RANGE_VPT);
+    if (blocks > 2048)
+        blocks = 2048;
     if (blocks > 0)
         hipLaunchKernelGGL(k_floor_range, dim3((unsigned)blocks), dim3(256), 0, s, vis->u, vis->v,
                            vis->w, vis->n, p->theta, g.N, g.chunk, g.cc, p->winc, d_range);
