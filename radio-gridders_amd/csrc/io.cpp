@@ -424,7 +424,11 @@ static int load_wkern(const char *filename, double theta, HostKern &k)
             if ((int)dims[0] == k.R && (int)dims[1] == k.R && (int)dims[2] == k.S &&
                 (int)dims[3] == k.S) {
                 std::vector<double> d((size_t)k.R * k.R * k.S * k.S * 2);
-                a->dread(ds, a->cpx, S_ALL, S_ALL, P_DEFAULT, d.data());
+                if (a->dread(ds, a->cpx, S_ALL, S_ALL, P_DEFAULT, d.data()) < 0) {
+                    status = fail(RG_ERR_IO, "cannot read %s/%s", gname, dn.c_str());
+                    a->dclose(ds);
+                    break;
+                }
                 ws.push_back(w);
                 planes.push_back(std::move(d));
             } else {
