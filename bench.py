@@ -578,8 +578,16 @@ def bench_wstack(ctx, steps, warmup, skip_cpu, cpu_seconds):
     ref = orc.predict_quantized(l, m, None, theta, lam, us, vs, ws)
     err = np.abs(out[sel].cpu().numpy() - ref)
     taps = kw.size * ku.size * ku.size
+    # compulsory bytes of the call (sort + degridding) per visibility: uvw in,
+    # the 16-byte record written and read at both sort levels and read by the
+    # degridding, the predicted visibility out; the stack regions each work
+    # item stages are not charged.  The per-tap count (every tap at 16 B, as
+    # the visibility's 4136 algorithmic bytes) is kept as `effective`: the
+    # taps come from LDS, so it is no floor on HBM traffic.
+    comp_vis = 24 + 4 * 16 + 16
     per_vis = taps * 16 + 24 + 16
-    ach = nv * per_vis / (degrid_ms * 1e-3) / 1e9
+    ach = nv * comp_vis / (degrid_ms * 1e-3) / 1e9
+    ach_eff = nv * per_vis / (degrid_ms * 1e-3) / 1e9
     r = {"n_vis": nv, "planes": wp, "plane": og, "sources": len(l), "skipped_sources": skipped,
          "not_predicted": int(st.n_dropped), "build_ms": build_ms, "degrid_ms": degrid_ms,
          "degrid_mvis_s": nv / (degrid_ms * 1e-3) / 1e6,
@@ -588,9 +596,11 @@ def bench_wstack(ctx, steps, warmup, skip_cpu, cpu_seconds):
                                     "per_source_mean": float(err.mean() / len(l))},
          "roofline": {"kernel": "k_ws_degrid_tile with the visibility sort (one call)",
                       "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                      "frac": ach / HBM_PEAK_GBS, "bytes_per_unit": {"visibility": per_vis},
+                      "frac": ach / HBM_PEAK_GBS, "bytes_per_unit": {"visibility": comp_vis},
                       "traffic": None,
-                      "note": "taps served from LDS count as algorithmic bytes"}}
+                      "effective": {"GBps": ach_eff, "bytes_per_unit": {"visibility": per_vis},
+                                    "note": "every tap charged at 16 B; taps are read from "
+                                            "LDS"}}}
     log(f"[bench] W-stack predict: {og}^2 x {wp} planes, build {build_ms:.2f} ms, degrid "
         f"{degrid_ms:.2f} ms ({r['degrid_mvis_s']:.0f} Mvis/s), mean error per source "
         f"{r['error_vs_quantised_dft']['per_source_mean']:.2e}")
