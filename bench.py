@@ -289,13 +289,22 @@ def main():
         if not args.no_graph:
             r["fp32_batched_graph"] = timed("W-towers FP32 batched", p32, fv64, k64_t, g32, K, W,
                                             graph=True)
-        i32, i64, ib = image(g32), image(g64), image(gb)
+        # the precision form with two-sum compensated register sums in the band lanes
+        pc = rgamd.GridParams.from_buffer_copy(p32)
+        pc.compensated = 1
+        gc = torch.empty_like(g64)
+        r["fp32_batched_compensated"] = timed("W-towers FP32 batched compensated", pc, fv64,
+                                              k64_t, gc, K, W)
+        i32, i64, ib, ic = image(g32), image(g64), image(gb), image(gc)
+        del gc
         r["error_fp32_batched_vs_fp64"] = errs(i32, i64)
+        r["error_fp32_compensated_vs_fp64"] = errs(ic, i64)
         r["error_fp64_batched_vs_fp64"] = errs(ib, i64)
         r["speedup_fp32_vs_fp64_batched"] = (r["fp64_batched"]["ms_per_step"] /
                                             r["fp32_batched"]["ms_per_step"])
         dft_images["wtowers_fp64" + ("_plain" if not args.skip_plain else "_batched")] = i64
         dft_images["wtowers_fp32_batched"] = i32
+        dft_images["wtowers_fp32_batched_compensated"] = ic
         res["wtowers"] = r
         del gb, ib
 

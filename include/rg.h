@@ -92,8 +92,9 @@ typedef struct rg_grid_params {
                                batched W-towers: band height in rows (4, 8 or 16 (16 in
                                FP32 only), 0 = 8) */
     int32_t vis_per_item;   /* batched visibilities per work item (0 = default) */
-    int32_t compensated;    /* batched FP32: 0 plain FP32 tile sums (default), 1 two-sum
-                               compensated tile sums (worth it only into an RG_C128 grid) */
+    int32_t compensated;    /* batched FP32: 0 plain FP32 sums (default), 1 two-sum
+                               compensated sums: W-projection tile sums (worth it only into
+                               an RG_C128 grid), W-towers band lanes' register sums */
     int32_t floor_batch;    /* batched W-towers: (tower, floor) tasks per batch (0 = as many as
                                fit 16 GiB of subgrids) */
     int32_t fft_radix;      /* batched W-towers: radix of the column pass's FFT stages (8 or

@@ -644,7 +644,7 @@ __global__ __launch_bounds__(256) void k_window_len(const uint32_t *__restrict__
 // then inverse FFT'd in LDS.  A key of one item stores the band straight into
 // the task's subgrid; a key split over several items stores each part in a
 // spill slot and k_band_combine sums the parts in item order.
-template <typename T, int S, int H, int LOGN, int GRP>
+template <typename T, int S, int H, int LOGN, int GRP, bool COMP>
 __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
     const BandRec<T> *__restrict__ recs, const Item *__restrict__ items,
     const uint32_t *__restrict__ item_start, BatchRef bref, int batch,
@@ -743,6 +743,34 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
         // cells are logical byte offsets; pad16 places them in LDS
         int cur = -1, carry = -1; // carry: the kept cell of the unpadded form
         T sr = T(0), si = T(0);
+        // COMP: two-sum (Knuth) compensation of the register sums; cr, ci hold
+        // the rounding errors, folded into sr, si before a sum leaves the lane
+        T cr = T(0), ci = T(0);
+        auto two_sum = [](T &s, T &c, T x) {
+            const T t = s + x;
+            const T bp = t - s;
+            c += (s - (t - bp)) + (x - bp);
+            s = t;
+        };
+        auto mac = [&](const T2 &k, const T2 &v) { // sr + i si += conj(k) v
+            if constexpr (COMP) {
+                two_sum(sr, cr, fma(k.x, v.x, k.y * v.y));
+                two_sum(si, ci, fma(k.x, v.y, -(k.y * v.x)));
+            } else {
+                sr = fma(k.x, v.x, sr);
+                sr = fma(k.y, v.y, sr);
+                si = fma(k.x, v.y, si);
+                si = fma(-k.y, v.x, si);
+            }
+        };
+        auto fold = [&]() {
+            if constexpr (COMP) {
+                sr += cr;
+                si += ci;
+                cr = T(0);
+                ci = T(0);
+            }
+        };
         constexpr int LOGZ = sizeof(T2) == 8 ? 3 : 4;
         // The groups take contiguous, column-sorted runs of each record stage,
         // so the cells of two groups can meet only within one footprint width
@@ -759,6 +787,7 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
         // a cell of the row's gap: flushes into it are discarded
         const int gap_cell = cell_row + (n + BAND_LEAD) * z;
         auto flush = [&]() {
+            fold();
             T *c = reinterpret_cast<T *>(band_b + cur + ((cur >> (LOGZ + 4)) << LOGZ));
             const int col = cur - cell_row;
             if (!REV && GRP > 1 && (col < safe_a || col >= safe_b)) {
@@ -843,17 +872,14 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
             auto update = [&](const Bt &bt) {
                 if (__all(bt.cell[U - 1] == cur)) {
     #pragma unroll
-                    for (int q = 0; q < U; ++q) {
-                        sr = fma(bt.k[q].x, bt.v[q].x, sr);
-                        sr = fma(bt.k[q].y, bt.v[q].y, sr);
-                        si = fma(bt.k[q].x, bt.v[q].y, si);
-                        si = fma(-bt.k[q].y, bt.v[q].x, si);
-                    }
+                    for (int q = 0; q < U; ++q)
+                        mac(bt.k[q], bt.v[q]);
                     return;
                 }
     #pragma unroll
                 for (int q = 0; q < U; ++q) {
                     if (bt.cell[q] != cur) {
+                        fold();
                         if (REV) {
                             T2 o = pre;
                             o.x += sr;
@@ -869,11 +895,7 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
                         sr = T(0);
                         si = T(0);
                     }
-                    // conj(k) * v
-                    sr = fma(bt.k[q].x, bt.v[q].x, sr);
-                    sr = fma(bt.k[q].y, bt.v[q].y, sr);
-                    si = fma(bt.k[q].x, bt.v[q].y, si);
-                    si = fma(-bt.k[q].y, bt.v[q].x, si);
+                    mac(bt.k[q], bt.v[q]);
                 }
             };
             pipelined_records<U, Bt>((uint32_t)nbu, fetch, update);
@@ -891,6 +913,7 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
                 si = T(0);
             } else if (GRP > 1) {
                 // the next stage's runs start elsewhere: nothing stays in flight
+                fold();
                 if (cur >= 0 && (sr != T(0) || si != T(0)))
                     flush();
                 cur = -1;
@@ -899,6 +922,7 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
                 si = T(0);
             }
         }
+        fold();
         if (GRP == 1 && cur >= 0 && (sr != T(0) || si != T(0)))
             flush();
 
@@ -1319,21 +1343,35 @@ struct BandLaunch {
     unsigned long long *stats;
     uint32_t blocks; // workgroups (each loops over the batch's items)
     rg_context *timer_ctx;
+    bool comp;       // FP32: two-sum compensated register sums
 };
+
+template <typename T, int S, int H, int LOGN, bool COMP>
+static int launch_band_items(const BandLaunch &a, hipStream_t s)
+{
+    using T2 = typename Vec2<T>::type;
+    constexpr int GRP = S * H >= 256 ? 1 : 256 / (S * H);
+    KernelTimer kt(a.timer_ctx, RG_TIMER_TOWER_TILES, s);
+    hipLaunchKernelGGL((k_tower_band_items<T, S, H, LOGN, GRP, COMP>), dim3(a.blocks),
+                       dim3(S * H * GRP), 0, s, (const BandRec<T> *)a.recs, a.items,
+                       a.item_start, a.bref, a.batch, a.rank_of_slot, (const T2 *)a.kern,
+                       a.kern_bytes, (const T2 *)a.tw, (T2 *)a.G, (T2 *)a.spill, a.spill_cap,
+                       a.spill_ctr, a.item_spill, a.stats);
+    RG_HIP(hipGetLastError());
+    return RG_OK;
+}
 
 template <typename T, int S, int H, int LOGN>
 static int launch_bands_h(const BandLaunch &a, hipStream_t s)
 {
     using T2 = typename Vec2<T>::type;
-    constexpr int GRP = S * H >= 256 ? 1 : 256 / (S * H);
-    {
-        KernelTimer kt(a.timer_ctx, RG_TIMER_TOWER_TILES, s);
-        hipLaunchKernelGGL((k_tower_band_items<T, S, H, LOGN, GRP>), dim3(a.blocks),
-                           dim3(S * H * GRP), 0, s, (const BandRec<T> *)a.recs, a.items,
-                           a.item_start, a.bref, a.batch, a.rank_of_slot, (const T2 *)a.kern,
-                           a.kern_bytes, (const T2 *)a.tw, (T2 *)a.G, (T2 *)a.spill, a.spill_cap,
-                           a.spill_ctr, a.item_spill, a.stats);
-        RG_HIP(hipGetLastError());
+    if constexpr (sizeof(T) == 4) {
+        if (a.comp)
+            RG_TRY((launch_band_items<T, S, H, LOGN, true>(a, s)));
+        else
+            RG_TRY((launch_band_items<T, S, H, LOGN, false>(a, s)));
+    } else {
+        RG_TRY((launch_band_items<T, S, H, LOGN, false>(a, s)));
     }
     KernelTimer kt(a.timer_ctx, RG_TIMER_TOWER_COMBINE, s);
     hipLaunchKernelGGL((k_band_combine<T2, H, LOGN>), dim3(a.blocks), dim3(256), 0, s, a.items,
@@ -1878,6 +1916,7 @@ static int wtowers_batched_t(rg_context *ctx, const rg_grid_params *p, const rg_
 
     BandLaunch bl;
     bl.timer_ctx = ctx;
+    bl.comp = p->compensated != 0;
     bl.recs = sorted;
     bl.items = items_w;
     bl.item_start = item_start_w;

@@ -54,13 +54,16 @@ def test_batched_fp64_matches_oracle(ctx, case, floor_batch, band, radix):
     assert st.n_gridded == obs.n_vis and st.n_dropped == 0
 
 
-@pytest.mark.parametrize("band,per_item", [(8, 0), (8, 37), (16, 0), (16, 37), (4, 0), (4, 37)])
-def test_batched_fp32_within_tolerance(ctx, case, band, per_item):
+@pytest.mark.parametrize("band,per_item,comp", [(8, 0, 0), (8, 37, 0), (16, 0, 0), (16, 37, 0),
+                                               (4, 0, 0), (4, 37, 0), (8, 37, 1), (16, 0, 1),
+                                               (4, 37, 1)])
+def test_batched_fp32_within_tolerance(ctx, case, band, per_item, comp):
     """FP32 bands of 8 / 16 / 4 rows; per_item 37 splits most band keys over
-    several work items (spilled parts summed by the combine pass)."""
+    several work items (spilled parts summed by the combine pass); comp: two-sum
+    compensated register sums."""
     obs, vis, table, ref = case
     g, st = run(ctx, obs, vis, table, rgamd.RG_MODE_BATCHED, dt=torch.complex64, tile=band,
-                vis_per_item=per_item)
+                vis_per_item=per_item, compensated=comp)
     img, img_ref = orc.grid_to_image(g), orc.grid_to_image(ref)
     assert rel_max(img, img_ref) < 1e-5 and rel_rms(img, img_ref) < 1e-5
     assert st.n_gridded == obs.n_vis

@@ -84,19 +84,20 @@ def test_batched_fp64(ctx, c2, band, per_item, floor_batch, radix):
     assert st.n_tasks == len(tasks) and st.n_gridded == obs.n_vis and st.n_dropped == 0
 
 
-@pytest.mark.parametrize("band,per_item,floor_batch,radix", [
-    (8, 0, 0, 0), (16, 0, 0, 0), (4, 0, 0, 0), (8, 37, 0, 0), (16, 37, 250, 0), (4, 53, 97, 0),
-    (16, 0, 0, 4)])
-def test_batched_fp32(ctx, c2, band, per_item, floor_batch, radix):
+@pytest.mark.parametrize("band,per_item,floor_batch,radix,comp", [
+    (8, 0, 0, 0, 0), (16, 0, 0, 0, 0), (4, 0, 0, 0, 0), (8, 37, 0, 0, 0), (16, 37, 250, 0, 0),
+    (4, 53, 97, 0, 0), (16, 0, 0, 4, 0), (8, 0, 0, 0, 1), (16, 37, 250, 0, 1), (4, 53, 97, 0, 1)])
+def test_batched_fp32(ctx, c2, band, per_item, floor_batch, radix, comp):
     """FP32 bands of 8 (default), 16 and 4 rows: the image within 1e-5 of the
-    FP64 oracle's (max and RMS relative), also with split keys, batches and
-    radix-4 column FFTs."""
+    FP64 oracle's (max and RMS relative), also with split keys, batches,
+    radix-4 column FFTs and two-sum compensated register sums."""
     obs, vis, table, _, img_ref, tasks = c2
     g, st = grid(ctx, obs, vis, table, rgamd.RG_MODE_BATCHED, torch.complex64, tile=band,
-                 vis_per_item=per_item, floor_batch=floor_batch, fft_radix=radix)
+                 vis_per_item=per_item, floor_batch=floor_batch, fft_radix=radix,
+                 compensated=comp)
     img = orc.grid_to_image(g)
-    assert rel_max(img, img_ref) < 1e-5, (band, per_item, floor_batch)
-    assert rel_rms(img, img_ref) < 1e-5, (band, per_item, floor_batch)
+    assert rel_max(img, img_ref) < 1e-5, (band, per_item, floor_batch, comp)
+    assert rel_rms(img, img_ref) < 1e-5, (band, per_item, floor_batch, comp)
     assert st.n_tasks == len(tasks) and st.n_gridded == obs.n_vis
 
 
