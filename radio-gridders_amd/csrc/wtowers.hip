@@ -652,7 +652,7 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
     const typename Vec2<T>::type *__restrict__ kern, uint32_t kern_bytes,
     const typename Vec2<T>::type *__restrict__ twg, typename Vec2<T>::type *G,
     typename Vec2<T>::type *spill, uint32_t spill_cap, uint32_t *spill_ctr, uint32_t *item_spill,
-    unsigned long long *stats)
+    uint32_t *spill_item, unsigned long long *stats)
 {
     using T2 = typename Vec2<T>::type;
     constexpr int n = 1 << LOGN;
@@ -937,6 +937,8 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
             if (threadIdx.x == 0) {
                 const uint32_t sl = atomicAdd(spill_ctr, 1u);
                 item_spill[it - ib] = sl;
+                if (sl < spill_cap)
+                    spill_item[sl] = it - ib;
                 s_slot = sl;
             }
             __syncthreads();
@@ -987,9 +989,10 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
 }
 
 // Sums the spilled parts of every key split over several items (in item
-// order) into the task's subgrid band.  Launched with the grid of
-// k_tower_band_items (one block per item of the batch); the P blocks of a key
-// split into P parts each sum a 1 / P share of the band's elements.
+// order) into the task's subgrid band.  Block s takes the item that holds
+// spill slot s (the band pass records it; slots past the batch's count exit),
+// so the P items of a key split into P parts each sum a 1 / P share of the
+// band's elements.
 template <typename T2, int H, int LOGN>
 __global__ __launch_bounds__(256) void k_band_combine(const Item *__restrict__ items,
                                                       const uint32_t *__restrict__ item_start,
@@ -997,18 +1000,19 @@ __global__ __launch_bounds__(256) void k_band_combine(const Item *__restrict__ i
                                                       const int32_t *__restrict__ rank_of_slot,
                                                       T2 *G, const T2 *__restrict__ spill,
                                                       uint32_t spill_cap,
-                                                      const uint32_t *__restrict__ item_spill)
+                                                      const uint32_t *__restrict__ item_spill,
+                                                      const uint32_t *__restrict__ spill_ctr,
+                                                      const uint32_t *__restrict__ spill_item)
 {
     constexpr int n = 1 << LOGN;
     constexpr int NB = n / H;
     int t0, t1;
     if (!batch_tasks(bref, batch, t0, t1))
         return;
-    const uint32_t ib = item_start[(uint32_t)bref.task_slot[t0] * (uint32_t)NB];
-    const uint32_t ie = item_start[(uint32_t)(bref.task_slot[t1 - 1] + 1) * (uint32_t)NB];
-    const uint32_t it = ib + blockIdx.x;
-    if (it >= ie)
+    if (blockIdx.x >= min(*spill_ctr, spill_cap))
         return;
+    const uint32_t ib = item_start[(uint32_t)bref.task_slot[t0] * (uint32_t)NB];
+    const uint32_t it = ib + spill_item[blockIdx.x];
     {
         const uint32_t key = items[it].key;
         const uint32_t k0 = item_start[key], k1 = item_start[key + 1];
@@ -1339,7 +1343,7 @@ struct BandLaunch {
     uint32_t kern_bytes; // kernel table size (< 2^31)
     void *G, *spill;
     uint32_t spill_cap;
-    uint32_t *spill_ctr, *item_spill;
+    uint32_t *spill_ctr, *item_spill, *spill_item;
     unsigned long long *stats;
     uint32_t blocks; // workgroups (each loops over the batch's items)
     rg_context *timer_ctx;
@@ -1356,7 +1360,7 @@ static int launch_band_items(const BandLaunch &a, hipStream_t s)
                        dim3(S * H * GRP), 0, s, (const BandRec<T> *)a.recs, a.items,
                        a.item_start, a.bref, a.batch, a.rank_of_slot, (const T2 *)a.kern,
                        a.kern_bytes, (const T2 *)a.tw, (T2 *)a.G, (T2 *)a.spill, a.spill_cap,
-                       a.spill_ctr, a.item_spill, a.stats);
+                       a.spill_ctr, a.item_spill, a.spill_item, a.stats);
     RG_HIP(hipGetLastError());
     return RG_OK;
 }
@@ -1374,9 +1378,11 @@ static int launch_bands_h(const BandLaunch &a, hipStream_t s)
         RG_TRY((launch_band_items<T, S, H, LOGN, false>(a, s)));
     }
     KernelTimer kt(a.timer_ctx, RG_TIMER_TOWER_COMBINE, s);
-    hipLaunchKernelGGL((k_band_combine<T2, H, LOGN>), dim3(a.blocks), dim3(256), 0, s, a.items,
-                       a.item_start, a.bref, a.batch, a.rank_of_slot, (T2 *)a.G,
-                       (const T2 *)a.spill, a.spill_cap, (const uint32_t *)a.item_spill);
+    hipLaunchKernelGGL((k_band_combine<T2, H, LOGN>), dim3(std::min(a.spill_cap, a.blocks)),
+                       dim3(256), 0, s, a.items, a.item_start, a.bref, a.batch, a.rank_of_slot,
+                       (T2 *)a.G, (const T2 *)a.spill, a.spill_cap,
+                       (const uint32_t *)a.item_spill, (const uint32_t *)a.spill_ctr,
+                       (const uint32_t *)a.spill_item);
     RG_HIP(hipGetLastError());
     return RG_OK;
 }
@@ -1890,7 +1896,9 @@ static int wtowers_batched_t(rg_context *ctx, const rg_grid_params *p, const rg_
     const size_t sp_bytes = band_bytes * max_spill;
     const size_t is_bytes = (sizeof(uint32_t) * (size_t)max_items_w + 255) / 256 * 256;
     const size_t ctr_bytes = (sizeof(uint32_t) * nbatch + 255) / 256 * 256;
-    RG_TRY(ctx->scratch[7].ensure(g_bytes + p_bytes + sp_bytes + is_bytes + ctr_bytes));
+    const size_t si_bytes = sizeof(uint32_t) * (size_t)max_spill;
+    RG_TRY(ctx->scratch[7].ensure(g_bytes + p_bytes + sp_bytes + is_bytes + ctr_bytes +
+                                  si_bytes));
     char *base7 = ctx->scratch[7].as<char>();
     T2 *G = reinterpret_cast<T2 *>(base7);
     T2 *Pp = reinterpret_cast<T2 *>(base7 + g_bytes);
@@ -1898,6 +1906,8 @@ static int wtowers_batched_t(rg_context *ctx, const rg_grid_params *p, const rg_
     uint32_t *item_spill = reinterpret_cast<uint32_t *>(base7 + g_bytes + p_bytes + sp_bytes);
     uint32_t *spill_ctr =
         reinterpret_cast<uint32_t *>(base7 + g_bytes + p_bytes + sp_bytes + is_bytes);
+    uint32_t *spill_item = reinterpret_cast<uint32_t *>(base7 + g_bytes + p_bytes + sp_bytes +
+                                                        is_bytes + ctr_bytes);
     RG_TRY(zero_fill(spill_ctr, sizeof(uint32_t) * nbatch, s));
 
     // every work item's records sorted by footprint column into the expanded
@@ -1950,6 +1960,7 @@ static int wtowers_batched_t(rg_context *ctx, const rg_grid_params *p, const rg_
     bl.spill = spill;
     bl.spill_cap = (uint32_t)std::min<uint64_t>(max_spill, 0xffffffffu);
     bl.item_spill = item_spill;
+    bl.spill_item = spill_item;
     bl.stats = stats;
     // one band-pass workgroup per work item (the plan's item capacity; a
     // batch with more items reports an overrun)
