@@ -163,7 +163,7 @@ int bin_plan(int64_t n, int64_t nkeys, uint32_t per_item, int64_t max_counters, 
     if (per_item == 0)
         return fail(RG_ERR_PARAM, "work items need at least one record");
     pl.nblocks_scan = (nkeys + SCAN_TILE - 1) / SCAN_TILE;
-    if (pl.nblocks_scan > 1024 * 64)
+    if (pl.nblocks_scan > 1024 * 256)
         return fail(RG_ERR_PARAM, "key space too large (%lld)", (long long)nkeys);
     pl.max_items = nkeys + n / per_item + 1;
     // counter replicas: up to 16, at most max_counters counters in all
@@ -190,7 +190,7 @@ int items_from_lengths(rg_context *ctx, const uint32_t *len, int64_t nkeys, uint
                        uint32_t *prefix, uint32_t *item_start, Item *items, hipStream_t s)
 {
     const int64_t nblocks_scan = (nkeys + SCAN_TILE - 1) / SCAN_TILE;
-    if (nkeys <= 0 || nblocks_scan > 1024 * 64)
+    if (nkeys <= 0 || nblocks_scan > 1024 * 256)
         return fail(RG_ERR_PARAM, "invalid key space %lld", (long long)nkeys);
     RG_TRY(ensure_counters(ctx));
     RG_TRY(ctx->scratch[6].ensure(sizeof(uint2) * (nblocks_scan + 1)));

@@ -25,7 +25,8 @@
 namespace rg {
 
 constexpr int SCAN_BLOCK = 256;
-constexpr int SCAN_PER_THREAD = 16;
+constexpr int SCAN_PER_THREAD = 4; // 1024 keys per block: W-towers key spaces of ~1 M keys
+                                   // give ~1000 blocks, enough to fill the CUs
 constexpr int SCAN_TILE = SCAN_BLOCK * SCAN_PER_THREAD; // keys per scan block
 
 struct Item {

@@ -611,23 +611,34 @@ __global__ __launch_bounds__(256) void k_sort_items(const Rec<T> *__restrict__ i
 // that can reach rows [H b, H b + H)); they are contiguous in the binned
 // record array.  Per band key: window length and start.
 // Band windows with records are counted (the bands the pass writes).
+// (grid-stride; the windows with records are counted per workgroup and added
+// with one atomic: one per wavefront on the same word serialised)
 __global__ __launch_bounds__(256) void k_window_len(const uint32_t *__restrict__ key_start,
                                                     int64_t nkeys, int nb, int mk, uint32_t *len,
                                                     uint32_t *wstart, unsigned long long *stats)
 {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t l = 0;
-    if (k < nkeys) {
+    __shared__ uint32_t s_any[4];
+    uint32_t nany = 0;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nkeys;
+         k += (int64_t)gridDim.x * blockDim.x) {
         const int b = (int)(k % nb);
         const int64_t k0 = k - min(b, mk - 1);
         const uint32_t w0 = key_start[k0];
-        l = key_start[k + 1] - w0;
+        const uint32_t l = key_start[k + 1] - w0;
         len[k] = l;
         wstart[k] = w0;
+        nany += l > 0 ? 1u : 0u;
     }
-    const unsigned long long any = __ballot(l > 0);
-    if (__lane_id() == 0 && any)
-        atomicAdd(&stats[CNT_BANDS], (unsigned long long)__popcll(any));
+    for (int o = 32; o > 0; o >>= 1)
+        nany += (uint32_t)__shfl_xor((int)nany, o);
+    if (__lane_id() == 0)
+        s_any[threadIdx.x / 64] = nany;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t tot = s_any[0] + s_any[1] + s_any[2] + s_any[3];
+        if (tot)
+            atomicAdd(&stats[CNT_BANDS], (unsigned long long)tot);
+    }
 }
 
 // ---- batched band pass -------------------------------------------------------------------
@@ -638,9 +649,10 @@ __global__ __launch_bounds__(256) void k_window_len(const uint32_t *__restrict__
 // (myU, myV) owns band row myV and the columns congruent to myU modulo S: a
 // footprint covering that row has exactly one cell there.  The lane keeps the
 // register sum of its current cell and adds it into the LDS band when the
-// cell changes (lanes of different groups own the same cells: see flush).  Band rows lie BS = n + S + 8 elements apart
-// after a lead of S elements, so a footprint column outside [0, n) lands in
-// the gap between rows and is dropped without a test.  The band's rows are
+// cell changes (lanes of different groups own the same cells: see flush).
+// Band rows lie LS = n + BAND_LEAD elements apart after a lead of BAND_LEAD,
+// so a footprint column outside [0, n) lands in the gap between rows and is
+// dropped without a test.  The band's rows are
 // then inverse FFT'd in LDS.  A key of one item stores the band straight into
 // the task's subgrid; a key split over several items stores each part in a
 // spill slot and k_band_combine sums the parts in item order.
@@ -1829,7 +1841,8 @@ static int wtowers_batched_t(rg_context *ctx, const rg_grid_params *p, const rg_
     {
         KernelTimer kt(ctx, RG_TIMER_BINNING, s);
         uint32_t *len = ctx->scratch[0].as<uint32_t>();
-        hipLaunchKernelGGL(k_window_len, dim3((unsigned)((nkeys + 255) / 256)), dim3(256), 0, s,
+        const unsigned wblocks = (unsigned)std::min<int64_t>((nkeys + 255) / 256, 1024);
+        hipLaunchKernelGGL(k_window_len, dim3(wblocks), dim3(256), 0, s,
                            (const uint32_t *)br.vis_start, nkeys, NB, g.mk, len, d_wstart, stats);
         RG_HIP(hipGetLastError());
         RG_TRY(items_from_lengths(ctx, len, nkeys, K, prefix, item_start_w, items_w, s));
