@@ -338,7 +338,8 @@ def main():
 
     if args.c5:
         try:
-            res["stress"] = bench_stress(ctx, K, W)
+            res["stress"] = bench_stress(ctx, K, W, tune_cache=args.tune_cache,
+                                         retune=args.retune, no_tune=args.no_tune)
         except Exception as e:  # reported, never hides the headline
             log(f"[bench] stress workload failed: {e!r}")
             res["stress"] = {"error": repr(e)}
@@ -797,7 +798,7 @@ def bench_channels(ctx, steps, warmup, tune_cache=None, retune=False):
     return out
 
 
-def bench_stress(ctx, steps, warmup, npix=4096):
+def bench_stress(ctx, steps, warmup, npix=4096, tune_cache=None, retune=False, no_tune=False):
     """Stress workload: 512 stations x 128 dumps x 16 channels (~260 M complex64
     visibilities), phase centre at zenith angle 60 deg, 4096^2 master grid
     (theta 0.2), W-towers subgrid 256, margin 16, winc 10.  FP32 batched timed;
@@ -807,6 +808,7 @@ def bench_stress(ctx, steps, warmup, npix=4096):
     import torch
 
     import rgamd
+    import rgtune
     import synth
 
     t0 = time.time()
@@ -834,6 +836,13 @@ def bench_stress(ctx, steps, warmup, npix=4096):
     log(f"[bench] stress: {nv} visibilities, {N}^2 ({time.time() - t0:.1f}s)")
     p = rgamd.params(N, obs.theta, rgamd.RG_MODE_BATCHED, subgrid=256, margin=16, winc=10.0)
     g32 = torch.empty((N, N), dtype=torch.complex128, device="cuda")
+    tuning = None
+    if not no_tune:
+        p, info = rgtune.tuned_params(ctx, p, fv64, k64, g32, cache=tune_cache, retune=retune,
+                                      reps=2)
+        tuning = {"source": info["source"], "chosen": info["chosen"], "ms": info["ms"],
+                  "candidates": info["table"]}
+        log(f"[bench] stress FP32 launch parameters ({info['source']}): {info['chosen']}")
     for _ in range(warmup):
         ctx.grid(p, fv64, k64, g32)
     torch.cuda.synchronize()
@@ -865,6 +874,7 @@ def bench_stress(ctx, steps, warmup, npix=4096):
     dft = dft_errors(ctx, fv64, N, obs.lam, obs.theta, 16, {"wtowers_fp32_batched": i32}, npix,
                      inner_only=True)
     out = {"n_vis": nv, "grid": N, "subgrid": 256, "tasks": st.n_tasks, "towers": st.n_towers,
+           "tuning": tuning,
            "fp32_batched": {"ms_per_step": ms, "mvis_s": nv / ms / 1e3},
            "fp64_batched": {"ms_per_step": ms64, "mvis_s": nv / ms64 / 1e3},
            "error_fp32_vs_fp64": err, "error_vs_dft": dft}
