@@ -958,7 +958,12 @@ def roofline(kind, head, st, nv, S, N, n, H, launch, table_bytes):
                 "bytes_per_launch": byt, "launches_per_step": launches, "avg_launch_ms": avg_ms,
                 "measured_GBps": (traffic / (avg_ms * 1e-3) / 1e9) if traffic else None,
                 "valu": {"achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                         "frac": tf / FP32_PEAK_TFLOPS, "flop_per_launch": f[part] / launches},
+                         "frac": tf / FP32_PEAK_TFLOPS, "flop_per_launch": f[part] / launches,
+                         # the published peak counts packed FP32 (two FMAs per lane);
+                         # the build keeps packed FP32 instructions off, so its FP32
+                         # issue limit is half of it
+                         "peak_unpacked": FP32_PEAK_TFLOPS / 2,
+                         "frac_unpacked": tf / (FP32_PEAK_TFLOPS / 2)},
                 "effective": {"GBps": eff / launches / (avg_ms * 1e-3) / 1e9,
                               "bytes_per_launch": eff / launches,
                               "bytes_per_unit": {"visibility": 24 + 8 + S * S * 8}
