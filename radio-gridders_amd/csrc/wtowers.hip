@@ -115,13 +115,17 @@ struct TowerKeyer {
 
 // min / max floor index over visibilities that fall in the tower set; four
 // visibilities per thread and pass (their loads issued together), wavefront
-// then workgroup reduction, one atomic pair per workgroup
+// then workgroup reduction into part[workgroup]; k_range_finish reduces the
+// parts (atomics on the same two words serialise: ~10 ns each)
 constexpr int RANGE_VPT = 4;
+constexpr int RANGE_BLOCKS = 2048;
+// the range (2 ints) and the workgroups' parts in one buffer
+constexpr size_t RANGE_BYTES = 256 + sizeof(int2) * RANGE_BLOCKS;
 
 __global__ __launch_bounds__(256) void k_floor_range(const double *u, const double *v,
                                                      const double *w, int64_t n, double theta,
                                                      int N, int chunk, int cc, double winc,
-                                                     int *range)
+                                                     int2 *part)
 {
     __shared__ int smin[4], smax[4];
     int lo = INT_MAX, hi = INT_MIN;
@@ -162,17 +166,37 @@ __global__ __launch_bounds__(256) void k_floor_range(const double *u, const doub
             lo = min(lo, smin[k]);
             hi = max(hi, smax[k]);
         }
-        if (lo != INT_MAX)
-            atomicMin(&range[0], lo);
-        if (hi != INT_MIN)
-            atomicMax(&range[1], hi);
+        part[blockIdx.x] = make_int2(lo, hi);
     }
 }
 
-__global__ void k_range_init(int *range)
+// range[0..1] = min / max over the nparts parts (INT_MAX / INT_MIN if none)
+__global__ __launch_bounds__(1024) void k_range_finish(const int2 *part, int nparts, int *range)
 {
-    range[0] = INT_MAX;
-    range[1] = INT_MIN;
+    __shared__ int smin[16], smax[16];
+    int lo = INT_MAX, hi = INT_MIN;
+    for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
+        const int2 v = part[k];
+        lo = min(lo, v.x);
+        hi = max(hi, v.y);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, __shfl_xor(lo, o));
+        hi = max(hi, __shfl_xor(hi, o));
+    }
+    if (__lane_id() == 0) {
+        smin[threadIdx.x / 64] = lo;
+        smax[threadIdx.x / 64] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < (int)blockDim.x / 64; ++k) {
+            lo = min(lo, smin[k]);
+            hi = max(hi, smax[k]);
+        }
+        range[0] = lo;
+        range[1] = hi;
+    }
 }
 
 // per task slot: visibility count and first work item (plain mode)
@@ -1536,15 +1560,16 @@ static TowerKeyer wt_keyer(const rg_grid_params *p, const rg_flat_vis *vis, cons
 static int floor_range(const rg_grid_params *p, const rg_flat_vis *vis, const WtGeom &g,
                        int *d_range, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_range_init, dim3(1), dim3(1), 0, s, d_range);
-    // few workgroups: each ends with one atomicMin / atomicMax on the same two
-    // words, which serialise
+    // d_range: RANGE_BYTES (the range, then the workgroups' parts)
+    int2 *part = reinterpret_cast<int2 *>(reinterpret_cast<char *>(d_range) + 256);
     int64_t blocks = (vis->n + 256 * RANGE_VPT - 1) / (256 * RANGE_VPT);
-    if (blocks > 2048)
-        blocks = 2048;
+    if (blocks > RANGE_BLOCKS)
+        blocks = RANGE_BLOCKS;
     if (blocks > 0)
         hipLaunchKernelGGL(k_floor_range, dim3((unsigned)blocks), dim3(256), 0, s, vis->u, vis->v,
-                           vis->w, vis->n, p->theta, g.N, g.chunk, g.cc, p->winc, d_range);
+                           vis->w, vis->n, p->theta, g.N, g.chunk, g.cc, p->winc, part);
+    hipLaunchKernelGGL(k_range_finish, dim3(1), dim3(1024), 0, s, (const int2 *)part,
+                       (int)blocks, d_range);
     RG_HIP(hipGetLastError());
     return RG_OK;
 }
@@ -1581,7 +1606,7 @@ static int wtowers_plain(rg_context *ctx, const rg_grid_params *p, const rg_flat
     RG_TRY(zero_fill(grid, sizeof(double2) * (size_t)N * N, s));
     if (vis->n == 0)
         return RG_OK;
-    RG_TRY(ctx->scratch[8].ensure(64));
+    RG_TRY(ctx->scratch[8].ensure(RANGE_BYTES));
     int *d_range = ctx->scratch[8].as<int>();
     RG_TRY(floor_range(p, vis, g, d_range, s));
     int wp_min = 0, nwp = 0;
@@ -1736,7 +1761,7 @@ static int wtowers_batched_t(rg_context *ctx, const rg_grid_params *p, const rg_
         return RG_OK;
 
     // floor range (device); the plan fixes the floors per tower slot
-    RG_TRY(ctx->scratch[8].ensure(256));
+    RG_TRY(ctx->scratch[8].ensure(RANGE_BYTES));
     int *d_range = ctx->scratch[8].as<int>();
     RG_TRY(floor_range(p, vis, g, d_range, s));
     int nwp_cap;
