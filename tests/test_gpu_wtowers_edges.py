@@ -30,7 +30,7 @@ def params(obs):
 
 def oracle(obs, u, v, w, vis, table):
     ref, _, _ = orc.grid_wtowers(obs.grid_size, obs.theta, u, v, w, vis, table, -5.0, 1.0, SUB,
-                                   MARGIN, WINC, nthreads=8)
+                                 MARGIN, WINC, nthreads=8)
     return ref
 
 
