@@ -8,7 +8,9 @@
 // Thread (myU, myV) handles, for every record, the footprint cell congruent to
 // (myU, myV) modulo S (Romein); it keeps a register sum while that cell stays
 // the same and adds it into LDS when the cell changes.  Two threads never
-// share a cell, so the LDS updates need no atomics.  At the end the tile is
+// share a cell, so the LDS updates need no atomics, and a thread reads a
+// cell's LDS value when it starts summing into it: leaving the cell is then a
+// store, and the read's latency is hidden behind the records in between.  At the end the tile is
 // added to the target with global atomics (neighbouring tiles and items of
 // the same tile overlap).  Storing per-item partials and summing them per cell
 // instead measured slower at the C2 shape: the atomics overlap with the
@@ -47,14 +49,13 @@ __global__ __launch_bounds__(S *S) void k_tiles(const Rec<T> *__restrict__ recs,
                                                 const typename Vec2<T>::type *__restrict__ kern,
                                                 TileSink sink)
 {
+    using T2 = typename Vec2<T>::type;
     constexpr int NT = S * S;
     constexpr int RT = TILE + S - 1;
     constexpr int CELLS = RT * RT;
     constexpr int LC = COMP ? CELLS : 1;
-    __shared__ T s_re[CELLS];
-    __shared__ T s_im[CELLS];
-    __shared__ T s_lre[LC];
-    __shared__ T s_lim[LC];
+    __shared__ T2 s_cell[CELLS]; // (re, im) of every cell of the tile and its halo
+    __shared__ T2 s_lo[LC];      // COMP: the two-sum low parts
     __shared__ Rec<T> s_rec[NT];
 
     const uint32_t end = stats ? (uint32_t)stats[CNT_ITEMS] : item_end;
@@ -64,17 +65,32 @@ __global__ __launch_bounds__(S *S) void k_tiles(const Rec<T> *__restrict__ recs,
     const Item item = items[it];
 
     for (int c = threadIdx.x; c < CELLS; c += NT) {
-        s_re[c] = T(0);
-        s_im[c] = T(0);
+        s_cell[c].x = T(0);
+        s_cell[c].y = T(0);
         if (COMP) {
-            s_lre[c] = T(0);
-            s_lim[c] = T(0);
+            s_lo[c].x = T(0);
+            s_lo[c].y = T(0);
         }
     }
 
     const int myU = threadIdx.x % S, myV = threadIdx.x / S;
     int cur = -1;
     T sr = T(0), si = T(0);
+    T2 pre; // !COMP: the LDS value of cell cur when the thread started it
+    pre.x = T(0);
+    pre.y = T(0);
+    // adds the register sum into cell cur
+    auto flush = [&]() {
+        if (COMP) {
+            two_sum_into(s_cell[cur].x, s_lo[cur].x, sr);
+            two_sum_into(s_cell[cur].y, s_lo[cur].y, si);
+        } else {
+            T2 o = pre;
+            o.x += sr;
+            o.y += si;
+            s_cell[cur] = o;
+        }
+    };
 
     // Records are staged NT at a time and consumed U at a time: the U tap
     // loads are issued together, then the U updates run in stream order.  A
@@ -113,16 +129,11 @@ __global__ __launch_bounds__(S *S) void k_tiles(const Rec<T> *__restrict__ recs,
 #pragma unroll
             for (int q = 0; q < U; ++q) {
                 if (bt.cell[q] != cur) {
-                    if (cur >= 0) {
-                        if (COMP) {
-                            two_sum_into(s_re[cur], s_lre[cur], sr);
-                            two_sum_into(s_im[cur], s_lim[cur], si);
-                        } else {
-                            s_re[cur] += sr;
-                            s_im[cur] += si;
-                        }
-                    }
+                    if (cur >= 0)
+                        flush();
                     cur = bt.cell[q];
+                    if (!COMP)
+                        pre = s_cell[cur];
                     sr = T(0);
                     si = T(0);
                 }
@@ -135,15 +146,8 @@ __global__ __launch_bounds__(S *S) void k_tiles(const Rec<T> *__restrict__ recs,
         };
         pipelined_records<U, B>(nlocU, fetch, update);
     }
-    if (cur >= 0) {
-        if (COMP) {
-            two_sum_into(s_re[cur], s_lre[cur], sr);
-            two_sum_into(s_im[cur], s_lim[cur], si);
-        } else {
-            s_re[cur] += sr;
-            s_im[cur] += si;
-        }
-    }
+    if (cur >= 0)
+        flush();
     __syncthreads();
 
     const uint32_t group = item.key / (uint32_t)sink.ntiles;
@@ -159,10 +163,10 @@ __global__ __launch_bounds__(S *S) void k_tiles(const Rec<T> *__restrict__ recs,
         const int ry = c / RT, rx = c - ry * RT;
         const int gx = ox + rx, gy = oy + ry;
         if ((unsigned)gx < (unsigned)G && (unsigned)gy < (unsigned)G) {
-            GT re = (GT)s_re[c], im = (GT)s_im[c];
+            GT re = (GT)s_cell[c].x, im = (GT)s_cell[c].y;
             if (COMP) {
-                re += (GT)s_lre[c];
-                im += (GT)s_lim[c];
+                re += (GT)s_lo[c].x;
+                im += (GT)s_lo[c].y;
             }
             if (re != GT(0) || im != GT(0)) {
                 GT *dst = g + 2 * ((size_t)gy * G + gx);
