@@ -406,8 +406,11 @@ __device__ inline void fft_block(T2 *lds, const T2 *tw)
 // words (a plain table read at r k conflicts for even r).
 __host__ __device__ constexpr int pad16(int x) { return x + (x >> 4); }
 
-template <typename T2, int LOGN, int R, bool INV>
-__device__ inline void fft_stage_band(T2 *row, const T2 *tws, int logns, int lt, bool active)
+// One stage: reads src (padded, or with SRC_PAD false the row's logical
+// layout), writes row (padded); every read precedes every write (barrier).
+template <typename T2, int LOGN, int R, bool INV, bool SRC_PAD = true>
+__device__ inline void fft_stage_band(const T2 *src, T2 *row, const T2 *tws, int logns, int lt,
+                                      bool active)
 {
     constexpr int n = 1 << LOGN;
     constexpr int TPL = n / 8;
@@ -421,7 +424,7 @@ __device__ inline void fft_stage_band(T2 *row, const T2 *tws, int logns, int lt,
             const int j = lt + q * TPL;
 #pragma unroll
             for (int r = 0; r < R; ++r)
-                v[q * R + r] = row[pad16(j + r * NBUT)];
+                v[q * R + r] = src[SRC_PAD ? pad16(j + r * NBUT) : j + r * NBUT];
         }
     }
     __syncthreads();
@@ -451,8 +454,12 @@ __device__ inline void fft_stage_band(T2 *row, const T2 *tws, int logns, int lt,
 }
 
 // Transforms rows 0..nl-1 of a padded band with NT threads; row l begins at
-// band + pad16(LEAD + l LS) (LEAD and LS multiples of 16).
-template <typename T2, int LOGN, int NT, int LEAD, int LS, bool INV>
+// band + pad16(LEAD + l LS) (LEAD and LS multiples of 16).  UNPADDED_IN: the
+// rows arrive in the logical layout (row l at band + LEAD + l LS) and the
+// first stage moves them to the padded one.  Padded row l lies above logical
+// row l and below padded row l + 1 (LS > n), so with the row groups taken
+// last first a stage never overwrites logical rows still to be read.
+template <typename T2, int LOGN, int NT, int LEAD, int LS, bool INV, bool UNPADDED_IN = false>
 __device__ inline void fft_band_rows(T2 *band, const T2 *tws, int nl)
 {
     static_assert(LEAD % 16 == 0 && LS % 16 == 0, "rows start on pad boundaries");
@@ -461,7 +468,9 @@ __device__ inline void fft_band_rows(T2 *band, const T2 *tws, int nl)
     static_assert(TPL <= NT, "line longer than 8 x threads");
     constexpr int B = NT / TPL;
     const int lt = threadIdx.x % TPL;
-    for (int l0 = 0; l0 < nl; l0 += B) {
+    const int ng = (nl + B - 1) / B;
+    for (int g = 0; g < ng; ++g) {
+        const int l0 = (UNPADDED_IN ? ng - 1 - g : g) * B;
         const int l = l0 + (int)threadIdx.x / TPL;
         const bool active = (int)threadIdx.x < B * TPL && l < nl;
         T2 *row = band + pad16(LEAD + (active ? l : 0) * LS);
@@ -471,14 +480,19 @@ __device__ inline void fft_band_rows(T2 *band, const T2 *tws, int nl)
             if (logns >= LOGN)
                 break;
             const int rem = LOGN - logns;
-            if (rem >= 3) {
-                fft_stage_band<T2, LOGN, 8, INV>(row, tws, logns, lt, active);
+            if (UNPADDED_IN && st == 0) {
+                static_assert(LOGN >= 3, "the first stage is radix 8");
+                const T2 *src = band + LEAD + (active ? l : 0) * LS;
+                fft_stage_band<T2, LOGN, 8, INV, false>(src, row, tws, logns, lt, active);
+                logns += 3;
+            } else if (rem >= 3) {
+                fft_stage_band<T2, LOGN, 8, INV>(row, row, tws, logns, lt, active);
                 logns += 3;
             } else if (rem == 2) {
-                fft_stage_band<T2, LOGN, 4, INV>(row, tws, logns, lt, active);
+                fft_stage_band<T2, LOGN, 4, INV>(row, row, tws, logns, lt, active);
                 logns += 2;
             } else {
-                fft_stage_band<T2, LOGN, 2, INV>(row, tws, logns, lt, active);
+                fft_stage_band<T2, LOGN, 2, INV>(row, row, tws, logns, lt, active);
                 logns += 1;
             }
         }

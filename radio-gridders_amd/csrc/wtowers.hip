@@ -449,9 +449,9 @@ __device__ inline size_t gcb_index(int row, int col, int n)
 // ---- band records ------------------------------------------------------------------------
 // A visibility as band b of its task sees it, with everything a band lane
 // needs precomputed (z = sizeof(T2), the element size):
-//   xy = gz | cy0 << 16, gz = (gx + BAND_LEAD) z the footprint's first column
-//        in the band's logical row layout, cy0 = H b - gy (signed) the
-//        footprint row that band row 0 meets;
+//   xy = gz | cy0 << 16 (FP64; FP32: xy = gz), gz = (gx + BAND_LEAD) z the
+//        footprint's first column in the band's logical row layout, cy0 =
+//        H b - gy (signed) the footprint row that band row 0 meets;
 //   kb, byte offset of that footprint row's first tap:
 //   - FP32 (padded table: H zero rows, then every S x S sub-kernel followed
 //     by H zero rows, PS = S + H): kb = ((H + kidx PS + min(cy0, S)) S) z;
@@ -477,8 +477,10 @@ __host__ __device__ constexpr int padded_rows(int S, int H) { return S + H; }
 
 // Band rows in LDS: logical index BAND_LEAD + l LS + c for row l, column c
 // (LS = n + BAND_LEAD), so the BAND_LEAD >= S columns left of a row and
-// right of it fall in gaps between rows; stored with one pad element per 16
-// (pad16, fft_core.hpp).
+// right of it fall in gaps between rows.  The scatter works in this layout;
+// the row IFFT's first stage reads it and writes the rows with one pad
+// element per 16 (pad16, fft_core.hpp), the layout of its later stages and
+// of the copy-out.
 constexpr int BAND_LEAD = 16;
 
 template <typename T>
@@ -487,7 +489,10 @@ __device__ inline BandRec<T> band_record(const Rec<T> &r, int S, int H, int Hb, 
     const int gx = (int)(r.pos & 0xffffu) - S, gy = (int)(r.pos >> 16) - S;
     const int cy0 = Hb - gy; // >= 1 - H for every record of band Hb / H's window
     BandRec<T> o;
-    o.xy = (uint32_t)((gx + BAND_LEAD) * z) | ((uint32_t)cy0 << 16);
+    if constexpr (padded_taps<T>())
+        o.xy = (uint32_t)((gx + BAND_LEAD) * z); // the band lanes need no cy0
+    else
+        o.xy = (uint32_t)((gx + BAND_LEAD) * z) | ((uint32_t)cy0 << 16);
     if constexpr (padded_taps<T>())
         o.kb = ((H + (int32_t)r.kidx * padded_rows(S, H) + min(cy0, S)) * S) * z;
     else
@@ -701,7 +706,8 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
     constexpr int RS = 256;
     constexpr int RG = RS / GRP;
     static_assert(RS % GRP == 0 && RS <= NT, "record staging");
-    // row l of the band at logical [BAND_LEAD + l LS, + n) (pad16 in LDS); the
+    // row l of the band at logical [BAND_LEAD + l LS, + n) (pad16 after the
+    // first IFFT stage, so the buffer holds the padded extent); the
     // S - 1 columns either side of a row fall in the gaps
     static_assert(BAND_LEAD >= S, "gaps hold the footprint columns outside a row");
     constexpr int LS = n + BAND_LEAD;
@@ -776,7 +782,7 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
         // per lane, in bytes: column residue, tap row, cell row
         const int uz = myU * z, tap_row = myV * S * z, cell_row = myV * LS * z;
         char *band_b = reinterpret_cast<char *>(band);
-        // cells are logical byte offsets; pad16 places them in LDS
+        // cells are byte offsets in the logical (unpadded) layout
         int cur = -1, carry = -1; // carry: the kept cell of the unpadded form
         T sr = T(0), si = T(0);
         // COMP: two-sum (Knuth) compensation of the register sums; cr, ci hold
@@ -807,7 +813,6 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
                 ci = T(0);
             }
         };
-        constexpr int LOGZ = sizeof(T2) == 8 ? 3 : 4;
         // The groups take contiguous, column-sorted runs of each record stage,
         // so the cells of two groups can meet only within one footprint width
         // of the column where their runs meet.  With two groups the second
@@ -824,7 +829,7 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
         const int gap_cell = cell_row + (n + BAND_LEAD) * z;
         auto flush = [&]() {
             fold();
-            T *c = reinterpret_cast<T *>(band_b + cur + ((cur >> (LOGZ + 4)) << LOGZ));
+            T *c = reinterpret_cast<T *>(band_b + cur);
             const int col = cur - cell_row;
             if (!REV && GRP > 1 && (col < safe_a || col >= safe_b)) {
                 atomicAdd(c, sr);
@@ -844,7 +849,7 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
         pre.x = T(0);
         pre.y = T(0);
         auto cellp = [&](int cl) {
-            return reinterpret_cast<T2 *>(band_b + cl + ((cl >> (LOGZ + 4)) << LOGZ));
+            return reinterpret_cast<T2 *>(band_b + cl);
         };
         if (REV) {
             cur = gap_cell;
@@ -885,7 +890,7 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
     #pragma unroll
                 for (int q = 0; q < U; ++q) {
                     const BandRec<T> r = gr[q0 + q];
-                    const int gz = (int)(r.xy & 0xffffu);
+                    const int gz = padded_taps<T>() ? (int)r.xy : (int)(r.xy & 0xffffu);
                     const int cxz = (uz - gz) & ((S - 1) * z);
                     bt.v[q] = r.vis;
                     if constexpr (padded_taps<T>()) {
@@ -993,7 +998,7 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
             if (threadIdx.x + q * NT < n)
                 tw[threadIdx.x + q * NT] = twr[q];
         __syncthreads();
-        fft_band_rows<T2, LOGN, NT, BAND_LEAD, LS, true>(band, tw, H);
+        fft_band_rows<T2, LOGN, NT, BAND_LEAD, LS, true, true>(band, tw, H);
         // element (l, c) of the band
         auto at = [&](int l, int c) { return band[pad16(BAND_LEAD + l * LS) + pad16(c)]; };
         // two elements per thread and store (columns c, c + 1 with c even are
