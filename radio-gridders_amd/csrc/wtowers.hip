@@ -744,12 +744,27 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
         const int b = (int)(item.key - tslot * (uint32_t)NB);
         const int j = rank_of_slot[tslot] - t0;
 
-        // thread t < RS stages record t of each RS-record stage; the next
-        // stage's record is loaded while the current stage is processed
+        // U records at a time: U tap loads in flight, then the U updates in order.
+        constexpr int U = 8;
+        static_assert(RG % U == 0, "record batch must be a multiple of the unroll");
+        // records of the stage at base taken by the first group: with two
+        // groups, half of them (a multiple of U), so that a short stage
+        // keeps both groups busy
+        auto split = [&](uint32_t base) {
+            const int nl = (int)min((uint32_t)RS, item.count - base);
+            return GRP == 2 ? min(RG, ((nl + 1) / 2 + U - 1) / U * U) : RG;
+        };
+        // thread t < RG stages record t of each RS-record stage if t < split,
+        // thread RG + u record split + u; the next stage's record is loaded
+        // while the current stage is processed
         auto stage_rec = [&](uint32_t base) {
             BandRec<T> r;
-            if (base + threadIdx.x < item.count) {
-                r = recs[item.start + base + threadIdx.x];
+            const int t = (int)threadIdx.x, h = split(base);
+            const uint32_t nl = min((uint32_t)RS, item.count - base);
+            const uint32_t i = GRP != 2 ? (uint32_t)t : t < RG ? (t < h ? (uint32_t)t : nl)
+                                                            : (uint32_t)(h + t - RG);
+            if (i < nl) {
+                r = recs[item.start + base + i];
             } else {
                 // padding, no visibility: padded table: column n, after every
                 // footprint column (its cells lie in the row gaps); otherwise a
@@ -855,9 +870,6 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
             cur = gap_cell;
             carry = gap_cell;
         }
-        // U records at a time: U tap loads in flight, then the U updates in order.
-        constexpr int U = 8;
-        static_assert(RG % U == 0, "record batch must be a multiple of the unroll");
         for (uint32_t base = 0; base < item.count; base += RS) {
             const uint32_t nloc = min((uint32_t)RS, item.count - base);
             __syncthreads();
@@ -869,7 +881,9 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
                     next = stage_rec(base + RS);
             }
             __syncthreads();
-            const int ng = min(RG, max(0, (int)nloc - grp * RG));
+            const int h = split(base);
+            const int ng = REV ? (grp == 0 ? h : max(0, (int)nloc - h))
+                               : min(RG, max(0, (int)nloc - grp * RG));
             const int nbu = (ng + U - 1) / U * U; // records walked, padding included
             const BandRec<T> *gr = s_rec + grp * RG;
             if (REV && grp == 1)
