@@ -537,6 +537,36 @@ __device__ inline double2 tap_load(__amdgpu_buffer_rsrc_t r, uint32_t off, doubl
     return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
 }
 constexpr uint32_t TAP_NONE = 0x80000000u;
+
+// Band records in and subgrid bands out are touched once: nontemporal loads
+// and stores keep them from evicting the kernel table, which every band lane
+// re-reads, from L2.
+template <typename R> __device__ inline R load_once(const R *p)
+{
+    typedef unsigned v4u __attribute__((ext_vector_type(4)));
+    static_assert(sizeof(R) % 16 == 0, "16-byte words");
+    struct W {
+        v4u w[sizeof(R) / 16];
+    } q;
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(R) / 16); ++i)
+        q.w[i] = __builtin_nontemporal_load(reinterpret_cast<const v4u *>(p) + i);
+    return __builtin_bit_cast(R, q);
+}
+// elements a, b at d[0], d[1]
+__device__ inline void store_once2(float2 *d, float2 a, float2 b)
+{
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const v4f o = {a.x, a.y, b.x, b.y};
+    __builtin_nontemporal_store(o, reinterpret_cast<v4f *>(d));
+}
+__device__ inline void store_once2(double2 *d, double2 a, double2 b)
+{
+    typedef double v2d __attribute__((ext_vector_type(2)));
+    const v2d o0 = {a.x, a.y}, o1 = {b.x, b.y};
+    __builtin_nontemporal_store(o0, reinterpret_cast<v2d *>(d));
+    __builtin_nontemporal_store(o1, reinterpret_cast<v2d *>(d + 1));
+}
 // word 3 of a raw (unswizzled, stride 0) buffer resource on gfx9
 constexpr int BUF_RSRC_W3 = 0x00020000;
 
@@ -764,7 +794,7 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
             const uint32_t i = GRP != 2 ? (uint32_t)t : t < RG ? (t < h ? (uint32_t)t : nl)
                                                             : (uint32_t)(h + t - RG);
             if (i < nl) {
-                r = recs[item.start + base + i];
+                r = load_once(recs + item.start + base + i);
             } else {
                 // padding, no visibility: padded table: column n, after every
                 // footprint column (its cells lie in the row gaps); otherwise a
@@ -1017,15 +1047,7 @@ __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
         auto at = [&](int l, int c) { return band[pad16(BAND_LEAD + l * LS) + pad16(c)]; };
         // two elements per thread and store (columns c, c + 1 with c even are
         // adjacent in LDS and in both destination layouts)
-        auto put2 = [&](T2 *d, int l, int c) {
-            if constexpr (sizeof(T2) == 8) {
-                const T2 a0 = at(l, c), a1 = at(l, c + 1);
-                *reinterpret_cast<float4 *>(d) = make_float4(a0.x, a0.y, a1.x, a1.y);
-            } else {
-                d[0] = at(l, c);
-                d[1] = at(l, c + 1);
-            }
-        };
+        auto put2 = [&](T2 *d, int l, int c) { store_once2(d, at(l, c), at(l, c + 1)); };
         if (whole) {
             // rows [H b, H b + H) of column block cb are H * GCB contiguous elements
             T2 *d0 = dst + (size_t)H * b * GCB;
