@@ -704,17 +704,18 @@ __global__ __launch_bounds__(256) void k_window_len(const uint32_t *__restrict__
 // One workgroup per work item: up to K band records of the window of band b of
 // a task -- records of keys b - mk + 1 .. b, sorted by footprint column --
 // accumulated into subgrid rows [H b, H b + H).  The workgroup is GRP groups
-// of S x H lanes; the groups take alternate runs of the item's records.  Lane
-// (myU, myV) owns band row myV and the columns congruent to myU modulo S: a
-// footprint covering that row has exactly one cell there.  The lane keeps the
-// register sum of its current cell and adds it into the LDS band when the
-// cell changes (lanes of different groups own the same cells: see flush).
-// Band rows lie LS = n + BAND_LEAD elements apart after a lead of BAND_LEAD,
-// so a footprint column outside [0, n) lands in the gap between rows and is
-// dropped without a test.  The band's rows are
-// then inverse FFT'd in LDS.  A key of one item stores the band straight into
-// the task's subgrid; a key split over several items stores each part in a
-// spill slot and k_band_combine sums the parts in item order.
+// of S x H lanes; the records are staged RS at a time and every stage is
+// split into one contiguous run per group.  Lane (myU, myV) owns band row
+// myV and the columns congruent to myU modulo S: a footprint covering that
+// row has exactly one cell there.  The lane keeps the register sum of its
+// current cell and adds it into the LDS band when the cell changes (lanes of
+// different groups own the same cells: see flush).  Band rows lie
+// LS = n + BAND_LEAD elements apart after a lead of BAND_LEAD, so a footprint
+// column outside [0, n) lands in the gap between rows and is dropped without
+// a test.  The band's rows are then inverse FFT'd in LDS.  A key of one item
+// stores the band straight into the task's subgrid; a key split over several
+// items stores each part in a spill slot and k_band_combine sums the parts in
+// item order.
 template <typename T, int S, int H, int LOGN, int GRP, bool COMP>
 __global__ __launch_bounds__(S *H *GRP) void k_tower_band_items(
     const BandRec<T> *__restrict__ recs, const Item *__restrict__ items,
