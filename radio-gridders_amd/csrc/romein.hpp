@@ -11,6 +11,20 @@
 
 namespace rg {
 
+// Kernel taps through a buffer resource: offsets beyond the table read zeros
+// (TAP_NONE for a lane whose row the footprint misses, unpadded table).
+__device__ inline float2 tap_load(__amdgpu_buffer_rsrc_t r, uint32_t off, float2 *)
+{
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ inline double2 tap_load(__amdgpu_buffer_rsrc_t r, uint32_t off, double2 *)
+{
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+constexpr uint32_t TAP_NONE = 0x80000000u;
+// word 3 of a raw (unswizzled, stride 0) buffer resource on gfx9
+constexpr int BUF_RSRC_W3 = 0x00020000;
+
 template <typename T, int U> struct RecBatch {
     int cell[U];
     typename Vec2<T>::type k[U];

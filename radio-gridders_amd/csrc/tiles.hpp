@@ -47,12 +47,17 @@ __global__ __launch_bounds__(S *S) void k_tiles(const Rec<T> *__restrict__ recs,
                                                 uint32_t item_end,
                                                 const unsigned long long *__restrict__ stats,
                                                 const typename Vec2<T>::type *__restrict__ kern,
-                                                TileSink sink)
+                                                uint32_t kern_bytes, TileSink sink)
 {
     using T2 = typename Vec2<T>::type;
     constexpr int NT = S * S;
     constexpr int RT = TILE + S - 1;
-    constexpr int CELLS = RT * RT;
+    // cell rows RTP apart (RT rounded up to S): a record's (lx, ly) residues
+    // follow from its byte offsets
+    constexpr int RTP = (RT + S - 1) / S * S;
+    constexpr int CELLS = RT * RTP;
+    constexpr int z = (int)sizeof(T2);
+    static_assert(CELLS * z <= 65536, "cell byte offsets fit 16 bits");
     constexpr int LC = COMP ? CELLS : 1;
     __shared__ T2 s_cell[CELLS]; // (re, im) of every cell of the tile and its halo
     __shared__ T2 s_lo[LC];      // COMP: the two-sum low parts
@@ -74,6 +79,13 @@ __global__ __launch_bounds__(S *S) void k_tiles(const Rec<T> *__restrict__ recs,
     }
 
     const int myU = threadIdx.x % S, myV = threadIdx.x / S;
+    const int myUz = myU * z, myVSz = myV * S * z;
+    char *cell_b = reinterpret_cast<char *>(s_cell);
+    char *lo_b = reinterpret_cast<char *>(s_lo);
+    auto cellp = [&](int c) { return reinterpret_cast<T2 *>(cell_b + c); };
+    auto lop = [&](int c) { return reinterpret_cast<T2 *>(lo_b + c); };
+    const __amdgpu_buffer_rsrc_t ktab =
+        __builtin_amdgcn_make_buffer_rsrc((void *)kern, 0, (int)kern_bytes, BUF_RSRC_W3);
     int cur = -1;
     T sr = T(0), si = T(0);
     T2 pre; // !COMP: the LDS value of cell cur when the thread started it
@@ -82,13 +94,13 @@ __global__ __launch_bounds__(S *S) void k_tiles(const Rec<T> *__restrict__ recs,
     // adds the register sum into cell cur
     auto flush = [&]() {
         if (COMP) {
-            two_sum_into(s_cell[cur].x, s_lo[cur].x, sr);
-            two_sum_into(s_cell[cur].y, s_lo[cur].y, si);
+            two_sum_into(cellp(cur)->x, lop(cur)->x, sr);
+            two_sum_into(cellp(cur)->y, lop(cur)->y, si);
         } else {
             T2 o = pre;
             o.x += sr;
             o.y += si;
-            s_cell[cur] = o;
+            *cellp(cur) = o;
         }
     };
 
@@ -116,13 +128,13 @@ __global__ __launch_bounds__(S *S) void k_tiles(const Rec<T> *__restrict__ recs,
         auto fetch = [&](uint32_t j, B &bt) {
 #pragma unroll
             for (int q = 0; q < U; ++q) {
+                // pos = (ly RTP + lx) z | (ly S z) << 16, kidx = sub-kernel byte offset
                 const Rec<T> r = s_rec[j + q];
-                const int lx = (int)(r.pos & 0xffffu), ly = (int)(r.pos >> 16);
-                const int cx = (myU - lx) & (S - 1);
-                const int cy = (myV - ly) & (S - 1);
-                bt.cell[q] = (ly + cy) * RT + (lx + cx);
+                const int cxz = (myUz - (int)r.pos) & ((S - 1) * z);
+                const int cyS = (myVSz - (int)(r.pos >> 16)) & ((S - 1) * S * z);
+                bt.cell[q] = (int)(r.pos & 0xffffu) + cyS * (RTP / S) + cxz;
                 bt.v[q] = r.vis;
-                bt.k[q] = kern[r.kidx * (uint32_t)NT + (uint32_t)(cy * S + cx)];
+                bt.k[q] = tap_load(ktab, r.kidx + (uint32_t)(cyS + cxz), (T2 *)nullptr);
             }
         };
         auto update = [&](const B &bt) {
@@ -133,7 +145,7 @@ __global__ __launch_bounds__(S *S) void k_tiles(const Rec<T> *__restrict__ recs,
                         flush();
                     cur = bt.cell[q];
                     if (!COMP)
-                        pre = s_cell[cur];
+                        pre = *cellp(cur);
                     sr = T(0);
                     si = T(0);
                 }
@@ -160,9 +172,9 @@ __global__ __launch_bounds__(S *S) void k_tiles(const Rec<T> *__restrict__ recs,
     const int G = sink.gsize;
     GT *g = reinterpret_cast<GT *>(sink.base) + (size_t)slot * G * G * 2;
     for (int c = threadIdx.x; c < CELLS; c += NT) {
-        const int ry = c / RT, rx = c - ry * RT;
+        const int ry = c / RTP, rx = c - ry * RTP;
         const int gx = ox + rx, gy = oy + ry;
-        if ((unsigned)gx < (unsigned)G && (unsigned)gy < (unsigned)G) {
+        if (rx < RT && (unsigned)gx < (unsigned)G && (unsigned)gy < (unsigned)G) {
             GT re = (GT)s_cell[c].x, im = (GT)s_cell[c].y;
             if (COMP) {
                 re += (GT)s_lo[c].x;

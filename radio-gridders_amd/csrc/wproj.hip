@@ -142,6 +142,7 @@ struct WprojKeyer {
     int N, S, R, P;
     double theta, w_min, w_step;
     int tile, ntx, ntiles;
+    int zb, rtp; // bytes per kernel tap; cell row stride of a tile (tile + S - 1 rounded up to S)
     const int64_t *chan_off;
     int nchan;
 
@@ -171,8 +172,10 @@ struct WprojKeyer {
         const int lx = gx + S - tx * tile, ly = gy + S - ty * tile;
         const int c = nchan > 1 ? chan_of(i) : 0;
         key = (uint32_t)(c * ntiles + ty * ntx + tx);
-        pos = (uint32_t)lx | ((uint32_t)ly << 16);
-        kidx = (uint32_t)((p * R + yf) * R + xf);
+        // byte offsets as the tile kernel uses them: cell (lx, ly) of the tile's
+        // LDS cells and row ly of a sub-kernel; first tap of the sub-kernel
+        pos = (uint32_t)((ly * rtp + lx) * zb) | ((uint32_t)(ly * S * zb) << 16);
+        kidx = (uint32_t)((p * R + yf) * R + xf) * (uint32_t)(S * S * zb);
         return VIS_OK;
     }
 
@@ -204,11 +207,15 @@ static int launch_tiles(const BinResult &br, rg_context *timer_ctx, unsigned lon
     sink.group_map = nullptr;
     sink.group_offset = 0;
     KernelTimer kt(timer_ctx, RG_TIMER_WPROJ_TILES, s);
+    const uint64_t kbytes = (uint64_t)wk->plane_count * wk->oversampling * wk->oversampling * S *
+                            S * sizeof(typename Vec2<T>::type);
+    if (kbytes > 0x7fffffffull)
+        return fail(RG_ERR_UNSUPPORTED, "batched W-projection kernel tables above 2 GiB");
     if (br.max_items > 0)
         hipLaunchKernelGGL((k_tiles<T, S, TILE, COMP, GT>), dim3((unsigned)br.max_items),
                            dim3(S * S), 0, s, (const Rec<T> *)br.recs, (const Item *)br.items, 0u,
                            0u, (const unsigned long long *)stats,
-                           (const typename Vec2<T>::type *)wk->data, sink);
+                           (const typename Vec2<T>::type *)wk->data, (uint32_t)kbytes, sink);
     RG_HIP(hipGetLastError());
     return RG_OK;
 }
@@ -273,7 +280,7 @@ static int wproj_shape(const rg_grid_params *p, const rg_wkern *wk, WprojShape &
 }
 
 static WprojKeyer wproj_keyer(const rg_grid_params *p, const rg_flat_vis *vis, const rg_wkern *wk,
-                              const WprojShape &sh, const int64_t *d_off, int n_chan)
+                              const WprojShape &sh, const int64_t *d_off, int n_chan, int zb)
 {
     WprojKeyer kf;
     kf.u = vis->u;
@@ -291,6 +298,8 @@ static WprojKeyer wproj_keyer(const rg_grid_params *p, const rg_flat_vis *vis, c
     kf.tile = sh.tile;
     kf.ntx = sh.ntx;
     kf.ntiles = sh.ntiles;
+    kf.zb = zb;
+    kf.rtp = (sh.tile + wk->size - 1 + wk->size - 1) / wk->size * wk->size;
     kf.chan_off = d_off;
     kf.nchan = n_chan;
     return kf;
@@ -308,7 +317,8 @@ static int wproj_batched_t(rg_context *ctx, const rg_grid_params *p, const rg_fl
     const size_t gbytes = (size_t)(grid_type == RG_C128 ? 16 : 8) * N * N * n_chan;
     RG_TRY(zero_fill(grid, gbytes, s));
 
-    const WprojKeyer kf = wproj_keyer(p, vis, wk, sh, d_off, n_chan);
+    const WprojKeyer kf =
+        wproj_keyer(p, vis, wk, sh, d_off, n_chan, (int)sizeof(typename Vec2<T>::type));
     BinResult br;
     RG_TRY((bin_visibilities<WprojKeyer, T>(ctx, kf, vis->n, (int64_t)sh.ntiles * n_chan,
                                             sh.per_item, s, br)));
@@ -401,7 +411,8 @@ static int wproj_noalloc_t(const rg_grid_params *p, const rg_flat_vis *vis, cons
         RG_TRY(zero_fill(grid, (size_t)(grid_type == RG_C128 ? 16 : 8) * N * N, s));
     if (vis->n == 0)
         return RG_OK;
-    const WprojKeyer kf = wproj_keyer(p, vis, wk, sh, nullptr, 1);
+    const WprojKeyer kf =
+        wproj_keyer(p, vis, wk, sh, nullptr, 1, (int)sizeof(typename Vec2<T>::type));
     BinResult br;
     RG_TRY((bin_run<WprojKeyer, T>(kf, vis->n, sh.ntiles, sh.per_item, pl, b, nullptr, s, br)));
     return dispatch_tiles<T>(br, nullptr, b.stats, wk, grid, grid_type, p->compensated != 0, N,

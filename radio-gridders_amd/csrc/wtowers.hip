@@ -526,17 +526,6 @@ __global__ __launch_bounds__(256) void k_pad_kernels(const T2 *__restrict__ src,
     }
 }
 
-// Kernel taps through a buffer resource: offsets beyond the table read zeros
-// (TAP_NONE for a lane whose row the footprint misses, unpadded table).
-__device__ inline float2 tap_load(__amdgpu_buffer_rsrc_t r, uint32_t off, float2 *)
-{
-    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
-}
-__device__ inline double2 tap_load(__amdgpu_buffer_rsrc_t r, uint32_t off, double2 *)
-{
-    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
-}
-constexpr uint32_t TAP_NONE = 0x80000000u;
 
 // Band records in and subgrid bands out are touched once: nontemporal loads
 // and stores keep them from evicting the kernel table, which every band lane
@@ -567,8 +556,6 @@ __device__ inline void store_once2(double2 *d, double2 a, double2 b)
     __builtin_nontemporal_store(o0, reinterpret_cast<v2d *>(d));
     __builtin_nontemporal_store(o1, reinterpret_cast<v2d *>(d + 1));
 }
-// word 3 of a raw (unswizzled, stride 0) buffer resource on gfx9
-constexpr int BUF_RSRC_W3 = 0x00020000;
 
 // ---- record order inside band items -------------------------------------------------------
 // A band lane's cell depends only on the footprint column gx (its row is fixed),
