@@ -4,7 +4,9 @@
 // One workgroup of S*S threads takes one work item: up to K sorted records of
 // one key.  A key is (group, tile): the group selects the target array (a
 // channel grid) and the tile a TILE x TILE block of footprint top-left
-// positions, whose footprints cover (TILE + S - 1)^2 cells held in LDS.
+// positions, whose footprints cover (TILE + S - 1)^2 cells held in LDS (rows
+// TILE + S - 1 rounded up to S apart, so that the binning can hand each record
+// its cell and sub-kernel as byte offsets).
 // Thread (myU, myV) handles, for every record, the footprint cell congruent to
 // (myU, myV) modulo S (Romein); it keeps a register sum while that cell stays
 // the same and adds it into LDS when the cell changes.  Two threads never
